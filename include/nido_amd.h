@@ -1,0 +1,201 @@
+/*
+ * nido_amd.h -- C ABI of the MI355X-native batched Louvain library
+ * (libnido_amd.so).
+ *
+ * The reference (sg0/nido) is a monolithic executable with no FFI; the seam
+ * this ABI replaces is the GraphGPU public method set that LouvainGPU::run
+ * calls (graph_gpu.hpp:168-247, louvain_gpu.cpp:28-124) together with the
+ * host Graph container (graph.hpp:14-102).  Each entry point below names the
+ * reference method it stands in for.
+ *
+ * Conventions
+ *   - Every call returns NIDO_OK (0) or a negative NIDO_ERR_* code; nothing
+ *     calls exit().  nido_last_error() gives a message for the calling
+ *     thread's last failure.
+ *   - Ids are int64, weights FP64 (types.hpp:13-22).  CSR = offsets[nv+1],
+ *     cols[ne], weights[ne] (graph.hpp:23-26).
+ *   - The library owns every device buffer; callers own the host arrays
+ *     they pass in or receive into.  One handle per host thread.
+ *   - All sums of input weights are exact (Q64.64 fixed point) and gain
+ *     ties go to the smallest community id, so results are bit-identical
+ *     for every launch configuration and every run.
+ */
+#ifndef NIDO_AMD_H_
+#define NIDO_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NIDO_OK            0
+#define NIDO_ERR_INVALID  -1   /* bad argument or malformed graph */
+#define NIDO_ERR_HIP      -2   /* HIP runtime error */
+#define NIDO_ERR_NOMEM    -3   /* device or host allocation failed */
+#define NIDO_ERR_CAPACITY -4   /* graph exceeds a supported size */
+#define NIDO_ERR_STATE    -5   /* call out of order */
+#define NIDO_ERR_IO       -6   /* file could not be read or written */
+
+#define NIDO_ABI_VERSION   1
+
+#define NIDO_MAX_PHASES  200   /* MAX_PHASES, types.hpp:24-26 */
+#define NIDO_DEFAULT_ITERATIONS 500    /* types.hpp:28-30 */
+#define NIDO_DEFAULT_BATCHES      2    /* types.hpp:32-34 */
+#define NIDO_DEFAULT_THRESHOLD 1.0e-6  /* types.hpp:36-38 */
+
+typedef struct nido_hgraph nido_hgraph;   /* host CSR graph (Graph) */
+typedef struct nido_graph nido_graph;     /* device-resident state (GraphGPU) */
+
+/* Per-phase record of a run (the PRINT output of louvain_gpu.cpp:43-118). */
+typedef struct {
+    int64_t nv;          /* vertices of the phase graph */
+    int64_t ne;          /* arcs (CSR entries) of the phase graph */
+    int64_t loops;       /* local-moving iterations */
+    double q0;           /* modularity of the singleton partition */
+    double q;            /* modularity at the end of the phase */
+    double t_loop;       /* seconds in the iteration loop ("Time elapse") */
+    double t_agg;        /* seconds in aggregation ("Aggregation time") */
+} nido_phase_stats;
+
+/* Whole-run summary (louvain_gpu.cpp:125-131). */
+typedef struct {
+    int32_t phases;
+    int32_t done;        /* 1 when the run stopped by convergence */
+    int64_t total_loops;
+    double q;            /* final modularity */
+    double t_total;      /* "Total time elapse" */
+    double t_loop;       /* sum of the phases' iteration-loop time */
+    double t_agg;        /* sum of aggregation time */
+} nido_run_stats;
+
+/* Knobs of the degree-binned local-moving kernels (results do not depend
+ * on them; only speed does). */
+typedef struct {
+    int32_t thread_max_deg;   /* rows up to this length: one lane per vertex (<= 8) */
+    int32_t use_graph;        /* capture each iteration in a hipGraph (0/1) */
+    int32_t reserved[6];
+} nido_tuning;
+
+const char* nido_last_error(void);
+int nido_abi_version(void);
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int nido_device_count(int* count);
+
+/* ---- host graph: Graph (graph.hpp:14-102) ------------------------------- */
+
+/* Graph::Graph(binfile), graph.cpp:322-495: header {int64 nv, int64 ne},
+ * (nv+1) int64 offsets, then ne records {int64 tail, double weight}
+ * (types.hpp:48-55).  Offsets are rebased to start at 0 and the arc count is
+ * taken from them (graph.cpp:376-382, 470-472). */
+int nido_hgraph_read(const char* path, nido_hgraph** out);
+/* Writes the same format (the converter in Vite that the README points at). */
+int nido_hgraph_write(const nido_hgraph* g, const char* path);
+/* Copies a caller CSR into a host graph. */
+int nido_hgraph_from_csr(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,
+                         const double* weights, nido_hgraph** out);
+/* get_num_vertices / get_num_edges (graph.hpp:72-73). */
+int nido_hgraph_info(const nido_hgraph* g, int64_t* nv, int64_t* ne);
+/* get_index_ranges / get_edges / get_edge_weights (graph.hpp:76-78): pointers
+ * stay valid until nido_hgraph_destroy. */
+int nido_hgraph_arrays(const nido_hgraph* g, const int64_t** offsets, const int64_t** cols,
+                       const double** weights);
+void nido_hgraph_destroy(nido_hgraph* g);
+
+/* Seeded synthetic inputs.  All graphs are symmetric, without self-loops or
+ * duplicate arcs, rows sorted by neighbour id; one weight per undirected edge
+ * copied to both arcs; weighted=0 gives unit weights, weighted=1 weights in
+ * (0,1] on a 2^-53 grid.  (The reference's own generator is the -r BA graph,
+ * graph.cpp:76-160.) */
+/* Planted partition: vertex v in block v/block_size, pairs u<v taken in
+ * lexicographic order, Bernoulli(p_in) inside a block, Bernoulli(p_out)
+ * across blocks. */
+int nido_gen_sbm(int64_t nv, int64_t block_size, double p_in, double p_out, int weighted,
+                 uint64_t seed, nido_hgraph** out);
+/* R-MAT: edge_factor * 2^scale samples with quadrant probabilities
+ * (a, b, c, 1-a-b-c); self-loops dropped, symmetrised, deduplicated. */
+int nido_gen_rmat(int scale, int64_t edge_factor, double a, double b, double c, int weighted,
+                  uint64_t seed, nido_hgraph** out);
+/* Chung-Lu: expected degrees Pareto(alpha) clipped to [dmin, dmax] and
+ * rescaled to the requested mean; nv*mean/2 endpoint pairs drawn in
+ * proportion to the expected degrees; self-loops dropped, deduplicated. */
+int nido_gen_chung_lu(int64_t nv, double alpha, double dmin, double dmax, double mean,
+                      int weighted, uint64_t seed, nido_hgraph** out);
+
+/* ---- device state: GraphGPU (graph_gpu.hpp:168-247) -------------------- */
+
+/* GraphGPU::GraphGPU(graph, nbatches, ...), graph_gpu.cpp:19-185: uploads the
+ * CSR, computes k_v (sum_vertex_weights), 2m (compute_mass), the maximum
+ * order and the -b batch boundaries (partition_graph_edge_batch).  Offsets
+ * must be non-decreasing, cols in [0, nv), weights finite with |w| < 2^62.
+ * stream: a hipStream_t to run on, or NULL for a stream owned by the
+ * handle.  nv must be below 2^31. */
+int nido_graph_create(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,
+                      const double* weights, int nbatches, void* stream, nido_graph** out);
+/* GraphGPU::~GraphGPU, graph_gpu.cpp:187-221. */
+int nido_graph_destroy(nido_graph* g);
+/* NV_, NE_, maxOrder_, mass_ of the current phase graph. */
+int nido_graph_info(const nido_graph* g, int64_t* nv, int64_t* ne, int64_t* max_order,
+                    double* mass);
+/* vertex_per_batch_ (graph_gpu.cpp:338-339): nbatches+1 entries. */
+int nido_graph_batch_bounds(const nido_graph* g, int64_t* parts);
+int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t);
+int nido_graph_get_tuning(const nido_graph* g, nido_tuning* t);
+/* Blocks until every call queued on the handle's stream has finished. */
+int nido_graph_sync(nido_graph* g);
+
+/* GraphGPU::singleton_partition, graph_gpu.cpp:519-536. */
+int nido_singleton_partition(nido_graph* g);
+/* GraphGPU::compute_modularity, graph_gpu.cpp:862-905 (blocks for the value). */
+int nido_compute_modularity(nido_graph* g, double* q);
+/* GraphGPU::louvain_update(batch, g), graph_gpu.cpp:602-624: decides the
+ * moves of one batch (writes newCommIds_). */
+int nido_louvain_update(nido_graph* g, int batch);
+/* GraphGPU::update_community_weights(batch, g), graph_gpu.cpp:779-791. */
+int nido_update_community_weights(nido_graph* g, int batch);
+/* GraphGPU::update_community_ids(batch, g), graph_gpu.cpp:808-820.  The
+ * device applies the weight update and the swap in one kernel: after
+ * nido_update_community_weights the swap is already done and this call only
+ * checks the order of calls. */
+int nido_update_community_ids(nido_graph* g, int batch);
+/* GraphGPU::restore_community, graph_gpu.cpp:822-842. */
+int nido_restore_community(nido_graph* g);
+/* GraphGPU::aggregation, graph_gpu.cpp:1382-1545: renumbers, updates the
+ * cumulative clustering, and (unless K==N or K==1, then *done=1) rebuilds the
+ * coarse graph on the device and recomputes k, m and the batches. */
+int nido_aggregation(nido_graph* g, int* done);
+
+/* State readers (D2H copies; the arrays are the current phase's). */
+int nido_get_communities(nido_graph* g, int64_t* comm);          /* commIds_ */
+int nido_get_new_communities(nido_graph* g, int64_t* comm);      /* newCommIds_ */
+int nido_get_vertex_weights(nido_graph* g, double* k);           /* vertexWeights_ */
+int nido_get_community_weights(nido_graph* g, double* a);        /* commWeights_ */
+/* Current phase CSR; rows of coarse graphs are in hash order (the results
+ * do not depend on arc order). */
+int nido_get_graph(nido_graph* g, int64_t* offsets, int64_t* cols, double* weights);
+/* Clustering::commIds_ (clustering.cpp:12-21): original vertex -> community
+ * of the last aggregated phase; n0 = original vertex count. */
+int nido_get_clustering(nido_graph* g, int64_t* comm);
+/* GraphGPU::dump_partition / Clustering::dump_partition, clustering.cpp:23-46:
+ * one "%lld\n" line per original vertex. */
+int nido_dump_partition(nido_graph* g, const char* path);
+
+/* ---- driver: LouvainGPU (louvain_gpu.hpp:15-23) ------------------------- */
+
+/* LouvainGPU::run, louvain_gpu.cpp:6-139: phases (<= 200) of iterations
+ * (while tol < dQ and loops < max_loops) of -b batches, modularity
+ * accept/rollback, aggregation while Q - Q0 > tol_phase.  record=1 keeps the
+ * per-phase communities for nido_run_phase_communities.  verbose=1 prints the
+ * reference's PRINT lines to stdout. */
+int nido_louvain_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, int record,
+                     int verbose, nido_run_stats* stats);
+int nido_run_phase(const nido_graph* g, int phase, nido_phase_stats* out);
+/* Modularity after every iteration of a phase (loops entries). */
+int nido_run_iteration_q(const nido_graph* g, int phase, double* q);
+/* Communities at the end of a phase, phase-local ids (record=1 runs). */
+int nido_run_phase_communities(const nido_graph* g, int phase, int64_t* comm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NIDO_AMD_H_ */

@@ -1,0 +1,407 @@
+"""nido_amd -- MI355X-native batched Louvain (local moving + coarsening).
+
+Python view of the C ABI in include/nido_amd.h (libnido_amd.so, built from
+nido_amd/csrc by `make -C nido_amd`).  The classes mirror the reference's
+host API: `Graph` (graph.hpp:14-102), `GraphGPU` (graph_gpu.hpp:168-247) and
+`LouvainGPU` (louvain_gpu.hpp:15-23).  Every call goes to the HIP library;
+there is no CPU fallback, and importing this module fails loudly when the
+library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+__all__ = [
+    "lib", "LIB_PATH", "NidoError", "Graph", "GraphGPU", "LouvainGPU", "PhaseStats", "RunStats",
+    "gen_sbm", "gen_rmat", "gen_chung_lu", "device_count", "batch_bounds_host",
+    "DEFAULT_BATCHES", "DEFAULT_ITERATIONS", "DEFAULT_THRESHOLD", "MAX_PHASES",
+]
+
+DEFAULT_BATCHES = 2          # types.hpp:32-34
+DEFAULT_ITERATIONS = 500     # types.hpp:28-30
+DEFAULT_THRESHOLD = 1.0e-6   # types.hpp:36-38
+MAX_PHASES = 200             # types.hpp:24-26
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnido_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (hipcc, gfx950). "
+        "nido_amd has no CPU fallback.")
+
+lib = C.CDLL(LIB_PATH)
+
+_i64 = C.c_int64
+_pi64 = C.POINTER(C.c_int64)
+_pdbl = C.POINTER(C.c_double)
+
+
+class PhaseStats(C.Structure):
+    _fields_ = [("nv", C.c_int64), ("ne", C.c_int64), ("loops", C.c_int64), ("q0", C.c_double),
+                ("q", C.c_double), ("t_loop", C.c_double), ("t_agg", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RunStats(C.Structure):
+    _fields_ = [("phases", C.c_int32), ("done", C.c_int32), ("total_loops", C.c_int64),
+                ("q", C.c_double), ("t_total", C.c_double), ("t_loop", C.c_double),
+                ("t_agg", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("thread_max_deg", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("nido_last_error", C.c_char_p)
+_sig("nido_abi_version", C.c_int)
+_sig("nido_device_count", C.c_int, C.POINTER(C.c_int))
+_sig("nido_hgraph_read", C.c_int, C.c_char_p, C.POINTER(C.c_void_p))
+_sig("nido_hgraph_write", C.c_int, C.c_void_p, C.c_char_p)
+_sig("nido_hgraph_from_csr", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.POINTER(C.c_void_p))
+_sig("nido_hgraph_info", C.c_int, C.c_void_p, _pi64, _pi64)
+_sig("nido_hgraph_arrays", C.c_int, C.c_void_p, C.POINTER(_pi64), C.POINTER(_pi64), C.POINTER(_pdbl))
+_sig("nido_hgraph_destroy", None, C.c_void_p)
+_sig("nido_gen_sbm", C.c_int, _i64, _i64, C.c_double, C.c_double, C.c_int, C.c_uint64,
+     C.POINTER(C.c_void_p))
+_sig("nido_gen_rmat", C.c_int, C.c_int, _i64, C.c_double, C.c_double, C.c_double, C.c_int,
+     C.c_uint64, C.POINTER(C.c_void_p))
+_sig("nido_gen_chung_lu", C.c_int, _i64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+     C.c_uint64, C.POINTER(C.c_void_p))
+_sig("nido_graph_create", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.c_int, C.c_void_p,
+     C.POINTER(C.c_void_p))
+_sig("nido_graph_destroy", C.c_int, C.c_void_p)
+_sig("nido_graph_info", C.c_int, C.c_void_p, _pi64, _pi64, _pi64, _pdbl)
+_sig("nido_graph_batch_bounds", C.c_int, C.c_void_p, _pi64)
+_sig("nido_graph_set_tuning", C.c_int, C.c_void_p, C.POINTER(Tuning))
+_sig("nido_graph_get_tuning", C.c_int, C.c_void_p, C.POINTER(Tuning))
+_sig("nido_graph_sync", C.c_int, C.c_void_p)
+_sig("nido_singleton_partition", C.c_int, C.c_void_p)
+_sig("nido_compute_modularity", C.c_int, C.c_void_p, _pdbl)
+_sig("nido_louvain_update", C.c_int, C.c_void_p, C.c_int)
+_sig("nido_update_community_weights", C.c_int, C.c_void_p, C.c_int)
+_sig("nido_update_community_ids", C.c_int, C.c_void_p, C.c_int)
+_sig("nido_restore_community", C.c_int, C.c_void_p)
+_sig("nido_aggregation", C.c_int, C.c_void_p, C.POINTER(C.c_int))
+_sig("nido_get_communities", C.c_int, C.c_void_p, _pi64)
+_sig("nido_get_new_communities", C.c_int, C.c_void_p, _pi64)
+_sig("nido_get_vertex_weights", C.c_int, C.c_void_p, _pdbl)
+_sig("nido_get_community_weights", C.c_int, C.c_void_p, _pdbl)
+_sig("nido_get_graph", C.c_int, C.c_void_p, _pi64, _pi64, _pdbl)
+_sig("nido_get_clustering", C.c_int, C.c_void_p, _pi64)
+_sig("nido_dump_partition", C.c_int, C.c_void_p, C.c_char_p)
+_sig("nido_louvain_run", C.c_int, C.c_void_p, _i64, C.c_double, C.c_double, C.c_int, C.c_int,
+     C.POINTER(RunStats))
+_sig("nido_run_phase", C.c_int, C.c_void_p, C.c_int, C.POINTER(PhaseStats))
+_sig("nido_run_iteration_q", C.c_int, C.c_void_p, C.c_int, _pdbl)
+_sig("nido_run_phase_communities", C.c_int, C.c_void_p, C.c_int, _pi64)
+
+
+class NidoError(RuntimeError):
+    """A negative status from libnido_amd, with the library's message."""
+
+    def __init__(self, code: int, where: str):
+        msg = (lib.nido_last_error() or b"").decode(errors="replace")
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise NidoError(rc, where)
+
+
+def _p64(a: np.ndarray):
+    return a.ctypes.data_as(_pi64)
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(_pdbl)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib.nido_device_count(C.byref(n)), "nido_device_count")
+    return n.value
+
+
+def batch_bounds_host(off: np.ndarray, nbatches: int) -> np.ndarray:
+    """partition_graph_edge_batch (graph_gpu.cpp:303-339) on the host, for
+    callers that want the boundaries before creating device state."""
+    nv = len(off) - 1
+    ave = int(off[nv] - off[0]) // nbatches
+    parts = [0]
+    start = 0
+    for i in range(1, nv + 1):
+        if off[i] - off[start] > ave:
+            parts.append(i)
+            start = i
+        elif i == nv:
+            parts.append(nv)
+    if len(parts) > nbatches + 1:
+        nnv = (nv - parts[nbatches]) // nbatches
+        for i in range(1, nbatches):
+            parts[i] += nnv
+        parts[nbatches] = nv
+    while len(parts) < nbatches + 1:
+        parts.append(nv)
+    return np.asarray(parts[: nbatches + 1], dtype=np.int64)
+
+
+class Graph:
+    """Host CSR graph (the reference's Graph, graph.hpp:14-102)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @classmethod
+    def read(cls, path: str) -> "Graph":
+        h = C.c_void_p()
+        _check(lib.nido_hgraph_read(os.fsencode(path), C.byref(h)), "nido_hgraph_read")
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, off, col, w) -> "Graph":
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib.nido_hgraph_from_csr(len(off) - 1, len(col), _p64(off), _p64(col), _pd(w),
+                                        C.byref(h)), "nido_hgraph_from_csr")
+        return cls(h)
+
+    def write(self, path: str) -> None:
+        _check(lib.nido_hgraph_write(self._h, os.fsencode(path)), "nido_hgraph_write")
+
+    @property
+    def num_vertices(self) -> int:
+        nv = C.c_int64()
+        _check(lib.nido_hgraph_info(self._h, C.byref(nv), None), "nido_hgraph_info")
+        return nv.value
+
+    @property
+    def num_edges(self) -> int:
+        ne = C.c_int64()
+        _check(lib.nido_hgraph_info(self._h, None, C.byref(ne)), "nido_hgraph_info")
+        return ne.value
+
+    def arrays(self, copy: bool = True):
+        """(offsets, cols, weights) as numpy arrays."""
+        nv, ne = self.num_vertices, self.num_edges
+        po, pc, pw = _pi64(), _pi64(), _pdbl()
+        _check(lib.nido_hgraph_arrays(self._h, C.byref(po), C.byref(pc), C.byref(pw)),
+               "nido_hgraph_arrays")
+        off = np.ctypeslib.as_array(po, shape=(nv + 1,))
+        col = np.ctypeslib.as_array(pc, shape=(ne,)) if ne else np.zeros(0, np.int64)
+        w = np.ctypeslib.as_array(pw, shape=(ne,)) if ne else np.zeros(0, np.float64)
+        if copy:
+            return off.copy(), col.copy(), w.copy()
+        return off, col, w
+
+    def close(self) -> None:
+        if self._h:
+            lib.nido_hgraph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gen_sbm(nv: int, block: int, p_in: float, p_out: float, weighted: bool, seed: int) -> Graph:
+    h = C.c_void_p()
+    _check(lib.nido_gen_sbm(nv, block, p_in, p_out, int(weighted), seed, C.byref(h)), "nido_gen_sbm")
+    return Graph(h)
+
+
+def gen_rmat(scale: int, edge_factor: int, a: float, b: float, c: float, weighted: bool,
+             seed: int) -> Graph:
+    h = C.c_void_p()
+    _check(lib.nido_gen_rmat(scale, edge_factor, a, b, c, int(weighted), seed, C.byref(h)),
+           "nido_gen_rmat")
+    return Graph(h)
+
+
+def gen_chung_lu(nv: int, alpha: float, dmin: float, dmax: float, mean: float, weighted: bool,
+                 seed: int) -> Graph:
+    h = C.c_void_p()
+    _check(lib.nido_gen_chung_lu(nv, alpha, dmin, dmax, mean, int(weighted), seed, C.byref(h)),
+           "nido_gen_chung_lu")
+    return Graph(h)
+
+
+class GraphGPU:
+    """Device-resident Louvain state (the reference's GraphGPU,
+    graph_gpu.hpp:168-247), one graph in HBM."""
+
+    def __init__(self, graph, nbatches: int = DEFAULT_BATCHES, stream: int | None = None):
+        if isinstance(graph, Graph):
+            off, col, w = graph.arrays(copy=False)
+        else:
+            off, col, w = graph
+            off = np.ascontiguousarray(off, dtype=np.int64)
+            col = np.ascontiguousarray(col, dtype=np.int64)
+            w = np.ascontiguousarray(w, dtype=np.float64)
+        self._keep = (graph,)
+        h = C.c_void_p()
+        _check(lib.nido_graph_create(len(off) - 1, len(col), _p64(off), _p64(col), _pd(w), nbatches,
+                                     C.c_void_p(stream) if stream else None, C.byref(h)),
+               "nido_graph_create")
+        self._h = h
+        self.nbatches = nbatches
+        self.n0 = len(off) - 1
+
+    # ---- info ----
+    def info(self):
+        nv, ne, mo, m = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib.nido_graph_info(self._h, C.byref(nv), C.byref(ne), C.byref(mo), C.byref(m)),
+               "nido_graph_info")
+        return {"nv": nv.value, "ne": ne.value, "max_order": mo.value, "mass": m.value}
+
+    @property
+    def num_vertices(self) -> int:
+        return self.info()["nv"]
+
+    def batch_bounds(self) -> np.ndarray:
+        out = np.zeros(self.nbatches + 1, dtype=np.int64)
+        _check(lib.nido_graph_batch_bounds(self._h, _p64(out)), "nido_graph_batch_bounds")
+        return out
+
+    def set_tuning(self, thread_max_deg: int | None = None, use_graph: int | None = None) -> None:
+        t = Tuning()
+        _check(lib.nido_graph_get_tuning(self._h, C.byref(t)), "nido_graph_get_tuning")
+        if thread_max_deg is not None:
+            t.thread_max_deg = thread_max_deg
+        if use_graph is not None:
+            t.use_graph = use_graph
+        _check(lib.nido_graph_set_tuning(self._h, C.byref(t)), "nido_graph_set_tuning")
+
+    def sync(self) -> None:
+        _check(lib.nido_graph_sync(self._h), "nido_graph_sync")
+
+    # ---- GraphGPU methods ----
+    def singleton_partition(self) -> None:
+        _check(lib.nido_singleton_partition(self._h), "nido_singleton_partition")
+
+    def compute_modularity(self) -> float:
+        q = C.c_double()
+        _check(lib.nido_compute_modularity(self._h, C.byref(q)), "nido_compute_modularity")
+        return q.value
+
+    def louvain_update(self, batch: int) -> None:
+        _check(lib.nido_louvain_update(self._h, batch), "nido_louvain_update")
+
+    def update_community_weights(self, batch: int) -> None:
+        _check(lib.nido_update_community_weights(self._h, batch), "nido_update_community_weights")
+
+    def update_community_ids(self, batch: int) -> None:
+        _check(lib.nido_update_community_ids(self._h, batch), "nido_update_community_ids")
+
+    def restore_community(self) -> None:
+        _check(lib.nido_restore_community(self._h), "nido_restore_community")
+
+    def aggregation(self) -> bool:
+        done = C.c_int()
+        _check(lib.nido_aggregation(self._h, C.byref(done)), "nido_aggregation")
+        return bool(done.value)
+
+    # ---- readers ----
+    def communities(self) -> np.ndarray:
+        out = np.zeros(self.num_vertices, dtype=np.int64)
+        _check(lib.nido_get_communities(self._h, _p64(out)), "nido_get_communities")
+        return out
+
+    def new_communities(self) -> np.ndarray:
+        out = np.zeros(self.num_vertices, dtype=np.int64)
+        _check(lib.nido_get_new_communities(self._h, _p64(out)), "nido_get_new_communities")
+        return out
+
+    def vertex_weights(self) -> np.ndarray:
+        out = np.zeros(self.num_vertices, dtype=np.float64)
+        _check(lib.nido_get_vertex_weights(self._h, _pd(out)), "nido_get_vertex_weights")
+        return out
+
+    def community_weights(self) -> np.ndarray:
+        out = np.zeros(self.num_vertices, dtype=np.float64)
+        _check(lib.nido_get_community_weights(self._h, _pd(out)), "nido_get_community_weights")
+        return out
+
+    def graph(self):
+        i = self.info()
+        off = np.zeros(i["nv"] + 1, dtype=np.int64)
+        col = np.zeros(i["ne"], dtype=np.int64)
+        w = np.zeros(i["ne"], dtype=np.float64)
+        _check(lib.nido_get_graph(self._h, _p64(off), _p64(col), _pd(w)), "nido_get_graph")
+        return off, col, w
+
+    def clustering(self) -> np.ndarray:
+        out = np.zeros(self.n0, dtype=np.int64)
+        _check(lib.nido_get_clustering(self._h, _p64(out)), "nido_get_clustering")
+        return out
+
+    def dump_partition(self, path: str) -> None:
+        _check(lib.nido_dump_partition(self._h, os.fsencode(path)), "nido_dump_partition")
+
+    # ---- run history ----
+    def phase(self, i: int) -> PhaseStats:
+        st = PhaseStats()
+        _check(lib.nido_run_phase(self._h, i, C.byref(st)), "nido_run_phase")
+        return st
+
+    def iteration_q(self, i: int) -> np.ndarray:
+        out = np.zeros(self.phase(i).loops, dtype=np.float64)
+        _check(lib.nido_run_iteration_q(self._h, i, _pd(out)), "nido_run_iteration_q")
+        return out
+
+    def phase_communities(self, i: int) -> np.ndarray:
+        out = np.zeros(self.phase(i).nv, dtype=np.int64)
+        _check(lib.nido_run_phase_communities(self._h, i, _p64(out)), "nido_run_phase_communities")
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.nido_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LouvainGPU:
+    """The reference's LouvainGPU (louvain_gpu.hpp:15-23): the driver loop
+    runs inside the library (nido_louvain_run)."""
+
+    def __init__(self, max_loops: int = DEFAULT_ITERATIONS, tol: float = DEFAULT_THRESHOLD,
+                 nbatches: int = DEFAULT_BATCHES, tol_phase: float | None = None):
+        self.max_loops = max_loops
+        self.tol = tol
+        self.tol_phase = tol if tol_phase is None else tol_phase
+        self.nbatches = nbatches
+
+    def run(self, graph: GraphGPU, record: bool = False, verbose: bool = False) -> RunStats:
+        st = RunStats()
+        _check(lib.nido_louvain_run(graph._h, self.max_loops, self.tol, self.tol_phase, int(record),
+                                    int(verbose), C.byref(st)), "nido_louvain_run")
+        return st

@@ -1,0 +1,118 @@
+// fixed.h -- exact Q64.64 accumulators for the Louvain sums (host + device).
+//
+// Every sum of input weights on the local-moving path (vertex weight k_v,
+// neighbour-community weight e_{v,c}, community total a_c, 2m, the internal
+// weight of the modularity and the coarse arc weights) is held as a
+// two's-complement 128-bit integer in units of 2^-64.  Integer addition is
+// associative, so LDS/global atomics in any order, any reduction tree and
+// any launch shape give identical bits; each sum is rounded to FP64 once,
+// round-to-nearest-even, where the FP64 gain/modularity formulas read it.
+//
+// The reference sums the same quantities in FP64 in launch-dependent order
+// (graph_cuda.cu:335-340 warp trees, graph_gpu.cpp:499 reduce_by_key,
+// graph_cuda.cu:1244-1248 FP64 atomics), which is what this replaces.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define NIDO_HD __host__ __device__ __forceinline__
+#else
+#define NIDO_HD static inline
+#endif
+
+typedef __int128 fx_t;
+typedef unsigned __int128 ufx_t;
+
+NIDO_HD uint64_t nido_dbits(double d) { return __builtin_bit_cast(uint64_t, d); }
+NIDO_HD double nido_bitsd(uint64_t b) { return __builtin_bit_cast(double, b); }
+
+NIDO_HD fx_t fx_make(uint64_t lo, int64_t hi)
+{
+    return (fx_t)(((ufx_t)(uint64_t)hi << 64) | (ufx_t)lo);
+}
+NIDO_HD uint64_t fx_lo(fx_t x) { return (uint64_t)x; }
+NIDO_HD int64_t fx_hi(fx_t x) { return (int64_t)((ufx_t)x >> 64); }
+
+// FP64 -> Q64.64, round-to-nearest-even at 2^-64.  Exact for every weight
+// on the 2^-64 grid (all |w| >= 2^-12).  The host checks |w| < 2^62 and
+// finiteness once, when a graph is created.
+NIDO_HD fx_t fx_from_double(double w)
+{
+    uint64_t b = nido_dbits(w);
+    uint32_t ex = (uint32_t)(b >> 52) & 0x7ffu;
+    uint64_t mt = b & 0x000fffffffffffffull;
+    if (ex == 0) {
+        if (mt == 0) return 0;
+        ex = 1;                                  // subnormal: below 2^-1022
+    } else {
+        mt |= 0x0010000000000000ull;
+    }
+    int sh = (int)ex - 1011;                     // w * 2^64 = mt * 2^sh
+    ufx_t u;
+    if (sh >= 0) {
+        u = (ufx_t)mt << sh;
+    } else {
+        int r = -sh;
+        if (r >= 54) return 0;
+        uint64_t q = mt >> r;
+        uint64_t rest = mt & ((1ull << r) - 1);
+        uint64_t half = 1ull << (r - 1);
+        q += (rest > half) || (rest == half && (q & 1ull));
+        u = q;
+    }
+    fx_t x = (fx_t)u;
+    return (b >> 63) ? -x : x;
+}
+
+// Q64.64 -> FP64, round-to-nearest-even.
+NIDO_HD double fx_to_double(fx_t x)
+{
+    if (x == 0) return 0.0;
+    bool neg = x < 0;
+    ufx_t u = neg ? (ufx_t)0 - (ufx_t)x : (ufx_t)x;
+    uint64_t hi = (uint64_t)(u >> 64);
+    int top = hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll((uint64_t)u);
+    double r;
+    if (top <= 52) {
+        // < 2^53 units: the integer converts exactly; 2^-64 scaling is exact
+        r = (double)(uint64_t)u * nido_bitsd((uint64_t)(1023 - 64) << 52);
+    } else {
+        int s = top - 52;
+        uint64_t mt = (uint64_t)(u >> s);
+        ufx_t rest = u & (((ufx_t)1 << s) - 1);
+        ufx_t half = (ufx_t)1 << (s - 1);
+        mt += (rest > half) || (rest == half && (mt & 1ull));
+        if (mt == (1ull << 53)) { mt >>= 1; ++s; }
+        r = (double)mt * nido_bitsd((uint64_t)(1023 + s - 64) << 52);
+    }
+    return neg ? -r : r;
+}
+
+#if defined(__HIPCC__)
+// Exact atomic add of a Q64.64 value held as two 64-bit words {lo, hi}.
+// The low word wraps modulo 2^64 and its carry is added to the high word, so
+// the final 128-bit value is the exact sum whatever the interleaving.
+// Works on LDS and global addresses alike (the compiler picks ds_/global_
+// atomics when the address space is known after inlining).
+__device__ __forceinline__ void fx_atomic_add(unsigned long long* lo, unsigned long long* hi, fx_t v)
+{
+    unsigned long long vl = (unsigned long long)fx_lo(v);
+    unsigned long long vh = (unsigned long long)fx_hi(v);
+    if (vl) {
+        unsigned long long old = atomicAdd(lo, vl);
+        vh += (old + vl) < old;
+    }
+    if (vh) atomicAdd(hi, vh);
+}
+
+// Lane-exchange helpers for 128-bit values within a wavefront.
+__device__ __forceinline__ fx_t fx_shfl_xor(fx_t x, int mask, int width)
+{
+    uint64_t lo = fx_lo(x);
+    int64_t hi = fx_hi(x);
+    lo = __shfl_xor(lo, mask, width);
+    hi = __shfl_xor(hi, mask, width);
+    return fx_make(lo, hi);
+}
+#endif

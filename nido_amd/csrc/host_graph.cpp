@@ -1,0 +1,413 @@
+// host_graph.cpp -- host CSR graph (the reference's Graph, graph.hpp:14-102):
+// binary reader/writer and the seeded synthetic-input generators.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <new>
+#include <string>
+#include <vector>
+#include <omp.h>
+#include <parallel/algorithm>
+
+#include "../../include/nido_amd.h"
+#include "errors.h"
+
+struct nido_hgraph {
+    int64_t nv = 0, ne = 0;
+    std::vector<int64_t> off;
+    std::vector<int64_t> col;
+    std::vector<double> w;
+};
+
+namespace nido {
+
+static thread_local char g_err[512];
+
+int set_error(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+void clear_error() { g_err[0] = 0; }
+
+const char* last_error() { return g_err; }
+
+namespace {
+
+// ---- counter-based randomness ----------------------------------------------
+inline uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next() { return mix64(s++ * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull); }
+    double uniform() { return (double)(next() >> 11) * 0x1p-53; }        // [0, 1)
+    double uniform_pos() { return 1.0 - uniform(); }                     // (0, 1]
+};
+
+inline uint64_t stream_seed(uint64_t seed, uint64_t stream)
+{
+    return mix64(seed ^ mix64(stream + 0x5851F42D4C957F2Dull));
+}
+
+// One weight per undirected edge {u, v}, u < v: (0, 1] on a 2^-53 grid,
+// a pure function of (seed, u, v) so duplicates and order never matter.
+inline double edge_weight(uint64_t seed, int64_t u, int64_t v)
+{
+    const uint64_t r = mix64(seed ^ mix64((uint64_t)u * 0x9E3779B97F4A7C15ull ^ (uint64_t)v));
+    return (double)((1ull << 53) - (r >> 11)) * 0x1p-53;
+}
+
+// Builds a symmetric CSR from unique undirected edges packed as (u<<32)|v
+// with u < v, sorted ascending.  Rows come out sorted by neighbour id: row x
+// receives its reverse arcs (from u < x) before its forward arcs (to v > x).
+nido_hgraph* csr_from_sorted_pairs(int64_t nv, const std::vector<uint64_t>& e, int weighted,
+                                   uint64_t wseed)
+{
+    nido_hgraph* g = new nido_hgraph;
+    g->nv = nv;
+    g->ne = 2 * (int64_t)e.size();
+    g->off.assign((size_t)nv + 1, 0);
+    for (uint64_t p : e) {
+        g->off[(p >> 32) + 1]++;
+        g->off[(p & 0xffffffffull) + 1]++;
+    }
+    for (int64_t i = 0; i < nv; ++i) g->off[i + 1] += g->off[i];
+    g->col.resize((size_t)g->ne);
+    g->w.resize((size_t)g->ne);
+    std::vector<int64_t> cur(g->off.begin(), g->off.end() - 1);
+    for (uint64_t p : e) {
+        const int64_t u = (int64_t)(p >> 32), v = (int64_t)(p & 0xffffffffull);
+        const double wt = weighted ? edge_weight(wseed, u, v) : 1.0;
+        int64_t a = cur[u]++, b = cur[v]++;
+        g->col[a] = v; g->w[a] = wt;
+        g->col[b] = u; g->w[b] = wt;
+    }
+    return g;
+}
+
+void sort_unique(std::vector<uint64_t>& e)
+{
+    __gnu_parallel::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+}
+
+int check_out(nido_hgraph** out)
+{
+    if (!out) return set_error(NIDO_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    return NIDO_OK;
+}
+
+}  // namespace
+}  // namespace nido
+
+using namespace nido;
+
+extern "C" {
+
+const char* nido_last_error(void) { return nido::last_error(); }
+
+int nido_hgraph_from_csr(int64_t nv, int64_t ne, const int64_t* off, const int64_t* col,
+                         const double* w, nido_hgraph** out)
+{
+    int rc = check_out(out);
+    if (rc) return rc;
+    if (nv < 0 || !off || (nv > 0 && off[nv] - off[0] > 0 && (!col || !w)))
+        return set_error(NIDO_ERR_INVALID, "invalid CSR arguments");
+    const int64_t base = off[0];
+    const int64_t arcs = off[nv] - base;
+    if (arcs < 0) return set_error(NIDO_ERR_INVALID, "offsets decrease");
+    (void)ne;
+    try {
+        nido_hgraph* g = new nido_hgraph;
+        g->nv = nv;
+        g->ne = arcs;
+        g->off.resize((size_t)nv + 1);
+        for (int64_t i = 0; i <= nv; ++i) g->off[i] = off[i] - base;
+        g->col.assign(col + base, col + base + arcs);
+        g->w.assign(w + base, w + base + arcs);
+        *out = g;
+    } catch (const std::bad_alloc&) {
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    }
+    return NIDO_OK;
+}
+
+int nido_hgraph_read(const char* path, nido_hgraph** out)
+{
+    int rc = check_out(out);
+    if (rc) return rc;
+    std::ifstream f(path ? path : "", std::ios::in | std::ios::binary);
+    if (!f.is_open()) return set_error(NIDO_ERR_IO, "cannot open %s", path ? path : "(null)");
+    int64_t hdr[2];
+    if (!f.read(reinterpret_cast<char*>(hdr), sizeof(hdr)))
+        return set_error(NIDO_ERR_IO, "%s: short header", path);
+    const int64_t nv = hdr[0];
+    if (nv < 0) return set_error(NIDO_ERR_INVALID, "%s: negative vertex count", path);
+    try {
+        nido_hgraph* g = new nido_hgraph;
+        g->nv = nv;
+        g->off.resize((size_t)nv + 1);
+        if (!f.read(reinterpret_cast<char*>(g->off.data()), (std::streamsize)(sizeof(int64_t) * (nv + 1)))) {
+            delete g;
+            return set_error(NIDO_ERR_IO, "%s: short offsets", path);
+        }
+        // graph.cpp:376-382: the arc count comes from the offsets
+        const int64_t base = g->off[0];
+        const int64_t arcs = g->off[nv] - base;
+        if (arcs < 0) {
+            delete g;
+            return set_error(NIDO_ERR_INVALID, "%s: offsets decrease", path);
+        }
+        for (int64_t i = 0; i <= nv; ++i) g->off[i] -= base;      // graph.cpp:470-472
+        g->ne = arcs;
+        g->col.resize((size_t)arcs);
+        g->w.resize((size_t)arcs);
+        // records {int64 tail, double weight} (types.hpp:48-55), read right
+        // after the offsets (the GRAPH_FT_LOAD path, graph.cpp:397-435)
+        const int64_t chunk = 1 << 20;
+        std::vector<char> buf;
+        buf.resize((size_t)std::min<int64_t>(arcs, chunk) * 16 + 16);
+        for (int64_t e0 = 0; e0 < arcs; e0 += chunk) {
+            const int64_t n = std::min(chunk, arcs - e0);
+            if (!f.read(buf.data(), (std::streamsize)(n * 16))) {
+                delete g;
+                return set_error(NIDO_ERR_IO, "%s: short edge records", path);
+            }
+            for (int64_t i = 0; i < n; ++i) {
+                std::memcpy(&g->col[e0 + i], buf.data() + 16 * i, 8);
+                std::memcpy(&g->w[e0 + i], buf.data() + 16 * i + 8, 8);
+            }
+        }
+        *out = g;
+    } catch (const std::bad_alloc&) {
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    }
+    return NIDO_OK;
+}
+
+int nido_hgraph_write(const nido_hgraph* g, const char* path)
+{
+    if (!g || !path) return set_error(NIDO_ERR_INVALID, "null argument");
+    std::ofstream f(path, std::ios::out | std::ios::binary | std::ios::trunc);
+    if (!f.is_open()) return set_error(NIDO_ERR_IO, "cannot open %s for writing", path);
+    const int64_t hdr[2] = {g->nv, g->ne};
+    f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+    f.write(reinterpret_cast<const char*>(g->off.data()), (std::streamsize)(sizeof(int64_t) * (g->nv + 1)));
+    std::vector<char> buf;
+    const int64_t chunk = 1 << 20;
+    buf.resize((size_t)std::min<int64_t>(g->ne, chunk) * 16 + 16);
+    for (int64_t e0 = 0; e0 < g->ne; e0 += chunk) {
+        const int64_t n = std::min(chunk, g->ne - e0);
+        for (int64_t i = 0; i < n; ++i) {
+            std::memcpy(buf.data() + 16 * i, &g->col[e0 + i], 8);
+            std::memcpy(buf.data() + 16 * i + 8, &g->w[e0 + i], 8);
+        }
+        f.write(buf.data(), (std::streamsize)(n * 16));
+    }
+    if (!f.good()) return set_error(NIDO_ERR_IO, "write to %s failed", path);
+    return NIDO_OK;
+}
+
+int nido_hgraph_info(const nido_hgraph* g, int64_t* nv, int64_t* ne)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (nv) *nv = g->nv;
+    if (ne) *ne = g->ne;
+    return NIDO_OK;
+}
+
+int nido_hgraph_arrays(const nido_hgraph* g, const int64_t** off, const int64_t** col,
+                       const double** w)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (off) *off = g->off.data();
+    if (col) *col = g->col.data();
+    if (w) *w = g->w.data();
+    return NIDO_OK;
+}
+
+void nido_hgraph_destroy(nido_hgraph* g) { delete g; }
+
+int nido_gen_sbm(int64_t nv, int64_t block, double p_in, double p_out, int weighted,
+                 uint64_t seed, nido_hgraph** out)
+{
+    int rc = check_out(out);
+    if (rc) return rc;
+    if (nv < 0 || nv >= (1ll << 31) || block < 1 || !(p_in >= 0 && p_in <= 1) ||
+        !(p_out >= 0 && p_out <= 1))
+        return set_error(NIDO_ERR_INVALID, "invalid SBM parameters");
+    try {
+        // per source vertex u, successes among v in (u, nv) by geometric skips,
+        // in lexicographic pair order; stream u of the seed drives vertex u
+        std::vector<std::vector<uint64_t>> rows((size_t)nv);
+        #pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t u = 0; u < nv; ++u) {
+            Rng r(stream_seed(seed, (uint64_t)u));
+            auto& row = rows[(size_t)u];
+            const int64_t bend = std::min(nv, (u / block + 1) * block);
+            auto sweep = [&](int64_t lo, int64_t hi, double p) {
+                if (p <= 0 || lo >= hi) return;
+                if (p >= 1) {
+                    for (int64_t v = lo; v < hi; ++v) row.push_back(((uint64_t)u << 32) | (uint64_t)v);
+                    return;
+                }
+                const double lq = std::log1p(-p);
+                int64_t v = lo - 1;
+                for (;;) {
+                    const double skip = std::floor(std::log(r.uniform_pos()) / lq);
+                    if (skip >= (double)(hi - v)) break;
+                    v += 1 + (int64_t)skip;
+                    if (v >= hi) break;
+                    row.push_back(((uint64_t)u << 32) | (uint64_t)v);
+                }
+            };
+            sweep(u + 1, bend, p_in);
+            sweep(bend, nv, p_out);
+        }
+        std::vector<uint64_t> e;
+        size_t total = 0;
+        for (auto& r : rows) total += r.size();
+        e.reserve(total);
+        for (auto& r : rows) {
+            e.insert(e.end(), r.begin(), r.end());
+            std::vector<uint64_t>().swap(r);
+        }
+        *out = csr_from_sorted_pairs(nv, e, weighted, seed ^ 0xA5A5A5A5DEADBEEFull);
+    } catch (const std::bad_alloc&) {
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    }
+    return NIDO_OK;
+}
+
+int nido_gen_rmat(int scale, int64_t edge_factor, double a, double b, double c, int weighted,
+                  uint64_t seed, nido_hgraph** out)
+{
+    int rc = check_out(out);
+    if (rc) return rc;
+    if (scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1)
+        return set_error(NIDO_ERR_INVALID, "invalid R-MAT parameters");
+    const int64_t nv = 1ll << scale;
+    const int64_t ns = edge_factor * nv;
+    try {
+        std::vector<uint64_t> e((size_t)ns);
+        const double ab = a + b, abc = a + b + c;
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < ns; ++i) {
+            Rng r(stream_seed(seed, (uint64_t)i));
+            uint64_t u = 0, v = 0;
+            for (int l = 0; l < scale; ++l) {
+                const double x = r.uniform();
+                const uint64_t bu = x >= ab;                        // c or d quadrant
+                const uint64_t bv = (x >= a && x < ab) || x >= abc; // b or d quadrant
+                u = (u << 1) | bu;
+                v = (v << 1) | bv;
+            }
+            if (u == v) e[(size_t)i] = ~0ull;                       // self-loop, dropped
+            else {
+                const uint64_t lo = u < v ? u : v, hi = u < v ? v : u;
+                e[(size_t)i] = (lo << 32) | hi;
+            }
+        }
+        sort_unique(e);
+        if (!e.empty() && e.back() == ~0ull) e.pop_back();
+        *out = csr_from_sorted_pairs(nv, e, weighted, seed ^ 0xA5A5A5A5DEADBEEFull);
+    } catch (const std::bad_alloc&) {
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    }
+    return NIDO_OK;
+}
+
+int nido_gen_chung_lu(int64_t nv, double alpha, double dmin, double dmax, double mean,
+                      int weighted, uint64_t seed, nido_hgraph** out)
+{
+    int rc = check_out(out);
+    if (rc) return rc;
+    if (nv < 2 || nv >= (1ll << 31) || !(alpha > 1) || !(dmin > 0) || !(dmax >= dmin) ||
+        !(mean >= dmin && mean <= dmax))
+        return set_error(NIDO_ERR_INVALID, "invalid Chung-Lu parameters");
+    try {
+        // expected degrees: Pareto(alpha) with x_min = 1, scaled so that the
+        // mean after clipping to [dmin, dmax] is `mean`
+        std::vector<double> base((size_t)nv), theta((size_t)nv);
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nv; ++i) {
+            Rng r(stream_seed(seed, (uint64_t)i));
+            base[(size_t)i] = std::pow(r.uniform_pos(), -1.0 / (alpha - 1.0));
+        }
+        double s = 1.0;
+        for (int it = 0; it < 40; ++it) {
+            double tot = 0;
+            #pragma omp parallel for reduction(+ : tot) schedule(static)
+            for (int64_t i = 0; i < nv; ++i)
+                tot += std::min(dmax, std::max(dmin, s * base[(size_t)i]));
+            s *= mean / (tot / (double)nv);
+        }
+        double tot = 0;
+        for (int64_t i = 0; i < nv; ++i) {
+            theta[(size_t)i] = std::min(dmax, std::max(dmin, s * base[(size_t)i]));
+            tot += theta[(size_t)i];
+        }
+        std::vector<double>().swap(base);
+        // Vose alias table over the expected degrees
+        std::vector<double> prob((size_t)nv);
+        std::vector<int64_t> alias((size_t)nv);
+        {
+            std::vector<int64_t> small, large;
+            std::vector<double> p((size_t)nv);
+            for (int64_t i = 0; i < nv; ++i) {
+                p[(size_t)i] = theta[(size_t)i] * (double)nv / tot;
+                (p[(size_t)i] < 1.0 ? small : large).push_back(i);
+            }
+            while (!small.empty() && !large.empty()) {
+                const int64_t l = small.back(); small.pop_back();
+                const int64_t g = large.back(); large.pop_back();
+                prob[(size_t)l] = p[(size_t)l];
+                alias[(size_t)l] = g;
+                p[(size_t)g] = (p[(size_t)g] + p[(size_t)l]) - 1.0;
+                (p[(size_t)g] < 1.0 ? small : large).push_back(g);
+            }
+            for (int64_t i : large) { prob[(size_t)i] = 1.0; alias[(size_t)i] = i; }
+            for (int64_t i : small) { prob[(size_t)i] = 1.0; alias[(size_t)i] = i; }
+        }
+        const int64_t ns = (int64_t)std::llround((double)nv * mean / 2.0);
+        std::vector<uint64_t> e((size_t)ns);
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < ns; ++i) {
+            Rng r(stream_seed(seed ^ 0x3C6EF372FE94F82Bull, (uint64_t)i));
+            int64_t end[2];
+            for (int k = 0; k < 2; ++k) {
+                const uint64_t x = r.next();
+                const int64_t col = (int64_t)((x >> 32) * (uint64_t)nv >> 32);
+                const double y = (double)(x & 0xffffffffull) * 0x1p-32;
+                end[k] = y < prob[(size_t)col] ? col : alias[(size_t)col];
+            }
+            if (end[0] == end[1]) e[(size_t)i] = ~0ull;
+            else {
+                const uint64_t lo = (uint64_t)std::min(end[0], end[1]);
+                const uint64_t hi = (uint64_t)std::max(end[0], end[1]);
+                e[(size_t)i] = (lo << 32) | hi;
+            }
+        }
+        sort_unique(e);
+        if (!e.empty() && e.back() == ~0ull) e.pop_back();
+        *out = csr_from_sorted_pairs(nv, e, weighted, seed ^ 0xA5A5A5A5DEADBEEFull);
+    } catch (const std::bad_alloc&) {
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    }
+    return NIDO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// internal.h -- declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fixed.h"
+
+namespace nido {
+
+// Degree bins of the local-moving kernel.  Bin b holds the vertices whose
+// row length d satisfies kBinMaxDeg[b-1] < d <= kBinMaxDeg[b] (defaults;
+// the tuning struct can move the thread-bin bound).
+enum LmBin : int {
+    BIN_THREAD = 0,   // one lane per vertex, neighbours in registers
+    BIN_G8,           // 8 lanes per vertex, 32-slot LDS table
+    BIN_G16,          // 16 lanes, 64 slots
+    BIN_G32,          // 32 lanes, 128 slots
+    BIN_G64,          // one wavefront, 256 slots
+    BIN_B256,         // one 256-lane workgroup, 1024 slots
+    BIN_HUGE,         // one workgroup, open-addressing table in HBM
+    NUM_BINS
+};
+
+constexpr int kThreadMaxDeg = 8;          // registers hold up to this many arcs
+constexpr int64_t kBinMaxDeg[NUM_BINS] = {8, 16, 32, 64, 128, 512, INT64_MAX};
+
+// Everything one local-moving launch reads or writes.
+struct LmArgs {
+    const int64_t* off;
+    const int64_t* col;
+    const double* w;
+    const int64_t* C;        // community of every vertex (snapshot)
+    int64_t* Cn;             // decided community of the batch's vertices
+    const double* kd;        // k_v rounded to FP64
+    const fx_t* k;           // k_v exact
+    const fx_t* a;           // a_c exact
+    double m;                // mass, FP64
+};
+
+// A slice of one row for the degree / modularity sweeps over long rows.
+struct RowItem {
+    int64_t v;
+    int64_t e0, e1;
+};
+
+// ---- launch helpers (kernels.hip) ----------------------------------------
+void launch_fill_i64(int64_t* p, int64_t val, int64_t n, hipStream_t s);
+void launch_fill_i32(int32_t* p, int32_t val, int64_t n, hipStream_t s);
+void launch_fill_fx(fx_t* p, int64_t n, hipStream_t s);
+
+// k_v and 2m.  `small` lists rows handled by one lane, `items` slices of
+// longer rows (one wavefront each).  acc receives sum(k) exactly.
+void launch_vertex_weights(const int64_t* off, const double* w, const int* small, int64_t nsmall,
+                           const RowItem* items, int64_t nitems, fx_t* k, double* kd,
+                           int64_t nv, fx_t* acc, hipStream_t s);
+void launch_singleton(int64_t* C, int64_t* Cn, fx_t* a, const fx_t* k, int64_t nv, hipStream_t s);
+
+void launch_lm_bin(int bin, const LmArgs& A, const int* verts, int64_t n,
+                   const int64_t* tab_off, int* tab_key, fx_t* tab_val, hipStream_t s);
+
+void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, const fx_t* k, fx_t* a,
+                   hipStream_t s);
+
+// Modularity: acc[0] += internal weight, acc[1] += sum_c (a_c/2m)^2; the
+// finalize kernel writes Q to *q and clears acc for the next call.
+void launch_modularity(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
+                       const fx_t* a, int64_t nv, const int* small, int64_t nsmall,
+                       const RowItem* items, int64_t nitems, double twom_d, fx_t* acc,
+                       double* q, hipStream_t s);
+
+// Exclusive scan of n int64 values (out may alias in); writes the total to
+// *total (device) when total is non-null.
+size_t scan_temp_bytes(int64_t n);
+void launch_exclusive_scan(const int64_t* in, int64_t* out, int64_t n, int64_t* total,
+                           void* temp, hipStream_t s);
+
+// ---- coarsening ------------------------------------------------------------
+void launch_mark_communities(const int64_t* C, int64_t nv, int64_t* flag, hipStream_t s);
+void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const int64_t* newid,
+                              hipStream_t s);
+void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t* off, int64_t nv,
+                          int64_t* cnt, int64_t* arcs, hipStream_t s);
+void launch_table_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, hipStream_t s);
+void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
+                          const int64_t* newid, int64_t nv, const int64_t* toff, int* tkey,
+                          fx_t* tval, hipStream_t s);
+void launch_coarse_count(const int64_t* toff, const int* tkey, int64_t K, int64_t* cnt,
+                         hipStream_t s);
+void launch_coarse_emit(const int64_t* toff, const int* tkey, const fx_t* tval, int64_t K,
+                        const int64_t* noff, int64_t* ncol, double* nw, hipStream_t s);
+
+}  // namespace nido

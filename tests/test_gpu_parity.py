@@ -1,0 +1,165 @@
+"""HIP path vs the oracle on the same seeded graphs: per-phase communities
+bit-exact, modularity within 1e-12 (identical in practice), through the
+C ABI (nido_amd -> libnido_amd.so)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from parity import Q_TOL, assert_runs_equal, run_gpu, run_oracle, sorted_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _na():
+    import nido_amd as na
+    assert na.device_count() >= 1, "no HIP device visible"
+    return na
+
+
+def _fixture(golden_dir):
+    na = _na()
+    off, col, w = na.Graph.read(os.path.join(golden_dir, "sbm1024_seed1.bin")).arrays()
+    with open(os.path.join(golden_dir, "sbm1024_seed1_expected.json")) as f:
+        exp = json.load(f)
+    return off, col, w, exp
+
+
+def test_sbm1024_full_run_matches_golden_and_oracle(golden_dir):
+    off, col, w, exp = _fixture(golden_dir)
+    print("case: sbm1024 seed1 -b 2 -t 1e-6", flush=True)
+    got = run_gpu(off, col, w, 2)
+    ref = run_oracle(off, col, w, 2)
+    assert_runs_equal(ref, got)
+    assert len(got["phases"]) == len(exp["phases"])
+    for p, e in zip(got["phases"], exp["phases"]):
+        assert p["loops"] == e["loops"]
+        assert np.array_equal(p["comm"], np.asarray(e["comm"], dtype=np.int64))
+        assert np.array_equal(p["iter_q"], np.asarray(e["iter_q"]))
+    assert np.array_equal(got["clustering"], np.asarray(exp["clustering"], dtype=np.int64))
+
+
+def _step_compare(off, col, w, nb, phases=2):
+    """Drive GraphGPU and the oracle call by call; compare after every step."""
+    na = _na()
+    g = na.GraphGPU((off, col, w), nb)
+    o = oracle.Oracle(off, col, w, nb)
+    for ph in range(phases):
+        assert np.array_equal(g.batch_bounds(), o.batch_bounds())
+        assert np.array_equal(g.vertex_weights(), o.vertex_weights())
+        assert g.info()["mass"] == o.mass
+        g.singleton_partition()
+        o.singleton_partition()
+        q = g.compute_modularity()
+        assert q == o.compute_modularity()
+        q_old = q
+        dq, loops = 1e308, 0
+        while 1e-6 < dq and loops < 50:
+            for b in range(nb):
+                g.louvain_update(b)
+                o.louvain_update(b)
+                assert np.array_equal(g.new_communities(), o.new_communities()), f"phase {ph} loop {loops} batch {b}"
+                g.update_community_weights(b)
+                g.update_community_ids(b)
+                o.update_community_weights(b)
+                o.update_community_ids(b)
+                assert np.array_equal(g.communities(), o.communities())
+                assert np.array_equal(g.community_weights(), o.community_weights())
+            qt = g.compute_modularity()
+            assert qt == o.compute_modularity()
+            dq = qt - q
+            loops += 1
+            if dq < 0:
+                g.restore_community()
+                o.restore_community()
+                assert np.array_equal(g.communities(), o.communities())
+                assert np.array_equal(g.community_weights(), o.community_weights())
+            else:
+                q = qt
+        if not (q - q_old > 1e-6):
+            break
+        d1 = g.aggregation()
+        d2 = o.aggregation()
+        assert d1 == d2
+        assert np.array_equal(g.clustering(), o.clustering())
+        if d1:
+            break
+        go = sorted_rows(*g.graph())
+        oo = o.graph()
+        for x, y in zip(go, oo):
+            assert np.array_equal(x, y)
+    g.close()
+    o.close()
+
+
+def test_sbm1024_step_by_step(golden_dir):
+    off, col, w, _ = _fixture(golden_dir)
+    print("case: sbm1024 stepwise", flush=True)
+    _step_compare(off, col, w, 2)
+
+
+@pytest.mark.parametrize("nb", [1, 3, 8])
+def test_weighted_sbm_step_by_step(nb):
+    na = _na()
+    off, col, w = na.gen_sbm(3000, 50, 0.2, 0.004, True, 7).arrays()
+    print(f"case: weighted sbm3000 nb={nb}", flush=True)
+    _step_compare(off, col, w, nb)
+
+
+@pytest.mark.parametrize("scale,ef,weighted,nb", [(12, 16, True, 8), (14, 8, False, 32)])
+def test_rmat_full_run(scale, ef, weighted, nb):
+    na = _na()
+    off, col, w = na.gen_rmat(scale, ef, 0.57, 0.19, 0.19, weighted, 7).arrays()
+    print(f"case: rmat s{scale} ef{ef} w{int(weighted)} nb={nb} maxdeg={int(np.max(np.diff(off)))}", flush=True)
+    got = run_gpu(off, col, w, nb)
+    ref = run_oracle(off, col, w, nb)
+    assert_runs_equal(ref, got)
+
+
+def test_launch_shape_does_not_change_results():
+    na = _na()
+    off, col, w = na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 3).arrays()
+    print("case: rmat s12 thread_max_deg sweep", flush=True)
+    outs = []
+    for tmd in (0, 4, 8):
+        g = na.GraphGPU((off, col, w), 4)
+        g.set_tuning(thread_max_deg=tmd)
+        st = na.LouvainGPU(500, 1e-6, 4).run(g, record=True)
+        outs.append((st.q, [g.phase_communities(i) for i in range(st.phases)]))
+        g.close()
+    for q, comms in outs[1:]:
+        assert q == outs[0][0]
+        assert len(comms) == len(outs[0][1])
+        for a, b in zip(comms, outs[0][1]):
+            assert np.array_equal(a, b)
+
+
+def test_repeat_runs_are_bit_identical():
+    na = _na()
+    off, col, w = na.gen_sbm(4000, 64, 0.25, 0.003, True, 11).arrays()
+    print("case: repeat sbm4000", flush=True)
+    a = run_gpu(off, col, w, 4)
+    b = run_gpu(off, col, w, 4)
+    assert_runs_equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["empty", "isolated", "one_edge_b1", "one_edge_b3", "path_rollback"])
+def test_tiny_graphs(case):
+    na = _na()
+    if case == "empty":
+        off, col, w, nb = np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0), 2
+    elif case == "isolated":
+        off, col, w, nb = np.zeros(6, np.int64), np.zeros(0, np.int64), np.zeros(0), 2
+    elif case.startswith("one_edge"):
+        off, col, w = np.array([0, 1, 2]), np.array([1, 0]), np.array([1.0, 1.0])
+        nb = 1 if case.endswith("b1") else 3
+    else:
+        off, col, w = np.array([0, 1, 3, 4]), np.array([1, 0, 2, 1]), np.ones(4)
+        nb = 1
+    print(f"case: tiny {case}", flush=True)
+    got = run_gpu(off, col, w, nb)
+    ref = run_oracle(off, col, w, nb)
+    assert_runs_equal(ref, got)
+    assert abs(got["q"] - ref["q"]) <= Q_TOL
