@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""bench.py -- GTEPS per Louvain iteration on one MI355X.
+
+Workload (the metric's single-GPU configuration): R-MAT scale 22, edge factor
+16, (a,b,c,d) = (0.57, 0.19, 0.19, 0.05), self-loops dropped, symmetrised and
+deduplicated (~128M arcs), FP64 weights uniform in (0,1], seed 7; -b 32,
+-t 1e-6.  The graph is generated here (synthetic) and resident in HBM before
+timing starts.
+
+A step is one Louvain iteration -- the loop body of LouvainGPU::run
+(louvain_gpu.cpp:55-90): all 32 batches of local moving and community-total
+commits, one modularity evaluation, accept or roll back.  Steps continue
+phase 0's iteration sequence; when the phase converges (dQ <= tol) the next
+step starts again from the singleton partition, so every step is a full
+sweep of the graph.
+
+value = arcs x steps / seconds / 1e9 (GTEPS per iteration).  roofline: the
+dominant kernel class by time, its algorithmic bytes (24 B/arc + 40 B/vertex
+for local moving, 24 B/arc + 24 B/vertex for modularity) over its summed
+HIP-event durations on the library's stream, against the 8 TB/s HBM peak;
+roofline_iter applies 48 B/arc + 64 B/vertex to the whole iteration.
+cpu_baseline: the oracle (C + OpenMP) timed on phase-0 iterations of the same
+graph on this host, bounded to ~20 s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+METRIC = "GTEPS per Louvain iteration (+% HBM roofline); batched graphs/sec vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(off, col, w, nb, tol, max_iters, budget_s, gpu_first_q):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    o = oracle.Oracle(off, col, w, nb, threads)
+    o.singleton_partition()
+    q = o.compute_modularity()
+    t_sum, iters, first_q = 0.0, 0, None
+    while iters < max_iters:
+        t0 = time.perf_counter()
+        q, dq, qe = o.louvain_iteration(q)
+        t_sum += time.perf_counter() - t0
+        if first_q is None:
+            first_q = qe
+        iters += 1
+        if t_sum > budget_s or not (tol < dq):
+            break
+    ne = len(col)
+    o.close()
+    return {
+        "value": ne * iters / t_sum / 1e9,
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{iters} phase-0 iterations of the same graph (-b {nb}) from the singleton "
+                  f"partition, {t_sum:.2f} s, C/OpenMP oracle",
+        "first_iteration_q_matches_gpu": bool(first_q == gpu_first_q),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--batches", type=int, default=32)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default="", help="comma list of nido_tuning knobs, e.g. big_slots=2048")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 plain device copies of BYTES each (scales the FETCH_SIZE / "
+                         "WRITE_SIZE counters of a rocprofv3 --pmc pass) and exit")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 5}))
+        return
+
+    import nido_amd as na
+
+    t0 = time.perf_counter()
+    hg = na.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, True, args.seed)
+    off, col, w = hg.arrays(copy=False)
+    nv, ne = len(off) - 1, len(col)
+    log(f"graph: R-MAT s{args.scale} ef{args.edge_factor}: {nv} vertices, {ne} arcs ({time.perf_counter()-t0:.1f}s)")
+
+    g = na.GraphGPU(hg, args.batches)
+    if args.tuning:
+        g.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
+    g.singleton_partition()
+    q = g.compute_modularity()
+    state = {"q": q, "first_q": None, "resets": 0}
+
+    def step():
+        qa, dq, qe = g.louvain_iteration(state["q"])
+        if state["first_q"] is None:
+            state["first_q"] = qe
+        state["q"] = qa
+        if not (args.tol < dq):            # phase 0 converged: start it again
+            g.singleton_partition()
+            state["q"] = g.compute_modularity()
+            state["resets"] += 1
+
+    for _ in range(args.warmup):
+        step()
+    g.sync()
+    g.set_profiling(True)
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    g.sync()
+    dt = time.perf_counter() - t0
+    ks = g.kernel_stats()
+    g.set_profiling(False)
+
+    gteps = ne * args.steps / dt / 1e9
+    ms_step = dt * 1e3 / args.steps
+    dom = max(ks, key=lambda k: ks[k]["ms"])
+    d = ks[dom]
+    ach = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
+    b_iter = 48.0 * ne + 64.0 * nv
+    ach_iter = b_iter * args.steps / dt / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"rmat-s{args.scale}-ef{args.edge_factor} (0.57,0.19,0.19,0.05) U(0,1] weights, "
+                        f"seed {args.seed}, -b {args.batches} -t {args.tol:g}, phase-0 Louvain iterations",
+            "vertices": nv, "arcs": ne, "nbatches": args.batches, "tol": args.tol,
+            "phase0_restarts_in_timed_region": state["resets"],
+            "tuning": g.get_tuning(),
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "launches": d["launches"], "avg_launch_us": round(d["ms"] * 1e3 / max(d["launches"], 1), 3),
+            "algorithmic_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1), 1),
+        },
+        "roofline_iter": {
+            "bytes_per_iter": b_iter, "achieved": round(ach_iter, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach_iter / HBM_PEAK_GBS, 4),
+        },
+        "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in ks.items()},
+    }
+    g.close()
+
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(off, col, w, args.batches, args.tol, args.cpu_iters,
+                                           args.cpu_budget, state["first_q"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

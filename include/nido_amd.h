@@ -72,9 +72,13 @@ typedef struct {
 /* Knobs of the degree-binned local-moving kernels (results do not depend
  * on them; only speed does). */
 typedef struct {
-    int32_t thread_max_deg;   /* rows up to this length: one lane per vertex (<= 8) */
+    int32_t thread_max_deg;   /* rows up to this length: one lane per vertex (0..8) */
+    int32_t big_min_deg;      /* longer rows: hash-partitioned workgroups (0..512) */
+    int32_t big_part_arcs;    /* distinct communities expected per partition, at most (>= 1) */
+    int32_t big_key_limit;    /* LDS keys per partition before the exact HBM fallback (1..big_slots/2) */
     int32_t use_graph;        /* capture each iteration in a hipGraph (0/1) */
-    int32_t reserved[6];
+    int32_t big_slots;        /* LDS table of a long-row workgroup: 2048 or 4096 slots */
+    int32_t reserved[2];
 } nido_tuning;
 
 const char* nido_last_error(void);
@@ -179,6 +183,27 @@ int nido_get_clustering(nido_graph* g, int64_t* comm);
 /* GraphGPU::dump_partition / Clustering::dump_partition, clustering.cpp:23-46:
  * one "%lld\n" line per original vertex. */
 int nido_dump_partition(nido_graph* g, const char* path);
+
+/* One pass of the LouvainGPU::run loop body (louvain_gpu.cpp:55-90): every
+ * batch's louvain_update / update_community_weights / update_community_ids,
+ * then compute_modularity; if the new modularity is below *q the iteration is
+ * rolled back (restore_community), otherwise *q takes the new value.  *dq
+ * receives Qnew - Q and *q_eval (if not NULL) Qnew itself. */
+int nido_louvain_iteration(nido_graph* g, double* q, double* dq, double* q_eval);
+
+/* Live per-kernel timing with HIP events on the handle's stream (off by
+ * default).  Kernel classes: */
+#define NIDO_K_LM_SHORT    0   /* short-row bins + long-row key fill, one launch per batch */
+#define NIDO_K_LM_LONG     1   /* hash-partitioned long rows, one launch per batch */
+#define NIDO_K_COMMIT      2   /* community-total update + swap, one launch per batch */
+#define NIDO_K_MODULARITY  3   /* modularity sweep + finish, one pair per evaluation */
+#define NIDO_K_CLASSES     4
+int nido_graph_set_profiling(nido_graph* g, int enable);
+/* Launch count, summed duration (ms) and algorithmic HBM bytes of one class
+ * since profiling was enabled (24 B per arc + 40 B per vertex for local
+ * moving, 24 B per arc + 24 B per vertex for modularity, 0 for commits). */
+int nido_graph_kernel_stats(nido_graph* g, int kernel_class, int64_t* launches, double* ms,
+                            double* bytes);
 
 /* ---- driver: LouvainGPU (louvain_gpu.hpp:15-23) ------------------------- */
 

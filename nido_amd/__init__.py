@@ -59,8 +59,12 @@ class RunStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("thread_max_deg", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+    _fields_ = [("thread_max_deg", C.c_int32), ("big_min_deg", C.c_int32),
+                ("big_part_arcs", C.c_int32), ("big_key_limit", C.c_int32),
+                ("use_graph", C.c_int32), ("big_slots", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 def _sig(name, res, *args):
@@ -107,8 +111,13 @@ _sig("nido_get_community_weights", C.c_int, C.c_void_p, _pdbl)
 _sig("nido_get_graph", C.c_int, C.c_void_p, _pi64, _pi64, _pdbl)
 _sig("nido_get_clustering", C.c_int, C.c_void_p, _pi64)
 _sig("nido_dump_partition", C.c_int, C.c_void_p, C.c_char_p)
+_sig("nido_louvain_iteration", C.c_int, C.c_void_p, _pdbl, _pdbl, _pdbl)
+_sig("nido_graph_set_profiling", C.c_int, C.c_void_p, C.c_int)
+_sig("nido_graph_kernel_stats", C.c_int, C.c_void_p, C.c_int, _pi64, _pdbl, _pdbl)
 _sig("nido_louvain_run", C.c_int, C.c_void_p, _i64, C.c_double, C.c_double, C.c_int, C.c_int,
      C.POINTER(RunStats))
+
+KERNEL_CLASSES = ("lm_short", "lm_long", "commit", "modularity")   # NIDO_K_* order
 _sig("nido_run_phase", C.c_int, C.c_void_p, C.c_int, C.POINTER(PhaseStats))
 _sig("nido_run_iteration_q", C.c_int, C.c_void_p, C.c_int, _pdbl)
 _sig("nido_run_phase_communities", C.c_int, C.c_void_p, C.c_int, _pi64)
@@ -286,13 +295,20 @@ class GraphGPU:
         _check(lib.nido_graph_batch_bounds(self._h, _p64(out)), "nido_graph_batch_bounds")
         return out
 
-    def set_tuning(self, thread_max_deg: int | None = None, use_graph: int | None = None) -> None:
+    def get_tuning(self) -> dict:
         t = Tuning()
         _check(lib.nido_graph_get_tuning(self._h, C.byref(t)), "nido_graph_get_tuning")
-        if thread_max_deg is not None:
-            t.thread_max_deg = thread_max_deg
-        if use_graph is not None:
-            t.use_graph = use_graph
+        return t.as_dict()
+
+    def set_tuning(self, **kw) -> None:
+        """Launch-shape knobs (see nido_tuning in include/nido_amd.h); the
+        results do not depend on them."""
+        t = Tuning()
+        _check(lib.nido_graph_get_tuning(self._h, C.byref(t)), "nido_graph_get_tuning")
+        for k, v in kw.items():
+            if k not in t.as_dict():
+                raise KeyError(k)
+            setattr(t, k, int(v))
         _check(lib.nido_graph_set_tuning(self._h, C.byref(t)), "nido_graph_set_tuning")
 
     def sync(self) -> None:
@@ -323,6 +339,26 @@ class GraphGPU:
         done = C.c_int()
         _check(lib.nido_aggregation(self._h, C.byref(done)), "nido_aggregation")
         return bool(done.value)
+
+    def louvain_iteration(self, q: float):
+        """One loop-body pass (all batches, modularity, accept/rollback):
+        returns (accepted Q, dQ, evaluated Q)."""
+        qq, dq, qe = C.c_double(q), C.c_double(), C.c_double()
+        _check(lib.nido_louvain_iteration(self._h, C.byref(qq), C.byref(dq), C.byref(qe)),
+               "nido_louvain_iteration")
+        return qq.value, dq.value, qe.value
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib.nido_graph_set_profiling(self._h, int(on)), "nido_graph_set_profiling")
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            n, ms, b = C.c_int64(), C.c_double(), C.c_double()
+            _check(lib.nido_graph_kernel_stats(self._h, i, C.byref(n), C.byref(ms), C.byref(b)),
+                   "nido_graph_kernel_stats")
+            out[name] = {"launches": n.value, "ms": ms.value, "bytes": b.value}
+        return out
 
     # ---- readers ----
     def communities(self) -> np.ndarray:

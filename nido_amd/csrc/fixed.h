@@ -106,6 +106,20 @@ __device__ __forceinline__ void fx_atomic_add(unsigned long long* lo, unsigned l
     if (vh) atomicAdd(hi, vh);
 }
 
+// Same, with a 32-bit high word: exact while |value| < 2^31 (in units of 1),
+// which holds for every partial sum of a row whose weights are non-negative
+// and sum to less than 2^31.
+__device__ __forceinline__ void fx_atomic_add_narrow(unsigned long long* lo, int* hi, fx_t v)
+{
+    unsigned long long vl = (unsigned long long)fx_lo(v);
+    int vh = (int)fx_hi(v);
+    if (vl) {
+        unsigned long long old = atomicAdd(lo, vl);
+        vh += (old + vl) < old;
+    }
+    if (vh) atomicAdd(hi, vh);
+}
+
 // Lane-exchange helpers for 128-bit values within a wavefront.
 __device__ __forceinline__ fx_t fx_shfl_xor(fx_t x, int mask, int width)
 {

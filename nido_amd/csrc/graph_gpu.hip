@@ -69,6 +69,85 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Per-kernel-class timing with HIP events on the handle's stream.
+struct Prof {
+    struct Entry {
+        int cls;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<Entry> pending;
+    int64_t launches[NIDO_K_CLASSES] = {};
+    double ms[NIDO_K_CLASSES] = {};
+    double bytes[NIDO_K_CLASSES] = {};
+
+    hipEvent_t get()
+    {
+        if (pool.empty()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    // Folds every finished interval into the totals (the stream has been
+    // synchronised by the caller).
+    void harvest()
+    {
+        for (auto& p : pending) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+                launches[p.cls]++;
+                ms[p.cls] += t;
+                bytes[p.cls] += p.bytes;
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset()
+    {
+        for (int i = 0; i < NIDO_K_CLASSES; ++i) {
+            launches[i] = 0;
+            ms[i] = 0;
+            bytes[i] = 0;
+        }
+    }
+    ~Prof()
+    {
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Brackets the launches issued in its lifetime when profiling is on.
+struct ProfScope {
+    Prof& P;
+    hipStream_t s;
+    int cls;
+    double bytes;
+    hipEvent_t a = nullptr;
+    ProfScope(Prof& p, hipStream_t st, int c, double b) : P(p), s(st), cls(c), bytes(b)
+    {
+        if (P.on && (a = P.get())) (void)hipEventRecord(a, s);
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        hipEvent_t b = P.get();
+        if (b && hipEventRecord(b, s) == hipSuccess) P.pending.push_back({cls, a, b, bytes});
+        else P.pool.push_back(a);
+    }
+};
+
 }  // namespace
 
 struct nido_graph {
@@ -95,17 +174,32 @@ struct nido_graph {
     double* d_kd = nullptr;
     int64_t* d_clus = nullptr;
     int64_t* d_work = nullptr;     // n0+1 int64 scratch (renumbering)
+    int* d_src = nullptr;          // row of every arc, capacity = phase-0 arcs
+    fx_t* d_part = nullptr;        // 2 * kPartBlocks reduction partials
     void* d_scan_tmp = nullptr;
     // scalars
     fx_t* d_acc = nullptr;         // [0] internal weight, [1] sum of squares, [2] 2m
     double* d_q = nullptr;
     int64_t* d_tot = nullptr;      // [2] scan totals
     double* h_q = nullptr;         // pinned
-    // local-moving bins
+    // local-moving bins: per batch, the fused small-row launch and the
+    // partitioned long-row launch
     int* d_binv = nullptr;
-    std::vector<int64_t> binoff;   // nb * (NUM_BINS + 1)
-    int64_t* d_htoff = nullptr;
-    std::vector<int64_t> htoff_base;
+    std::vector<SmallBins> small_bins;   // nb
+    std::vector<BigArgs> big_args;       // nb
+    BigItem* d_big_items = nullptr;
+    KeyChunk* d_key_chunks = nullptr;
+    int* d_ckey = nullptr;
+    int64_t key_cap = 0;
+    int64_t* d_big_poff = nullptr;
+    int64_t* d_big_toff = nullptr;
+    double* d_pg = nullptr;
+    int* d_pc = nullptr;
+    fx_t* d_pself = nullptr;
+    int64_t part_cap = 0;
+    int* d_cnt = nullptr;
+    int* d_ovf = nullptr;
+    int64_t slot_cap = 0;
     int* d_tkey = nullptr;
     fx_t* d_tval = nullptr;
     int64_t tab_cap = 0;
@@ -117,7 +211,11 @@ struct nido_graph {
 
     fx_t twom = 0;
     double twom_d = 0, m = 0;
+    bool nonneg = true;            // every weight >= 0 (coarse weights are sums of these)
     std::vector<int> committed;    // per batch: weights applied since its last update
+
+    std::vector<double> short_bytes, long_bytes;   // per batch, algorithmic
+    Prof prof;
 
     std::vector<PhaseHist> hist;
     nido_run_stats last{};
@@ -127,10 +225,22 @@ namespace {
 
 int bin_of(int64_t d, const nido_tuning& t)
 {
+    if (d > t.big_min_deg) return BIN_BIG;
     if (d <= t.thread_max_deg) return BIN_THREAD;
-    for (int b = BIN_G8; b < NUM_BINS; ++b)
+    for (int b = BIN_G8; b < NUM_SMALL_BINS; ++b)
         if (d <= kBinMaxDeg[b]) return b;
-    return BIN_HUGE;
+    return BIN_BIG;
+}
+
+void default_tuning(nido_tuning& t)
+{
+    t = nido_tuning{};
+    t.thread_max_deg = kThreadMaxDeg;
+    t.big_min_deg = (int32_t)kBinMaxDeg[BIN_B256];
+    t.big_part_arcs = (int32_t)kBigPartArcs;
+    t.big_key_limit = kBigLimit;
+    t.use_graph = 0;
+    t.big_slots = kBigSlots;
 }
 
 // partition_graph_edge_batch with part_on_batch=1 on one device,
@@ -190,46 +300,135 @@ int setup_phase(nido_graph* g)
     batch_bounds(off, nv, g->nb, g->parts);
     g->committed.assign((size_t)g->nb, 0);
 
-    // vertices grouped by (batch, bin), ascending id inside a group
-    const int NB1 = NUM_BINS + 1;
-    g->binoff.assign((size_t)g->nb * NB1, 0);
+    // Short rows: vertices grouped by (batch, bin), ascending id inside a
+    // group.  Long rows: per batch, P = 2^ceil(log2(d / kBigPartArcs))
+    // partition items, their partial slots and HBM fallback tables.
     std::vector<int> binv((size_t)std::max<int64_t>(nv, 1));
-    std::vector<int64_t> htoff;
-    g->htoff_base.assign((size_t)g->nb, 0);
-    int64_t pos = 0, tab_need = 0;
+    std::vector<BigItem> bitems;
+    std::vector<KeyChunk> chunks;
+    std::vector<int64_t> poff, toff;
+    std::vector<int64_t> item_base((size_t)g->nb), poff_base((size_t)g->nb), toff_base((size_t)g->nb),
+        chunk_base((size_t)g->nb + 1);
+    std::vector<int64_t> small_start((size_t)g->nb * NUM_SMALL_BINS), small_n((size_t)g->nb * NUM_SMALL_BINS);
+    int64_t pos = 0, tab_need = 0, part_need = 0, slot_need = 0, key_need = 0;
+    g->short_bytes.assign((size_t)g->nb, 0.0);
+    g->long_bytes.assign((size_t)g->nb, 0.0);
     for (int b = 0; b < g->nb; ++b) {
+        chunk_base[(size_t)b] = (int64_t)chunks.size();
+        int64_t ksum = 0;
         int64_t cnt[NUM_BINS] = {0};
         for (int64_t v = g->parts[b]; v < g->parts[b + 1]; ++v) cnt[bin_of(off[v + 1] - off[v], g->tuning)]++;
-        int64_t cur[NUM_BINS];
-        for (int k = 0; k < NUM_BINS; ++k) {
-            g->binoff[(size_t)b * NB1 + k] = pos;
+        int64_t cur[NUM_SMALL_BINS];
+        for (int k = 0; k < NUM_SMALL_BINS; ++k) {
+            small_start[(size_t)b * NUM_SMALL_BINS + k] = pos;
+            small_n[(size_t)b * NUM_SMALL_BINS + k] = cnt[k];
             cur[k] = pos;
             pos += cnt[k];
         }
-        g->binoff[(size_t)b * NB1 + NUM_BINS] = pos;
-        g->htoff_base[(size_t)b] = (int64_t)htoff.size();
-        int64_t tsum = 0;
-        htoff.push_back(0);
+        item_base[(size_t)b] = (int64_t)bitems.size();
+        poff_base[(size_t)b] = (int64_t)poff.size();
+        toff_base[(size_t)b] = (int64_t)toff.size();
+        int64_t psum = 0, tsum = 0;
+        int slot = 0;
+        toff.push_back(0);
         for (int64_t v = g->parts[b]; v < g->parts[b + 1]; ++v) {
             const int64_t d = off[v + 1] - off[v];
             const int k = bin_of(d, g->tuning);
-            binv[(size_t)cur[k]++] = (int)v;
-            if (k == BIN_HUGE) {
-                tsum += pow2_at_least(2 * d);
-                htoff.push_back(tsum);
+            // algorithmic bytes of the sweep: 24 per arc (col, weight,
+            // C[col]) + 40 per vertex (offset, C, k, a_C, C')
+            (k != BIN_BIG ? g->short_bytes : g->long_bytes)[(size_t)b] += 24.0 * (double)d + 40.0;
+            if (k != BIN_BIG) {
+                binv[(size_t)cur[k]++] = (int)v;
+                continue;
             }
+            int P = 1;
+            while (P < 65536 && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
+            for (int p = 0; p < P; ++p) bitems.push_back(BigItem{(int)v, p, P, slot, ksum});
+            for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs)
+                chunks.push_back(KeyChunk{(int)v, e, std::min(off[v + 1], e + kKeyChunkArcs), ksum + (e - off[v])});
+            ksum += d;
+            poff.push_back(psum);
+            psum += P;
+            tsum += pow2_at_least(2 * d);
+            toff.push_back(tsum);
+            ++slot;
         }
+        // longest rows first: their partition workgroups are the long poles
+        std::stable_sort(bitems.begin() + item_base[(size_t)b], bitems.end(),
+                         [&](const BigItem& x, const BigItem& y) {
+                             return off[x.v + 1] - off[x.v] > off[y.v + 1] - off[y.v];
+                         });
         tab_need = std::max(tab_need, tsum);
+        part_need = std::max(part_need, psum);
+        slot_need = std::max<int64_t>(slot_need, slot);
+        key_need = std::max(key_need, ksum);
     }
+    chunk_base[(size_t)g->nb] = (int64_t)chunks.size();
     NIDO_TRY(upload(&g->d_binv, binv, g->stream));
-    if (htoff.empty()) htoff.push_back(0);
-    NIDO_TRY(upload(&g->d_htoff, htoff, g->stream));
+    NIDO_TRY(upload(&g->d_big_items, bitems, g->stream));
+    NIDO_TRY(upload(&g->d_key_chunks, chunks, g->stream));
+    if (key_need > g->key_cap) {
+        dev_free(g->d_ckey);
+        NIDO_TRY(dev_alloc(&g->d_ckey, key_need));
+        g->key_cap = key_need;
+    }
+    NIDO_TRY(upload(&g->d_big_poff, poff, g->stream));
+    NIDO_TRY(upload(&g->d_big_toff, toff, g->stream));
     if (tab_need > g->tab_cap) {
         dev_free(g->d_tkey);
         dev_free(g->d_tval);
         NIDO_TRY(dev_alloc(&g->d_tkey, tab_need));
         NIDO_TRY(dev_alloc(&g->d_tval, tab_need));
         g->tab_cap = tab_need;
+    }
+    if (part_need > g->part_cap) {
+        dev_free(g->d_pg);
+        dev_free(g->d_pc);
+        dev_free(g->d_pself);
+        NIDO_TRY(dev_alloc(&g->d_pg, part_need));
+        NIDO_TRY(dev_alloc(&g->d_pc, part_need));
+        NIDO_TRY(dev_alloc(&g->d_pself, part_need));
+        g->part_cap = part_need;
+    }
+    if (slot_need > g->slot_cap) {
+        dev_free(g->d_cnt);
+        dev_free(g->d_ovf);
+        NIDO_TRY(dev_alloc(&g->d_cnt, slot_need));
+        NIDO_TRY(dev_alloc(&g->d_ovf, slot_need));
+        launch_fill_i32(g->d_cnt, 0, slot_need, g->stream);   // the kernels keep them zero afterwards
+        launch_fill_i32(g->d_ovf, 0, slot_need, g->stream);
+        g->slot_cap = slot_need;
+    }
+    const size_t nbig_items = bitems.size();
+    g->small_bins.assign((size_t)g->nb, SmallBins{});
+    g->big_args.assign((size_t)g->nb, BigArgs{});
+    for (int b = 0; b < g->nb; ++b) {
+        SmallBins& L = g->small_bins[(size_t)b];
+        for (int k = 0; k < NUM_SMALL_BINS; ++k) {
+            L.verts[k] = g->d_binv + small_start[(size_t)b * NUM_SMALL_BINS + k];
+            L.n[k] = small_n[(size_t)b * NUM_SMALL_BINS + k];
+        }
+        L.chunks = g->d_key_chunks + chunk_base[(size_t)b];
+        L.nchunks = chunk_base[(size_t)b + 1] - chunk_base[(size_t)b];
+        L.ckey = g->d_ckey;
+        small_bins_blocks(L, 4096);
+        BigArgs& B = g->big_args[(size_t)b];
+        const int64_t i0 = item_base[(size_t)b];
+        const int64_t i1 = b + 1 < g->nb ? item_base[(size_t)b + 1] : (int64_t)nbig_items;
+        B.items = g->d_big_items + i0;
+        B.nitems = i1 - i0;
+        B.ckey = g->d_ckey;
+        B.key_limit = g->tuning.big_key_limit;
+        B.slots = g->tuning.big_slots;
+        B.poff = g->d_big_poff + poff_base[(size_t)b];
+        B.pg = g->d_pg;
+        B.pc = g->d_pc;
+        B.pself = g->d_pself;
+        B.cnt = g->d_cnt;
+        B.ovf = g->d_ovf;
+        B.toff = g->d_big_toff + toff_base[(size_t)b];
+        B.tkey = g->d_tkey;
+        B.tval = g->d_tval;
     }
 
     std::vector<int> small;
@@ -248,16 +447,25 @@ int setup_phase(nido_graph* g)
     NIDO_TRY(upload(&g->d_small, small, g->stream));
     NIDO_TRY(upload(&g->d_items, items, g->stream));
 
+    launch_row_ids(g->d_off, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_src, g->stream);
+    unsigned long long* d_maxk = reinterpret_cast<unsigned long long*>(g->d_tot + 1);
     launch_vertex_weights(g->d_off, g->d_w, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_k,
-                          g->d_kd, nv, g->d_acc + 2, g->stream);
+                          g->d_kd, nv, g->d_acc + 2, d_maxk, g->d_part, g->stream);
     HIP_TRY(hipGetLastError());
     fx_t twom = 0;
+    unsigned long long maxk_bits = 0;
     HIP_TRY(hipMemcpyAsync(&twom, g->d_acc + 2, sizeof(fx_t), hipMemcpyDeviceToHost, g->stream));
-    launch_fill_fx(g->d_acc + 2, 1, g->stream);
+    HIP_TRY(hipMemcpyAsync(&maxk_bits, d_maxk, sizeof(maxk_bits), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->twom = twom;
     g->twom_d = fx_to_double(twom);
     g->m = g->twom_d * 0.5;                     // mass_ = 0.5 * sum(k), graph_cuda.cu:1855
+    // Long-row tables may use 32-bit high words when no partial sum of a row
+    // can reach 2^31: weights non-negative and every k_v < 2^31 (a rounded
+    // k_v below 2^31 - 2^8 leaves room for the rounding).
+    const double maxk = nido_bitsd(maxk_bits);
+    const int narrow = g->nonneg && maxk < 2147483648.0 - 256.0;
+    for (auto& B : g->big_args) B.narrow = narrow;
     return NIDO_OK;
 }
 
@@ -290,11 +498,15 @@ int do_modularity(nido_graph* g, double* q)
         *q = 0.0;
         return NIDO_OK;
     }
-    launch_modularity(g->d_off, g->d_col, g->d_w, g->d_C, g->d_a, g->nv, g->d_small, g->nsmall,
-                      g->d_items, g->nitems, g->twom_d, g->d_acc, g->d_q, g->stream);
+    {
+        ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
+        launch_modularity(g->d_src, g->d_col, g->d_w, g->d_C, g->ne, g->d_a, g->nv, g->twom_d,
+                          g->d_part, g->d_q, g->stream);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(g->h_q, g->d_q, sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
+    g->prof.harvest();
     *q = *g->h_q;
     return NIDO_OK;
 }
@@ -310,12 +522,14 @@ int do_lm(nido_graph* g, int b)
         return NIDO_OK;
     }
     LmArgs A{g->d_off, g->d_col, g->d_w, g->d_C, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m};
-    const int NB1 = NUM_BINS + 1;
-    for (int k = 0; k < NUM_BINS; ++k) {
-        const int64_t s = g->binoff[(size_t)b * NB1 + k], e = g->binoff[(size_t)b * NB1 + k + 1];
-        if (e > s)
-            launch_lm_bin(k, A, g->d_binv + s, e - s, g->d_htoff + g->htoff_base[(size_t)b], g->d_tkey,
-                          g->d_tval, g->stream);
+    // launch 1 fills the long rows' community keys that launch 2 reads
+    {
+        ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
+        launch_lm_small(A, g->small_bins[(size_t)b], g->stream);
+    }
+    if (g->big_args[(size_t)b].nitems > 0) {
+        ProfScope ps(g->prof, g->stream, NIDO_K_LM_LONG, g->long_bytes[(size_t)b]);
+        launch_lm_big(A, g->big_args[(size_t)b], g->stream);
     }
     HIP_TRY(hipGetLastError());
     return NIDO_OK;
@@ -323,8 +537,11 @@ int do_lm(nido_graph* g, int b)
 
 int do_commit(nido_graph* g, int b)
 {
-    launch_commit(g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_C, g->d_Cn, g->d_k, g->d_a,
-                  g->stream);
+    {
+        ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
+        launch_commit(g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_C, g->d_Cn, g->d_k, g->d_a,
+                      g->stream);
+    }
     HIP_TRY(hipGetLastError());
     g->committed[(size_t)b] = 1;
     return NIDO_OK;
@@ -334,6 +551,23 @@ int do_restore(nido_graph* g)
 {
     launch_commit(0, g->nv, g->d_C, g->d_Cn, g->d_k, g->d_a, g->stream);
     HIP_TRY(hipGetLastError());
+    return NIDO_OK;
+}
+
+// The loop body of LouvainGPU::run, louvain_gpu.cpp:55-90; *qt receives
+// the modularity evaluated after the batches (before any rollback).
+int do_iteration(nido_graph* g, double* q, double* dq, double* qt_out = nullptr)
+{
+    for (int b = 0; b < g->nb; ++b) {
+        NIDO_TRY(do_lm(g, b));
+        NIDO_TRY(do_commit(g, b));
+    }
+    double qt = 0;
+    NIDO_TRY(do_modularity(g, &qt));
+    if (qt_out) *qt_out = qt;
+    *dq = qt - *q;
+    if (*dq < 0) NIDO_TRY(do_restore(g));
+    else *q = qt;
     return NIDO_OK;
 }
 
@@ -438,17 +672,10 @@ int do_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, int r
         int64_t loops = 0;
         const double ts = now_s();
         while (tol < dQ && loops < max_loops) {   // louvain_gpu.cpp:53
-            for (int b = 0; b < g->nb; ++b) {
-                NIDO_TRY(do_lm(g, b));
-                NIDO_TRY(do_commit(g, b));
-            }
             double Qtmp = 0;
-            NIDO_TRY(do_modularity(g, &Qtmp));
-            dQ = Qtmp - Q;
+            NIDO_TRY(do_iteration(g, &Q, &dQ, &Qtmp));
             h.iter_q.push_back(Qtmp);
             loops++;
-            if (dQ < 0) NIDO_TRY(do_restore(g));
-            else Q = Qtmp;
             if (verbose) printf("%lld \t%g \t%g\n", (long long)loops, Qtmp, dQ);
         }
         HIP_TRY(hipStreamSynchronize(g->stream));
@@ -512,12 +739,14 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         if (off[i + 1] < off[i]) return set_error(NIDO_ERR_INVALID, "offsets decrease at %lld", (long long)i);
     const int64_t ne = off[nv] - base;
     if (ne > 0 && (!col || !w)) return set_error(NIDO_ERR_INVALID, "null cols/weights");
+    bool nonneg = true;
     for (int64_t e = 0; e < ne; ++e) {
         const int64_t c = col[base + e];
         if (c < 0 || c >= nv) return set_error(NIDO_ERR_INVALID, "col[%lld]=%lld out of range", (long long)e, (long long)c);
         const double x = w[base + e];
         if (!(x == x) || x >= 0x1p62 || x <= -0x1p62)
             return set_error(NIDO_ERR_INVALID, "weight[%lld] is not finite or |w| >= 2^62", (long long)e);
+        nonneg = nonneg && !(x < 0);
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -526,8 +755,8 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     }
     nido_graph* g = new nido_graph;
     g->nb = nb;
-    g->tuning.thread_max_deg = kThreadMaxDeg;
-    g->tuning.use_graph = 0;
+    g->nonneg = nonneg;
+    default_tuning(g->tuning);
     auto fail = [&](int rc) {
         nido_graph_destroy(g);
         return rc;
@@ -552,7 +781,8 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         (rc = dev_alloc(&g->d_a, nv)) || (rc = dev_alloc(&g->d_kd, nv)) ||
         (rc = dev_alloc(&g->d_clus, nv)) || (rc = dev_alloc(&g->d_work, nv + 1)) ||
         (rc = dev_alloc(&g->d_acc, 4)) || (rc = dev_alloc(&g->d_q, 1)) ||
-        (rc = dev_alloc(&g->d_tot, 2)))
+        (rc = dev_alloc(&g->d_tot, 2)) || (rc = dev_alloc(&g->d_src, ne)) ||
+        (rc = dev_alloc(&g->d_part, 2 * (int64_t)kPartBlocks)))
         return fail(rc);
     if (hipMalloc(&g->d_scan_tmp, scan_temp_bytes(nv + 1)) != hipSuccess)
         return fail(set_error(NIDO_ERR_NOMEM, "scan scratch allocation failed"));
@@ -624,7 +854,11 @@ int nido_graph_destroy(nido_graph* g)
     free_graph_arrays(g);
     dev_free(g->d_C); dev_free(g->d_Cn); dev_free(g->d_k); dev_free(g->d_a); dev_free(g->d_kd);
     dev_free(g->d_clus); dev_free(g->d_work); dev_free(g->d_acc); dev_free(g->d_q); dev_free(g->d_tot);
-    dev_free(g->d_binv); dev_free(g->d_htoff); dev_free(g->d_tkey); dev_free(g->d_tval);
+    dev_free(g->d_src); dev_free(g->d_part);
+    dev_free(g->d_binv); dev_free(g->d_big_items); dev_free(g->d_big_poff); dev_free(g->d_big_toff);
+    dev_free(g->d_key_chunks); dev_free(g->d_ckey);
+    dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt); dev_free(g->d_ovf);
+    dev_free(g->d_tkey); dev_free(g->d_tval);
     dev_free(g->d_small); dev_free(g->d_items);
     if (g->d_scan_tmp) (void)hipFree(g->d_scan_tmp);
     if (g->h_q) (void)hipHostFree(g->h_q);
@@ -655,6 +889,14 @@ int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
     if (!g || !t) return set_error(NIDO_ERR_INVALID, "null argument");
     if (t->thread_max_deg < 0 || t->thread_max_deg > kThreadMaxDeg)
         return set_error(NIDO_ERR_INVALID, "thread_max_deg must be in [0, %d]", kThreadMaxDeg);
+    if (t->big_min_deg < 0 || t->big_min_deg > kBinMaxDeg[BIN_B256])
+        return set_error(NIDO_ERR_INVALID, "big_min_deg must be in [0, %d]", (int)kBinMaxDeg[BIN_B256]);
+    if (t->big_part_arcs < 1)
+        return set_error(NIDO_ERR_INVALID, "big_part_arcs must be >= 1");
+    if (t->big_slots != 2048 && t->big_slots != 4096)
+        return set_error(NIDO_ERR_INVALID, "big_slots must be 2048 or 4096");
+    if (t->big_key_limit < 1 || t->big_key_limit > t->big_slots / 2)
+        return set_error(NIDO_ERR_INVALID, "big_key_limit must be in [1, big_slots/2]");
     API_GUARD(g->tuning = *t; return setup_phase(g);)
 }
 
@@ -770,6 +1012,34 @@ int nido_dump_partition(nido_graph* g, const char* path)
         for (int64_t i = 0; i < g->n0; ++i) fprintf(f, "%lld\n", (long long)c[(size_t)i]);
         fclose(f);
         return NIDO_OK;)
+}
+
+int nido_louvain_iteration(nido_graph* g, double* q, double* dq, double* q_eval)
+{
+    if (!g || !q || !dq) return set_error(NIDO_ERR_INVALID, "null argument");
+    API_GUARD(return do_iteration(g, q, dq, q_eval);)
+}
+
+int nido_graph_set_profiling(nido_graph* g, int enable)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    API_GUARD(g->prof.harvest();)
+    g->prof.reset();
+    g->prof.on = enable != 0;
+    return NIDO_OK;
+}
+
+int nido_graph_kernel_stats(nido_graph* g, int cls, int64_t* launches, double* ms, double* bytes)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (cls < 0 || cls >= NIDO_K_CLASSES) return set_error(NIDO_ERR_INVALID, "kernel class %d", cls);
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    API_GUARD(g->prof.harvest();)
+    if (launches) *launches = g->prof.launches[cls];
+    if (ms) *ms = g->prof.ms[cls];
+    if (bytes) *bytes = g->prof.bytes[cls];
+    return NIDO_OK;
 }
 
 int nido_louvain_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, int record,

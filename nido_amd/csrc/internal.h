@@ -6,7 +6,7 @@
 
 namespace nido {
 
-// Degree bins of the local-moving kernel.  Bin b holds the vertices whose
+// Degree bins of the local-moving kernels.  Bin b holds the vertices whose
 // row length d satisfies kBinMaxDeg[b-1] < d <= kBinMaxDeg[b] (defaults;
 // the tuning struct can move the thread-bin bound).
 enum LmBin : int {
@@ -16,12 +16,60 @@ enum LmBin : int {
     BIN_G32,          // 32 lanes, 128 slots
     BIN_G64,          // one wavefront, 256 slots
     BIN_B256,         // one 256-lane workgroup, 1024 slots
-    BIN_HUGE,         // one workgroup, open-addressing table in HBM
+    BIN_BIG,          // hash-partitioned rows, one workgroup per (row, partition)
     NUM_BINS
 };
+constexpr int NUM_SMALL_BINS = BIN_BIG;
 
 constexpr int kThreadMaxDeg = 8;          // registers hold up to this many arcs
 constexpr int64_t kBinMaxDeg[NUM_BINS] = {8, 16, 32, 64, 128, 512, INT64_MAX};
+constexpr int64_t kBigPartArcs = 1024;    // expected distinct keys per partition, at most
+constexpr int kBigSlots = 4096;           // LDS table of a partition workgroup (64/80 KB)
+constexpr int kBigLimit = kBigSlots / 2;  // distinct keys before the exact HBM fallback
+
+// Arcs [e0, e1) of long row v, whose community keys go to ckey[k0 ...].
+struct KeyChunk {
+    int v;
+    int64_t e0, e1, k0;
+};
+constexpr int64_t kKeyChunkArcs = 2048;
+
+// The first launch of a batch: the small bins plus the key fill of the
+// batch's long rows (ckey[k] = C[col] as int, -1 for a self arc).
+struct SmallBins {
+    const int* verts[NUM_SMALL_BINS];
+    int64_t n[NUM_SMALL_BINS];
+    const KeyChunk* chunks;
+    int64_t nchunks;
+    int* ckey;
+    int blk0[NUM_SMALL_BINS + 2];          // first block of each bin, then of the key fill
+};
+
+// One workgroup's share of a long row: partition p of P (P a power of 2);
+// `slot` indexes the row within its batch's list of long rows; the row's
+// keys start at ckey[k0].
+struct BigItem {
+    int v, p, P, slot;
+    int64_t k0;
+};
+
+struct BigArgs {
+    const BigItem* items;
+    int64_t nitems;
+    const int* ckey;         // community keys of the batch's long rows
+    int key_limit;           // distinct keys a partition may hold in LDS (<= slots/2)
+    int narrow;              // every row's weights are >= 0 and sum below 2^31
+    int slots;               // LDS table size of a partition workgroup: 2048 or 4096
+    const int64_t* poff;     // per slot: first partial
+    double* pg;              // partial best gain
+    int* pc;                 // partial best community
+    fx_t* pself;             // partial e_{v,own}
+    int* cnt;                // per slot: partitions finished (reset by the last one)
+    int* ovf;                // per slot: some partition outgrew its LDS table
+    const int64_t* toff;     // per slot: HBM fallback table [toff[s], toff[s+1])
+    int* tkey;
+    fx_t* tval;
+};
 
 // Everything one local-moving launch reads or writes.
 struct LmArgs {
@@ -47,25 +95,35 @@ void launch_fill_i64(int64_t* p, int64_t val, int64_t n, hipStream_t s);
 void launch_fill_i32(int32_t* p, int32_t val, int64_t n, hipStream_t s);
 void launch_fill_fx(fx_t* p, int64_t n, hipStream_t s);
 
+// Blocks of the reduction kernels, each leaving one partial sum.
+constexpr unsigned kPartBlocks = 1024;
+
 // k_v and 2m.  `small` lists rows handled by one lane, `items` slices of
-// longer rows (one wavefront each).  acc receives sum(k) exactly.
+// longer rows (one wavefront each).  *acc receives sum(k) exactly, *maxabs
+// the bits of max |FP64(k_v)|; part holds kPartBlocks partials.
 void launch_vertex_weights(const int64_t* off, const double* w, const int* small, int64_t nsmall,
                            const RowItem* items, int64_t nitems, fx_t* k, double* kd,
-                           int64_t nv, fx_t* acc, hipStream_t s);
+                           int64_t nv, fx_t* acc, unsigned long long* maxabs, fx_t* part,
+                           hipStream_t s);
+// src[e] = row of arc e.
+void launch_row_ids(const int64_t* off, const int* small, int64_t nsmall, const RowItem* items,
+                    int64_t nitems, int* src, hipStream_t s);
 void launch_singleton(int64_t* C, int64_t* Cn, fx_t* a, const fx_t* k, int64_t nv, hipStream_t s);
 
-void launch_lm_bin(int bin, const LmArgs& A, const int* verts, int64_t n,
-                   const int64_t* tab_off, int* tab_key, fx_t* tab_val, hipStream_t s);
+// Fills L.blk0 from L.n and L.nchunks; returns the number of blocks of the
+// fused launch.
+int small_bins_blocks(SmallBins& L, unsigned cap_per_bin);
+void launch_lm_small(const LmArgs& A, const SmallBins& L, hipStream_t s);
+void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s);
 
 void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, const fx_t* k, fx_t* a,
                    hipStream_t s);
 
-// Modularity: acc[0] += internal weight, acc[1] += sum_c (a_c/2m)^2; the
-// finalize kernel writes Q to *q and clears acc for the next call.
-void launch_modularity(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
-                       const fx_t* a, int64_t nv, const int* small, int64_t nsmall,
-                       const RowItem* items, int64_t nitems, double twom_d, fx_t* acc,
-                       double* q, hipStream_t s);
+// Modularity over the arc-parallel CSR (src = row ids): writes Q to *q;
+// part holds 2 * kPartBlocks partials.
+void launch_modularity(const int* src, const int64_t* col, const double* w, const int64_t* C,
+                       int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
+                       hipStream_t s);
 
 // Exclusive scan of n int64 values (out may alias in); writes the total to
 // *total (device) when total is non-null.

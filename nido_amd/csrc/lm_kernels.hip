@@ -1,0 +1,557 @@
+// lm_kernels.hip -- local-moving kernels (one -b batch per launch pair).
+//
+// Reference path being replaced (sg0/nido): fill_edges_community_ids +
+// copy_weights_exclude_self_loops + Thrust stable_sort_by_key/reduce_by_key
+// + fill_unique_community_counts + inclusive_scan + louvain_update_kernel
+// (graph_gpu.cpp:457-517, graph_cuda.cu:49-98, 1450-1611, 2289-2348).
+// Here each row is scanned coalesced over CSR; C[col] is gathered, the
+// weight converted to Q64.64 and summed in an open-addressing table keyed by
+// community id; the FP64 gain argmax (min-id ties) is taken over the table.
+//
+// Degree bins (rows of d arcs):
+//   d <= 8      one lane per vertex, neighbours in registers
+//   d <= 16/32/64/128   8/16/32/64 lanes per vertex, LDS table of 2d..4d slots
+//   d <= 512    one workgroup per vertex, 1024-slot LDS table
+//   d >  512    long rows.  Launch 1 stores each arc's community as a 4-byte
+//               key (ckey).  Launch 2 splits the community ids into P hash
+//               partitions: workgroup (v, p) streams the row's keys (L2
+//               resident, coalesced) and aggregates the arcs whose community
+//               is in partition p, so every e_{v,c} is complete and exact in
+//               one workgroup's LDS table; the last of the P workgroups to
+//               finish combines the P partial argmaxes.
+// Launch 1 = short-row bins + long-row key fill (20 KB LDS per workgroup);
+// launch 2 = long-row partitions (80 KB LDS).  Row loops keep two to eight
+// independent loads in flight per lane.
+#include "device_common.h"
+
+namespace nido {
+namespace {
+
+// g_c = (e_{v,c} - k_v a_c / (2m)) / m, graph_cuda.cu:1505-1506.
+__device__ __forceinline__ double gain_of(const LmArgs& A, double ki, fx_t e, int c)
+{
+    double g = fx_to_double(e);
+    const double ac = fx_to_double(A.a[c]);
+    g -= ki * ac / (2. * A.m);
+    g /= A.m;
+    return g;
+}
+
+// Move iff g_best - s' > 0 with s' = (e_{v,own} - k_v (a_own - k_v)/(2m))/m,
+// graph_cuda.cu:1527-1536.
+__device__ __forceinline__ int64_t decide(const LmArgs& A, int v, int own, fx_t self,
+                                          double best_g, int best_c)
+{
+    if (best_c == kNone) return own;    // gain stays -MAX_FLOAT, graph_cuda.cu:1489
+    const double m = A.m;
+    const double ki = A.kd[v];
+    double s = fx_to_double(self);
+    const double ar = fx_to_double(A.a[own] - A.k[v]);
+    s -= ki * ar / (2. * m);
+    s /= m;
+    double gain = best_g;
+    gain -= s;
+    return gain > 0 ? (int64_t)best_c : (int64_t)own;
+}
+
+// Candidate update for one table entry.
+__device__ __forceinline__ void consider(const LmArgs& A, double ki, int own, int c, fx_t e,
+                                         fx_t& self, double& bg, int& bc)
+{
+    if (c == own) {
+        self = e;
+        return;
+    }
+    const double g = gain_of(A, ki, e, c);
+    if (better(g, c, bg, bc)) {
+        bg = g;
+        bc = c;
+    }
+}
+
+// Insert (c, w) into an LDS open-addressing table of S slots (S a power of
+// 2, never more than half full by construction of the bins).
+template <int S>
+__device__ __forceinline__ void lds_insert(int* key, unsigned long long* lo, unsigned long long* hi,
+                                           int c, fx_t wf)
+{
+    unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(S));
+    for (;;) {
+        const int kk = key[h];
+        if (kk == c) break;
+        if (kk < 0) {
+            const int prev = atomicCAS(&key[h], -1, c);
+            if (prev < 0 || prev == c) break;
+        }
+        h = (h + 1) & (S - 1);
+    }
+    fx_atomic_add(&lo[h], &hi[h], wf);
+}
+
+// Scans row v with `lanes` lanes (lane index `lane`), two arcs per lane per
+// step, inserting every non-self arc into the S-slot table.
+template <int S>
+__device__ __forceinline__ void scan_row(const LmArgs& A, int v, int lane, int lanes, int* key,
+                                         unsigned long long* lo, unsigned long long* hi)
+{
+    const int64_t e1 = A.off[v + 1];
+    for (int64_t e = A.off[v] + lane; e < e1; e += 2 * (int64_t)lanes) {
+        const int64_t f = e + lanes;
+        const bool two = f < e1;
+        const int u1 = (int)A.col[e];
+        const int u2 = two ? (int)A.col[f] : v;
+        const double w1 = A.w[e];
+        const double w2 = two ? A.w[f] : 0.0;
+        const int c1 = u1 != v ? (int)A.C[u1] : -1;     // self arcs weigh 0 (graph_cuda.cu:2312-2315)
+        const int c2 = u2 != v ? (int)A.C[u2] : -1;
+        if (c1 >= 0) lds_insert<S>(key, lo, hi, c1, fx_from_double(w1));
+        if (c2 >= 0) lds_insert<S>(key, lo, hi, c2, fx_from_double(w2));
+    }
+}
+
+// ---- launch 1 bodies (blocks [bi, bi+nbk) of the fused launch) -------------
+
+__device__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk)
+{
+    for (int64_t i = (int64_t)bi * kBlock + threadIdx.x; i < n; i += (int64_t)nbk * kBlock) {
+        const int v = verts[i];
+        const int own = (int)A.C[v];
+        const int64_t e0 = A.off[v];
+        const int d = (int)(A.off[v + 1] - e0);
+        int uu[kThreadMaxDeg];
+        double wd[kThreadMaxDeg];
+#pragma unroll
+        for (int j = 0; j < kThreadMaxDeg; ++j) {
+            uu[j] = j < d ? (int)A.col[e0 + j] : v;
+            wd[j] = j < d ? A.w[e0 + j] : 0.0;
+        }
+        int cc[kThreadMaxDeg];
+#pragma unroll
+        for (int j = 0; j < kThreadMaxDeg; ++j) cc[j] = uu[j] != v ? (int)A.C[uu[j]] : -1;
+        const double ki = A.kd[v];
+        fx_t self = 0;
+        double bg = -kMaxFloat;
+        int bc = kNone;
+#pragma unroll
+        for (int j = 0; j < kThreadMaxDeg; ++j) {
+            bool first = cc[j] >= 0;
+#pragma unroll
+            for (int q = 0; q < j; ++q) first = first && (cc[q] != cc[j]);
+            if (!first) continue;
+            fx_t e = fx_from_double(wd[j]);
+#pragma unroll
+            for (int q = j + 1; q < kThreadMaxDeg; ++q)
+                if (cc[q] == cc[j]) e += fx_from_double(wd[q]);
+            consider(A, ki, own, cc[j], e, self, bg, bc);
+        }
+        A.Cn[v] = decide(A, v, own, self, bg, bc);
+    }
+}
+
+// G lanes per vertex (G <= 64), S-slot LDS table per vertex; 256/G vertices
+// per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
+template <int G, int S>
+__device__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
+                           char* lds)
+{
+    constexpr int NG = kBlock / G;
+    int* skey = reinterpret_cast<int*>(lds);
+    unsigned long long* slo = reinterpret_cast<unsigned long long*>(lds + NG * S * 4);
+    unsigned long long* shi = slo + NG * S;
+    const int grp = threadIdx.x / G;
+    const int lane = threadIdx.x & (G - 1);
+    int* key = skey + grp * S;
+    unsigned long long* lo = slo + grp * S;
+    unsigned long long* hi = shi + grp * S;
+
+    for (int64_t base = (int64_t)bi * NG; base < n; base += (int64_t)nbk * NG) {
+        for (int s = lane; s < S; s += G) {
+            key[s] = -1;
+            lo[s] = 0;
+            hi[s] = 0;
+        }
+        __syncthreads();
+        const int64_t idx = base + grp;
+        const bool act = idx < n;
+        const int v = act ? verts[idx] : 0;
+        if (act) scan_row<S>(A, v, lane, G, key, lo, hi);
+        __syncthreads();
+        if (act) {   // uniform within the group
+            const int own = (int)A.C[v];
+            const double ki = A.kd[v];
+            fx_t self = 0;
+            double bg = -kMaxFloat;
+            int bc = kNone;
+            for (int s = lane; s < S; s += G) {
+                const int c = key[s];
+                if (c >= 0) consider(A, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+            }
+#pragma unroll
+            for (int msk = G / 2; msk > 0; msk >>= 1) {
+                const double og = __shfl_xor(bg, msk, G);
+                const int oc = __shfl_xor(bc, msk, G);
+                self += fx_shfl_xor(self, msk, G);
+                if (better(og, oc, bg, bc)) {
+                    bg = og;
+                    bc = oc;
+                }
+            }
+            if (lane == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per vertex, S-slot LDS table.
+template <int S>
+__device__ void block_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
+                           char* lds, BlockRed& R)
+{
+    int* key = reinterpret_cast<int*>(lds);
+    unsigned long long* lo = reinterpret_cast<unsigned long long*>(lds + S * 4);
+    unsigned long long* hi = lo + S;
+    for (int64_t idx = bi; idx < n; idx += nbk) {
+        for (int s = threadIdx.x; s < S; s += kBlock) {
+            key[s] = -1;
+            lo[s] = 0;
+            hi[s] = 0;
+        }
+        __syncthreads();
+        const int v = verts[idx];
+        scan_row<S>(A, v, threadIdx.x, kBlock, key, lo, hi);
+        __syncthreads();
+        const int own = (int)A.C[v];
+        const double ki = A.kd[v];
+        fx_t self = 0;
+        double bg = -kMaxFloat;
+        int bc = kNone;
+        for (int s = threadIdx.x; s < S; s += kBlock) {
+            const int c = key[s];
+            if (c >= 0) consider(A, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+        }
+        block_reduce(R, self, bg, bc);
+        if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+    }
+}
+
+// ckey[k0 + i] = C[col[e0 + i]] for the long rows' arcs (-1 for a self arc).
+constexpr int kFillUnroll = (int)(kKeyChunkArcs / kBlock);
+
+__device__ void keyfill_body(const LmArgs& A, const SmallBins& L, int bi, int nbk)
+{
+    for (int64_t i = bi; i < L.nchunks; i += nbk) {
+        const KeyChunk ch = L.chunks[i];
+        int uu[kFillUnroll];
+#pragma unroll
+        for (int j = 0; j < kFillUnroll; ++j) {
+            const int64_t e = ch.e0 + j * kBlock + threadIdx.x;
+            uu[j] = e < ch.e1 ? (int)A.col[e] : ch.v;
+        }
+#pragma unroll
+        for (int j = 0; j < kFillUnroll; ++j) {
+            const int64_t e = ch.e0 + j * kBlock + threadIdx.x;
+            const int c = uu[j] != ch.v ? (int)A.C[uu[j]] : -1;
+            if (e < ch.e1) L.ckey[ch.k0 + (e - ch.e0)] = c;
+        }
+    }
+}
+
+constexpr int kSmallLds = 20480;   // (256/G)*S*20 for every short-row bin
+
+__global__ void __launch_bounds__(kBlock)
+lm_small_kernel(LmArgs A, SmallBins L)
+{
+    __shared__ __align__(16) char lds[kSmallLds];
+    __shared__ BlockRed R;
+    int bin = 0;
+    while (bin < NUM_SMALL_BINS && (int)blockIdx.x >= L.blk0[bin + 1]) ++bin;
+    const int bi = (int)blockIdx.x - L.blk0[bin];
+    const int nbk = L.blk0[bin + 1] - L.blk0[bin];
+    switch (bin) {
+    case BIN_THREAD: thread_body(A, L.verts[bin], L.n[bin], bi, nbk); break;
+    case BIN_G8: group_body<8, 32>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
+    case BIN_G16: group_body<16, 64>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
+    case BIN_G32: group_body<32, 128>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
+    case BIN_G64: group_body<64, 256>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
+    case BIN_B256: block_body<1024>(A, L.verts[bin], L.n[bin], bi, nbk, lds, R); break;
+    default: keyfill_body(A, L, bi, nbk); break;
+    }
+}
+
+// ---- launch 2: partitioned long rows ---------------------------------------
+
+// Whole-row aggregation in an HBM table of `size` slots (>= 2 * row length):
+// the exact fallback taken when a partition outgrows its LDS table.
+__device__ void global_row_decide(const LmArgs& A, int v, int* __restrict__ key,
+                                  fx_t* __restrict__ tval, int64_t size, BlockRed& R)
+{
+    unsigned long long* val = reinterpret_cast<unsigned long long*>(tval);
+    const int sh = 32 - (63 - __builtin_clzll((unsigned long long)size));
+    for (int64_t s = threadIdx.x; s < size; s += kBlock) {
+        key[s] = -1;
+        val[2 * s] = 0;
+        val[2 * s + 1] = 0;
+    }
+    __threadfence();
+    __syncthreads();
+    const int own = (int)A.C[v];
+    const int64_t e1 = A.off[v + 1];
+    for (int64_t e = A.off[v] + threadIdx.x; e < e1; e += kBlock) {
+        const int u = (int)A.col[e];
+        if (u == v) continue;
+        const int c = (int)A.C[u];
+        const fx_t wf = fx_from_double(A.w[e]);
+        uint64_t h = sh >= 32 ? 0 : (((unsigned)c * kHashMul) >> sh);
+        for (;;) {
+            const int kk = ld_i32_agent(&key[h]);
+            if (kk == c) break;
+            if (kk < 0) {
+                const int prev = atomicCAS(&key[h], -1, c);
+                if (prev < 0 || prev == c) break;
+            }
+            h = (h + 1) & (uint64_t)(size - 1);
+        }
+        fx_atomic_add(&val[2 * h], &val[2 * h + 1], wf);
+    }
+    __threadfence();
+    __syncthreads();
+    const double ki = A.kd[v];
+    fx_t self = 0;
+    double bg = -kMaxFloat;
+    int bc = kNone;
+    for (int64_t s = threadIdx.x; s < size; s += kBlock) {
+        const int c = ld_i32_agent(&key[s]);
+        if (c >= 0) consider(A, ki, own, c, ld_fx_agent(tval + s), self, bg, bc);
+    }
+    block_reduce(R, self, bg, bc);
+    if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+}
+
+constexpr int kBigUnroll = 8;
+constexpr int kMaxProbe = 256;
+
+template <bool Narrow> struct HiWord { using type = unsigned long long; };
+template <> struct HiWord<true> { using type = int; };
+
+__device__ __forceinline__ void lds_add(unsigned long long* lo, unsigned long long* hi, fx_t w)
+{
+    fx_atomic_add(lo, hi, w);
+}
+__device__ __forceinline__ void lds_add(unsigned long long* lo, int* hi, fx_t w)
+{
+    fx_atomic_add_narrow(lo, hi, w);
+}
+
+// S-slot LDS table per workgroup.  Narrow: 32-bit high words (S*16 bytes),
+// used when every row's weights are non-negative and sum to less than 2^31.
+// Items come longest row first, so the long poles start first; partitions of
+// one row spread over all XCDs (a row's keys are small next to L2).
+template <bool Narrow, int S>
+__global__ void __launch_bounds__(kBlock)
+lm_big_kernel(LmArgs A, BigArgs B)
+{
+    using Hi = typename HiWord<Narrow>::type;
+    __shared__ int key[S];
+    __shared__ unsigned long long lo[S];
+    __shared__ Hi hi[S];
+    __shared__ BlockRed R;
+    __shared__ int s_nkeys, s_ovf, s_last;
+
+    for (int64_t it = blockIdx.x; it < B.nitems; it += gridDim.x) {
+        const BigItem item = B.items[it];
+        const int v = item.v, p = item.p, P = item.P, slot = item.slot;
+        const int psh = 32 - (31 - __builtin_clz((unsigned)P));   // top log2(P) bits select the partition
+        for (int s = threadIdx.x; s < S; s += kBlock) {
+            key[s] = -1;
+            lo[s] = 0;
+            hi[s] = 0;
+        }
+        if (threadIdx.x == 0) {
+            s_nkeys = 0;
+            s_ovf = 0;
+        }
+        __syncthreads();
+        const int own = (int)A.C[v];
+        const int64_t e0 = A.off[v];
+        const int64_t d = A.off[v + 1] - e0;
+        const int* ck = B.ckey + item.k0;
+        for (int64_t i0 = 0; i0 < d; i0 += (int64_t)kBigUnroll * kBlock) {
+            if (s_ovf) break;
+            int cj[kBigUnroll];
+#pragma unroll
+            for (int j = 0; j < kBigUnroll; ++j) {
+                const int64_t i = i0 + j * kBlock + threadIdx.x;
+                cj[j] = i < d ? ck[i] : -1;
+            }
+            double wj[kBigUnroll];
+#pragma unroll
+            for (int j = 0; j < kBigUnroll; ++j) {
+                if (cj[j] >= 0 && P > 1 && (int)(((unsigned)cj[j] * kPartMul) >> psh) != p) cj[j] = -1;
+                wj[j] = cj[j] >= 0 ? A.w[e0 + i0 + j * kBlock + threadIdx.x] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < kBigUnroll; ++j) {
+                const int c = cj[j];
+                // Probing gives up after kMaxProbe slots and marks the
+                // partition overflowed (the row then takes the exact HBM
+                // path), so no lane can spin on a full table.
+                bool fresh = false;
+                if (c >= 0 && !s_ovf) {
+                    const fx_t wf = fx_from_double(wj[j]);
+                    unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(S));
+                    bool ok = false;
+                    for (int probes = 0; probes < kMaxProbe; ++probes) {
+                        const int kk = key[h];
+                        if (kk == c) { ok = true; break; }
+                        if (kk < 0) {
+                            const int prev = atomicCAS(&key[h], -1, c);
+                            if (prev < 0 || prev == c) {
+                                ok = true;
+                                fresh = prev < 0;
+                                break;
+                            }
+                        }
+                        h = (h + 1) & (S - 1);
+                    }
+                    if (ok) lds_add(&lo[h], &hi[h], wf);
+                    else s_ovf = 1;
+                }
+                // New keys are counted once per wavefront and step; past
+                // key_limit (<= S/2) the partition is marked overflowed.  Keys
+                // not yet counted are at most one step of the 4 wavefronts, so
+                // the table never holds more than key_limit + 320 < S keys.
+                const unsigned long long nf = __ballot(fresh);
+                if ((threadIdx.x & 63) == 0 && nf) {
+                    if (atomicAdd(&s_nkeys, __popcll(nf)) + __popcll(nf) > B.key_limit) s_ovf = 1;
+                }
+            }
+        }
+        __syncthreads();
+        const bool ovf = s_ovf != 0;
+        const double ki = A.kd[v];
+        fx_t self = 0;
+        double bg = -kMaxFloat;
+        int bc = kNone;
+        if (!ovf) {
+            for (int s = threadIdx.x; s < S; s += kBlock) {
+                const int c = key[s];
+                if (c >= 0) consider(A, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+            }
+        }
+        block_reduce(R, self, bg, bc);
+        const int64_t tb = B.toff[slot];
+        const int64_t tsize = B.toff[slot + 1] - tb;
+        if (P == 1) {
+            if (ovf) global_row_decide(A, v, B.tkey + tb, B.tval + tb, tsize, R);
+            else if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+            __syncthreads();
+            continue;
+        }
+        if (threadIdx.x == 0) {
+            const int64_t q = B.poff[slot] + p;
+            B.pg[q] = bg;
+            B.pc[q] = bc;
+            B.pself[q] = self;
+            if (ovf) atomicOr(&B.ovf[slot], 1);
+            __threadfence();
+            s_last = atomicAdd(&B.cnt[slot], 1) == P - 1;
+        }
+        __syncthreads();
+        if (s_last) {                      // this workgroup finished the row last
+            __threadfence();
+            const bool any_ovf = ld_i32_agent(&B.ovf[slot]) != 0;
+            if (any_ovf) {
+                global_row_decide(A, v, B.tkey + tb, B.tval + tb, tsize, R);
+            } else if (threadIdx.x == 0) {
+                fx_t s2 = 0;
+                double g2 = -kMaxFloat;
+                int c2 = kNone;
+                const int64_t q0 = B.poff[slot];
+                for (int i = 0; i < P; ++i) {
+                    s2 += ld_fx_agent(B.pself + q0 + i);
+                    const double gi = __longlong_as_double(
+                        (long long)ld_u64_agent(reinterpret_cast<const unsigned long long*>(B.pg + q0 + i)));
+                    const int ci = ld_i32_agent(B.pc + q0 + i);
+                    if (better(gi, ci, g2, c2)) {
+                        g2 = gi;
+                        c2 = ci;
+                    }
+                }
+                A.Cn[v] = decide(A, v, own, s2, g2, c2);
+            }
+            if (threadIdx.x == 0) {        // leave the counters clean for the next batch
+                B.cnt[slot] = 0;
+                B.ovf[slot] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// update_community_weights + update_community_ids of one batch, fused
+// (graph_cuda.cu:1234-1249 then 1191-1196): each thread touches only its own
+// vertex's C/C' entries, so the swap needs no second launch.
+__global__ void __launch_bounds__(kBlock)
+commit_kernel(int64_t v0, int64_t v1, int64_t* __restrict__ C, int64_t* __restrict__ Cn,
+              const fx_t* __restrict__ k, fx_t* __restrict__ a)
+{
+    for (int64_t v = v0 + blockIdx.x * (int64_t)kBlock + threadIdx.x; v < v1;
+         v += (int64_t)gridDim.x * kBlock) {
+        const int64_t src = C[v];
+        const int64_t dst = Cn[v];
+        if (src != dst) {
+            const fx_t kv = k[v];
+            acc_add(a + src, -kv);
+            acc_add(a + dst, kv);
+        }
+        C[v] = dst;
+        Cn[v] = src;
+    }
+}
+
+}  // namespace
+
+// Rows per workgroup of each short-row bin.
+static constexpr int kRowsPerBlock[NUM_SMALL_BINS] = {kBlock, kBlock / 8, kBlock / 16, kBlock / 32,
+                                                      kBlock / 64, 1};
+
+int small_bins_blocks(SmallBins& L, unsigned cap_per_bin)
+{
+    int total = 0;
+    for (int b = 0; b < NUM_SMALL_BINS; ++b) {
+        L.blk0[b] = total;
+        if (L.n[b] > 0) total += (int)grid_for(L.n[b], kRowsPerBlock[b], cap_per_bin);
+    }
+    L.blk0[NUM_SMALL_BINS] = total;
+    if (L.nchunks > 0) total += (int)grid_for(L.nchunks, 1, cap_per_bin);
+    L.blk0[NUM_SMALL_BINS + 1] = total;
+    return total;
+}
+
+void launch_lm_small(const LmArgs& A, const SmallBins& L, hipStream_t s)
+{
+    const int blocks = L.blk0[NUM_SMALL_BINS + 1];
+    if (blocks > 0) lm_small_kernel<<<blocks, kBlock, 0, s>>>(A, L);
+}
+
+void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s)
+{
+    if (B.nitems <= 0) return;
+    const unsigned g = grid_for(B.nitems, 1, 2048);
+    if (B.slots == 2048) {
+        if (B.narrow) lm_big_kernel<true, 2048><<<g, kBlock, 0, s>>>(A, B);
+        else lm_big_kernel<false, 2048><<<g, kBlock, 0, s>>>(A, B);
+    } else {
+        if (B.narrow) lm_big_kernel<true, 4096><<<g, kBlock, 0, s>>>(A, B);
+        else lm_big_kernel<false, 4096><<<g, kBlock, 0, s>>>(A, B);
+    }
+}
+
+void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, const fx_t* k, fx_t* a,
+                   hipStream_t s)
+{
+    if (v1 <= v0) return;
+    commit_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(v0, v1, C, Cn, k, a);
+}
+
+}  // namespace nido

@@ -67,6 +67,7 @@ _sig("orc_update_community_ids", None, C.c_void_p, C.c_int)
 _sig("orc_restore_community", None, C.c_void_p)
 _sig("orc_aggregation", C.c_int, C.c_void_p)
 _sig("orc_run", None, C.c_void_p, _i64, C.c_double, C.c_double, C.c_int)
+_sig("orc_louvain_iteration", C.c_double, C.c_void_p, _pdbl, _pdbl)
 _sig("orc_hist_phases", C.c_int, C.c_void_p)
 _sig("orc_hist_phase", None, C.c_void_p, C.c_int, _pi64, _pi64, _pi64, _pdbl, _pdbl, _pdbl, _pdbl)
 _sig("orc_hist_iter_q", None, C.c_void_p, C.c_int, _pdbl)
@@ -219,6 +220,12 @@ class Oracle:
 
     def aggregation(self) -> bool:
         return bool(_lib.orc_aggregation(self._h))
+
+    def louvain_iteration(self, q: float):
+        """One loop-body pass; returns (accepted Q, dQ, evaluated Q)."""
+        qq, dq = C.c_double(q), C.c_double()
+        qt = _lib.orc_louvain_iteration(self._h, C.byref(qq), C.byref(dq))
+        return qq.value, dq.value, qt
 
     # LouvainGPU::run
     def run(self, max_loops: int = 500, tol: float = 1e-6, tol_phase: float | None = None,

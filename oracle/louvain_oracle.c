@@ -684,6 +684,22 @@ int orc_aggregation(orc_ctx* c)
     return 0;
 }
 
+/* The loop body of LouvainGPU::run, louvain_gpu.cpp:55-90: all batches,
+ * modularity, accept or roll back.  Returns the evaluated modularity. */
+double orc_louvain_iteration(orc_ctx* c, double* q, double* dq)
+{
+    for (int b = 0; b < c->ph.nb; ++b) {
+        orc_louvain_update(c, b);
+        orc_update_community_weights(c, b);
+        orc_update_community_ids(c, b);
+    }
+    double qt = orc_compute_modularity(c);
+    *dq = qt - *q;
+    if (*dq < 0) orc_restore_community(c);
+    else *q = qt;
+    return qt;
+}
+
 /* LouvainGPU::run, louvain_gpu.cpp:6-139.  Records per-phase history. */
 void orc_run(orc_ctx* c, int64_t max_loops, double tol, double tol_phase, int record)
 {

@@ -118,22 +118,44 @@ def test_rmat_full_run(scale, ef, weighted, nb):
     assert_runs_equal(ref, got)
 
 
-def test_launch_shape_does_not_change_results():
+KNOBS = [
+    {"thread_max_deg": 0},
+    {"thread_max_deg": 4},
+    {"big_min_deg": 16, "big_part_arcs": 4},                        # many partitions per row
+    {"big_min_deg": 0, "big_part_arcs": 1},                         # every row partitioned
+    {"big_min_deg": 32, "big_part_arcs": 64, "big_key_limit": 2},   # forces the HBM fallback
+]
+
+
+@pytest.mark.parametrize("knobs", KNOBS, ids=[",".join(f"{k}={v}" for k, v in d.items()) for d in KNOBS])
+def test_launch_shape_does_not_change_results(knobs):
     na = _na()
     off, col, w = na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 3).arrays()
-    print("case: rmat s12 thread_max_deg sweep", flush=True)
-    outs = []
-    for tmd in (0, 4, 8):
-        g = na.GraphGPU((off, col, w), 4)
-        g.set_tuning(thread_max_deg=tmd)
-        st = na.LouvainGPU(500, 1e-6, 4).run(g, record=True)
-        outs.append((st.q, [g.phase_communities(i) for i in range(st.phases)]))
-        g.close()
-    for q, comms in outs[1:]:
-        assert q == outs[0][0]
-        assert len(comms) == len(outs[0][1])
-        for a, b in zip(comms, outs[0][1]):
-            assert np.array_equal(a, b)
+    print(f"case: rmat s12 knobs {knobs}", flush=True)
+    ref = run_oracle(off, col, w, 4)
+    g = na.GraphGPU((off, col, w), 4)
+    g.set_tuning(**knobs)
+    assert all(g.get_tuning()[k] == v for k, v in knobs.items())
+    st = na.LouvainGPU(500, 1e-6, 4).run(g, record=True)
+    assert st.phases == len(ref["phases"])
+    assert st.q == ref["q"]
+    for i, p in enumerate(ref["phases"]):
+        assert np.array_equal(g.phase_communities(i), p["comm"]), f"phase {i}"
+    g.close()
+
+
+@pytest.mark.parametrize("scale_w", [2.0 ** 33, 1.0 / 1024])
+def test_weight_ranges(scale_w):
+    """Row sums >= 2^31 take the 64-bit-high-word long-row tables; tiny
+    weights exercise the low end of the Q64.64 grid.  Scaling by a power of
+    two keeps every weight exact."""
+    na = _na()
+    off, col, w = na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 5).arrays()
+    w = w * scale_w
+    print(f"case: rmat s12 weights x {scale_w}", flush=True)
+    got = run_gpu(off, col, w, 4)
+    ref = run_oracle(off, col, w, 4)
+    assert_runs_equal(ref, got)
 
 
 def test_repeat_runs_are_bit_identical():
