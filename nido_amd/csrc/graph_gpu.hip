@@ -169,12 +169,14 @@ struct nido_graph {
     // state, capacity n0
     int64_t* d_C = nullptr;
     int64_t* d_Cn = nullptr;
+    int* d_C32 = nullptr;          // (int)d_C, kept in step by singleton and commit
     fx_t* d_k = nullptr;
     fx_t* d_a = nullptr;
     double* d_kd = nullptr;
     int64_t* d_clus = nullptr;
     int64_t* d_work = nullptr;     // n0+1 int64 scratch (renumbering)
     int* d_src = nullptr;          // row of every arc, capacity = phase-0 arcs
+    int* d_col32 = nullptr;        // (int)d_col, capacity = phase-0 arcs
     fx_t* d_part = nullptr;        // 2 * kPartBlocks reduction partials
     void* d_scan_tmp = nullptr;
     // scalars
@@ -198,7 +200,6 @@ struct nido_graph {
     fx_t* d_pself = nullptr;
     int64_t part_cap = 0;
     int* d_cnt = nullptr;
-    int* d_ovf = nullptr;
     int64_t slot_cap = 0;
     int* d_tkey = nullptr;
     fx_t* d_tval = nullptr;
@@ -392,11 +393,8 @@ int setup_phase(nido_graph* g)
     }
     if (slot_need > g->slot_cap) {
         dev_free(g->d_cnt);
-        dev_free(g->d_ovf);
         NIDO_TRY(dev_alloc(&g->d_cnt, slot_need));
-        NIDO_TRY(dev_alloc(&g->d_ovf, slot_need));
         launch_fill_i32(g->d_cnt, 0, slot_need, g->stream);   // the kernels keep them zero afterwards
-        launch_fill_i32(g->d_ovf, 0, slot_need, g->stream);
         g->slot_cap = slot_need;
     }
     const size_t nbig_items = bitems.size();
@@ -425,7 +423,6 @@ int setup_phase(nido_graph* g)
         B.pc = g->d_pc;
         B.pself = g->d_pself;
         B.cnt = g->d_cnt;
-        B.ovf = g->d_ovf;
         B.toff = g->d_big_toff + toff_base[(size_t)b];
         B.tkey = g->d_tkey;
         B.tval = g->d_tval;
@@ -448,6 +445,7 @@ int setup_phase(nido_graph* g)
     NIDO_TRY(upload(&g->d_items, items, g->stream));
 
     launch_row_ids(g->d_off, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_src, g->stream);
+    launch_narrow_ids(g->d_col, g->d_col32, g->ne, g->stream);
     unsigned long long* d_maxk = reinterpret_cast<unsigned long long*>(g->d_tot + 1);
     launch_vertex_weights(g->d_off, g->d_w, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_k,
                           g->d_kd, nv, g->d_acc + 2, d_maxk, g->d_part, g->stream);
@@ -486,7 +484,7 @@ void free_graph_arrays(nido_graph* g)
 // ---- the GraphGPU method set ---------------------------------------------
 int do_singleton(nido_graph* g)
 {
-    launch_singleton(g->d_C, g->d_Cn, g->d_a, g->d_k, g->nv, g->stream);
+    launch_singleton(g->d_C, g->d_Cn, g->d_C32, g->d_a, g->d_k, g->nv, g->stream);
     HIP_TRY(hipGetLastError());
     std::fill(g->committed.begin(), g->committed.end(), 0);
     return NIDO_OK;
@@ -500,7 +498,7 @@ int do_modularity(nido_graph* g, double* q)
     }
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
-        launch_modularity(g->d_src, g->d_col, g->d_w, g->d_C, g->ne, g->d_a, g->nv, g->twom_d,
+        launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d,
                           g->d_part, g->d_q, g->stream);
     }
     HIP_TRY(hipGetLastError());
@@ -521,7 +519,7 @@ int do_lm(nido_graph* g, int b)
                                hipMemcpyDeviceToDevice, g->stream));
         return NIDO_OK;
     }
-    LmArgs A{g->d_off, g->d_col, g->d_w, g->d_C, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m};
+    LmArgs A{g->d_off, g->d_col32, g->d_w, g->d_C32, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m};
     // launch 1 fills the long rows' community keys that launch 2 reads
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
@@ -539,8 +537,8 @@ int do_commit(nido_graph* g, int b)
 {
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
-        launch_commit(g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_C, g->d_Cn, g->d_k, g->d_a,
-                      g->stream);
+        launch_commit(g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_C, g->d_Cn, g->d_C32, g->d_k,
+                      g->d_a, g->stream);
     }
     HIP_TRY(hipGetLastError());
     g->committed[(size_t)b] = 1;
@@ -549,7 +547,7 @@ int do_commit(nido_graph* g, int b)
 
 int do_restore(nido_graph* g)
 {
-    launch_commit(0, g->nv, g->d_C, g->d_Cn, g->d_k, g->d_a, g->stream);
+    launch_commit(0, g->nv, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->stream);
     HIP_TRY(hipGetLastError());
     return NIDO_OK;
 }
@@ -782,6 +780,7 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         (rc = dev_alloc(&g->d_clus, nv)) || (rc = dev_alloc(&g->d_work, nv + 1)) ||
         (rc = dev_alloc(&g->d_acc, 4)) || (rc = dev_alloc(&g->d_q, 1)) ||
         (rc = dev_alloc(&g->d_tot, 2)) || (rc = dev_alloc(&g->d_src, ne)) ||
+        (rc = dev_alloc(&g->d_col32, ne)) || (rc = dev_alloc(&g->d_C32, nv)) ||
         (rc = dev_alloc(&g->d_part, 2 * (int64_t)kPartBlocks)))
         return fail(rc);
     if (hipMalloc(&g->d_scan_tmp, scan_temp_bytes(nv + 1)) != hipSuccess)
@@ -854,10 +853,10 @@ int nido_graph_destroy(nido_graph* g)
     free_graph_arrays(g);
     dev_free(g->d_C); dev_free(g->d_Cn); dev_free(g->d_k); dev_free(g->d_a); dev_free(g->d_kd);
     dev_free(g->d_clus); dev_free(g->d_work); dev_free(g->d_acc); dev_free(g->d_q); dev_free(g->d_tot);
-    dev_free(g->d_src); dev_free(g->d_part);
+    dev_free(g->d_src); dev_free(g->d_part); dev_free(g->d_col32); dev_free(g->d_C32);
     dev_free(g->d_binv); dev_free(g->d_big_items); dev_free(g->d_big_poff); dev_free(g->d_big_toff);
     dev_free(g->d_key_chunks); dev_free(g->d_ckey);
-    dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt); dev_free(g->d_ovf);
+    dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt);
     dev_free(g->d_tkey); dev_free(g->d_tval);
     dev_free(g->d_small); dev_free(g->d_items);
     if (g->d_scan_tmp) (void)hipFree(g->d_scan_tmp);

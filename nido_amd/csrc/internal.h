@@ -23,7 +23,7 @@ constexpr int NUM_SMALL_BINS = BIN_BIG;
 
 constexpr int kThreadMaxDeg = 8;          // registers hold up to this many arcs
 constexpr int64_t kBinMaxDeg[NUM_BINS] = {8, 16, 32, 64, 128, 512, INT64_MAX};
-constexpr int64_t kBigPartArcs = 1024;    // expected distinct keys per partition, at most
+constexpr int64_t kBigPartArcs = 1536;    // expected distinct keys per partition, at most
 constexpr int kBigSlots = 4096;           // LDS table of a partition workgroup (64/80 KB)
 constexpr int kBigLimit = kBigSlots / 2;  // distinct keys before the exact HBM fallback
 
@@ -65,18 +65,19 @@ struct BigArgs {
     int* pc;                 // partial best community
     fx_t* pself;             // partial e_{v,own}
     int* cnt;                // per slot: partitions finished (reset by the last one)
-    int* ovf;                // per slot: some partition outgrew its LDS table
     const int64_t* toff;     // per slot: HBM fallback table [toff[s], toff[s+1])
     int* tkey;
     fx_t* tval;
 };
 
-// Everything one local-moving launch reads or writes.
+// Everything one local-moving launch reads or writes.  Vertex and community
+// ids are below 2^31 on the device, so the hot loops read 32-bit mirrors of
+// the int64 column array (col) and community array (C).
 struct LmArgs {
     const int64_t* off;
-    const int64_t* col;
+    const int* col;          // 32-bit mirror of the CSR columns
     const double* w;
-    const int64_t* C;        // community of every vertex (snapshot)
+    const int* C;            // 32-bit mirror of the communities (snapshot)
     int64_t* Cn;             // decided community of the batch's vertices
     const double* kd;        // k_v rounded to FP64
     const fx_t* k;           // k_v exact
@@ -108,7 +109,10 @@ void launch_vertex_weights(const int64_t* off, const double* w, const int* small
 // src[e] = row of arc e.
 void launch_row_ids(const int64_t* off, const int* small, int64_t nsmall, const RowItem* items,
                     int64_t nitems, int* src, hipStream_t s);
-void launch_singleton(int64_t* C, int64_t* Cn, fx_t* a, const fx_t* k, int64_t nv, hipStream_t s);
+void launch_singleton(int64_t* C, int64_t* Cn, int* C32, fx_t* a, const fx_t* k, int64_t nv,
+                      hipStream_t s);
+// col32[e] = (int)col[e]
+void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s);
 
 // Fills L.blk0 from L.n and L.nchunks; returns the number of blocks of the
 // fused launch.
@@ -116,12 +120,12 @@ int small_bins_blocks(SmallBins& L, unsigned cap_per_bin);
 void launch_lm_small(const LmArgs& A, const SmallBins& L, hipStream_t s);
 void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s);
 
-void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, const fx_t* k, fx_t* a,
+void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, const fx_t* k, fx_t* a,
                    hipStream_t s);
 
 // Modularity over the arc-parallel CSR (src = row ids): writes Q to *q;
 // part holds 2 * kPartBlocks partials.
-void launch_modularity(const int* src, const int64_t* col, const double* w, const int64_t* C,
+void launch_modularity(const int* src, const int* col, const double* w, const int* C,
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
                        hipStream_t s);
 

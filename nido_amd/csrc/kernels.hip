@@ -100,15 +100,24 @@ sum_partials_kernel(const fx_t* __restrict__ part, int n, fx_t* out)
 }
 
 __global__ void __launch_bounds__(kBlock)
-singleton_kernel(int64_t* __restrict__ C, int64_t* __restrict__ Cn, fx_t* __restrict__ a,
-                 const fx_t* __restrict__ k, int64_t nv)
+singleton_kernel(int64_t* __restrict__ C, int64_t* __restrict__ Cn, int* __restrict__ C32,
+                 fx_t* __restrict__ a, const fx_t* __restrict__ k, int64_t nv)
 {
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nv;
          i += (int64_t)gridDim.x * kBlock) {
         C[i] = i;
         Cn[i] = i;
+        C32[i] = (int)i;
         a[i] = k[i];
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+narrow_kernel(const int64_t* __restrict__ col, int* __restrict__ col32, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        col32[i] = (int)col[i];
 }
 
 // src[e] = the row of arc e, from the per-phase row lists.
@@ -141,8 +150,8 @@ src_items_kernel(const RowItem* __restrict__ items, int64_t n, int* __restrict__
 constexpr int kModUnroll = 8;
 
 __global__ void __launch_bounds__(kBlock)
-mod_kernel(const int* __restrict__ src, const int64_t* __restrict__ col,
-           const double* __restrict__ w, const int64_t* __restrict__ C, int64_t ne,
+mod_kernel(const int* __restrict__ src, const int* __restrict__ col,
+           const double* __restrict__ w, const int* __restrict__ C, int64_t ne,
            const fx_t* __restrict__ a, int64_t nv, double twom_d, fx_t* __restrict__ part)
 {
     __shared__ fx_t red[kBlock / 64];
@@ -150,7 +159,7 @@ mod_kernel(const int* __restrict__ src, const int64_t* __restrict__ col,
     const int64_t tile = (int64_t)kBlock * kModUnroll;
     for (int64_t t0 = blockIdx.x * tile; t0 < ne; t0 += (int64_t)gridDim.x * tile) {
         int s[kModUnroll];
-        int64_t u[kModUnroll];
+        int u[kModUnroll];
 #pragma unroll
         for (int j = 0; j < kModUnroll; ++j) {
             const int64_t e = t0 + j * kBlock + threadIdx.x;
@@ -446,13 +455,20 @@ void launch_row_ids(const int64_t* off, const int* small, int64_t nsmall, const 
     if (nitems > 0) src_items_kernel<<<grid_for(nitems, 1), kBlock, 0, s>>>(items, nitems, src);
 }
 
-void launch_singleton(int64_t* C, int64_t* Cn, fx_t* a, const fx_t* k, int64_t nv, hipStream_t s)
+void launch_singleton(int64_t* C, int64_t* Cn, int* C32, fx_t* a, const fx_t* k, int64_t nv,
+                      hipStream_t s)
 {
     if (nv <= 0) return;
-    singleton_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, Cn, a, k, nv);
+    singleton_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, Cn, C32, a, k, nv);
 }
 
-void launch_modularity(const int* src, const int64_t* col, const double* w, const int64_t* C,
+void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    narrow_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(col, col32, n);
+}
+
+void launch_modularity(const int* src, const int* col, const double* w, const int* C,
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
                        hipStream_t s)
 {

@@ -342,10 +342,33 @@ __device__ __forceinline__ void lds_add(unsigned long long* lo, int* hi, fx_t w)
     fx_atomic_add_narrow(lo, hi, w);
 }
 
+// Makes one lane's LDS writes visible to the other lanes of its wavefront.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Set bits of `mask` below this lane.
+__device__ __forceinline__ int lane_rank(unsigned long long mask)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+constexpr int kQueue = 128;   // per-wavefront compaction queue (entries)
+constexpr int kOverflow = -2; // partial community of a partition that outgrew its table
+
 // S-slot LDS table per workgroup.  Narrow: 32-bit high words (S*16 bytes),
 // used when every row's weights are non-negative and sum to less than 2^31.
 // Items come longest row first, so the long poles start first; partitions of
 // one row spread over all XCDs (a row's keys are small next to L2).
+//
+// Only ~1/P of a row's keys belong to a partition, so the keys that do (and
+// later the occupied table slots) are first compacted into a per-wavefront
+// LDS queue and then processed 64 at a time with every lane busy; without it
+// a wavefront would run the whole exact insert for one or two live lanes.
 template <bool Narrow, int S>
 __global__ void __launch_bounds__(kBlock)
 lm_big_kernel(LmArgs A, BigArgs B)
@@ -354,14 +377,31 @@ lm_big_kernel(LmArgs A, BigArgs B)
     __shared__ int key[S];
     __shared__ unsigned long long lo[S];
     __shared__ Hi hi[S];
+    __shared__ int qkey[kBlock / 64][kQueue];
+    __shared__ int qidx[kBlock / 64][kQueue];
     __shared__ BlockRed R;
     __shared__ int s_nkeys, s_ovf, s_last;
+    const int lane = threadIdx.x & 63;
+    int* const qk = qkey[threadIdx.x >> 6];
+    int* const qi = qidx[threadIdx.x >> 6];
 
     for (int64_t it = blockIdx.x; it < B.nitems; it += gridDim.x) {
         const BigItem item = B.items[it];
         const int v = item.v, p = item.p, P = item.P, slot = item.slot;
         const int psh = 32 - (31 - __builtin_clz((unsigned)P));   // top log2(P) bits select the partition
-        for (int s = threadIdx.x; s < S; s += kBlock) {
+        const int own = (int)A.C[v];
+        const int64_t e0 = A.off[v];
+        const int64_t d = A.off[v + 1] - e0;
+        // A single-partition row holds at most d keys: its table uses the
+        // smallest power of two >= 2d slots (at least 256), so the clear and
+        // the sweep cost O(d) rather than O(S).
+        int lg = ilog2c(S);
+        if (P == 1) {
+            lg = 8;
+            while ((1ll << lg) < 2 * d && lg < ilog2c(S)) ++lg;
+        }
+        const int ts = 1 << lg;
+        for (int s = threadIdx.x; s < ts; s += kBlock) {
             key[s] = -1;
             lo[s] = 0;
             hi[s] = 0;
@@ -371,10 +411,51 @@ lm_big_kernel(LmArgs A, BigArgs B)
             s_ovf = 0;
         }
         __syncthreads();
-        const int own = (int)A.C[v];
-        const int64_t e0 = A.off[v];
-        const int64_t d = A.off[v + 1] - e0;
         const int* ck = B.ckey + item.k0;
+        const double* wrow = A.w + e0;
+
+        // Inserts queue entries [0, cnt) (one per lane), then moves entries
+        // [cnt, qn) to the front.  Probing gives up after kMaxProbe slots and
+        // marks the partition overflowed (the row then takes the exact HBM
+        // path), so no lane can spin on a full table.  New keys are counted
+        // once per flush; past key_limit (<= S/2) the partition is marked
+        // overflowed, and as a flush adds at most 64 keys per wavefront the
+        // table never holds more than key_limit + 256 < S keys.
+        auto flush = [&](int cnt, int qn) {
+            bool fresh = false;
+            if (lane < cnt && !s_ovf) {
+                const int c = qk[lane];
+                const fx_t wf = fx_from_double(wrow[qi[lane]]);
+                unsigned h = ((unsigned)c * kHashMul) >> (32 - lg);
+                bool ok = false;
+                for (int probes = 0; probes < kMaxProbe; ++probes) {
+                    const int kk = key[h];
+                    if (kk == c) { ok = true; break; }
+                    if (kk < 0) {
+                        const int prev = atomicCAS(&key[h], -1, c);
+                        if (prev < 0 || prev == c) {
+                            ok = true;
+                            fresh = prev < 0;
+                            break;
+                        }
+                    }
+                    h = (h + 1) & (ts - 1);
+                }
+                if (ok) lds_add(&lo[h], &hi[h], wf);
+                else s_ovf = 1;
+            }
+            const unsigned long long nf = __ballot(fresh);
+            if (lane == 0 && nf) {
+                if (atomicAdd(&s_nkeys, __popcll(nf)) + __popcll(nf) > B.key_limit) s_ovf = 1;
+            }
+            if (lane < qn - cnt) {       // each index is read and rewritten by one lane
+                qk[lane] = qk[cnt + lane];
+                qi[lane] = qi[cnt + lane];
+            }
+            wave_lds_sync();
+        };
+
+        int qn = 0;   // queue length, uniform in the wavefront
         for (int64_t i0 = 0; i0 < d; i0 += (int64_t)kBigUnroll * kBlock) {
             if (s_ovf) break;
             int cj[kBigUnroll];
@@ -383,49 +464,25 @@ lm_big_kernel(LmArgs A, BigArgs B)
                 const int64_t i = i0 + j * kBlock + threadIdx.x;
                 cj[j] = i < d ? ck[i] : -1;
             }
-            double wj[kBigUnroll];
 #pragma unroll
             for (int j = 0; j < kBigUnroll; ++j) {
-                if (cj[j] >= 0 && P > 1 && (int)(((unsigned)cj[j] * kPartMul) >> psh) != p) cj[j] = -1;
-                wj[j] = cj[j] >= 0 ? A.w[e0 + i0 + j * kBlock + threadIdx.x] : 0.0;
-            }
-#pragma unroll
-            for (int j = 0; j < kBigUnroll; ++j) {
-                const int c = cj[j];
-                // Probing gives up after kMaxProbe slots and marks the
-                // partition overflowed (the row then takes the exact HBM
-                // path), so no lane can spin on a full table.
-                bool fresh = false;
-                if (c >= 0 && !s_ovf) {
-                    const fx_t wf = fx_from_double(wj[j]);
-                    unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(S));
-                    bool ok = false;
-                    for (int probes = 0; probes < kMaxProbe; ++probes) {
-                        const int kk = key[h];
-                        if (kk == c) { ok = true; break; }
-                        if (kk < 0) {
-                            const int prev = atomicCAS(&key[h], -1, c);
-                            if (prev < 0 || prev == c) {
-                                ok = true;
-                                fresh = prev < 0;
-                                break;
-                            }
-                        }
-                        h = (h + 1) & (S - 1);
-                    }
-                    if (ok) lds_add(&lo[h], &hi[h], wf);
-                    else s_ovf = 1;
+                const bool mine = cj[j] >= 0 && (P == 1 || (int)(((unsigned)cj[j] * kPartMul) >> psh) == p);
+                const unsigned long long mask = __ballot(mine);
+                if (mine) {
+                    const int pos = qn + lane_rank(mask);
+                    qk[pos] = cj[j];
+                    qi[pos] = (int)(i0 + j * kBlock + threadIdx.x);
                 }
-                // New keys are counted once per wavefront and step; past
-                // key_limit (<= S/2) the partition is marked overflowed.  Keys
-                // not yet counted are at most one step of the 4 wavefronts, so
-                // the table never holds more than key_limit + 320 < S keys.
-                const unsigned long long nf = __ballot(fresh);
-                if ((threadIdx.x & 63) == 0 && nf) {
-                    if (atomicAdd(&s_nkeys, __popcll(nf)) + __popcll(nf) > B.key_limit) s_ovf = 1;
+                qn += __popcll(mask);
+                if (qn >= 64) {          // qn <= 127 < kQueue here
+                    wave_lds_sync();
+                    flush(64, qn);
+                    qn -= 64;
                 }
             }
         }
+        wave_lds_sync();
+        if (qn > 0) flush(qn, qn);
         __syncthreads();
         const bool ovf = s_ovf != 0;
         const double ki = A.kd[v];
@@ -433,10 +490,32 @@ lm_big_kernel(LmArgs A, BigArgs B)
         double bg = -kMaxFloat;
         int bc = kNone;
         if (!ovf) {
-            for (int s = threadIdx.x; s < S; s += kBlock) {
-                const int c = key[s];
-                if (c >= 0) consider(A, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+            // each wavefront sweeps a quarter of the table, queueing occupied
+            // slots and scoring them 64 at a time
+            auto score = [&](int cnt, int qn) {
+                if (lane < cnt) {
+                    const int s = qk[lane];
+                    consider(A, ki, own, key[s], fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+                }
+                if (lane < qn - cnt) qk[lane] = qk[cnt + lane];
+                wave_lds_sync();
+            };
+            const int quarter = ts / (kBlock / 64);
+            const int s_begin = (threadIdx.x >> 6) * quarter;
+            int qn = 0;
+            for (int s0 = s_begin; s0 < s_begin + quarter; s0 += 64) {
+                const bool occ = key[s0 + lane] >= 0;
+                const unsigned long long mask = __ballot(occ);
+                if (occ) qk[qn + lane_rank(mask)] = s0 + lane;
+                qn += __popcll(mask);
+                if (qn >= 64) {
+                    wave_lds_sync();
+                    score(64, qn);
+                    qn -= 64;
+                }
             }
+            wave_lds_sync();
+            if (qn > 0) score(qn, qn);
         }
         block_reduce(R, self, bg, bc);
         const int64_t tb = B.toff[slot];
@@ -447,42 +526,53 @@ lm_big_kernel(LmArgs A, BigArgs B)
             __syncthreads();
             continue;
         }
+        // The partial reaches the row's last workgroup through memory-side
+        // atomics only: agent-scope exchanges, whose returned old values are
+        // awaited before the counter increment is issued, and agent-scope
+        // atomic loads on the reading side.  An agent-scope fence is avoided
+        // on purpose: on this GPU it writes back and invalidates the XCD's
+        // whole L2 under every other workgroup resident there.  An overflowed
+        // partition reports community kOverflow.
         if (threadIdx.x == 0) {
             const int64_t q = B.poff[slot] + p;
-            B.pg[q] = bg;
-            B.pc[q] = bc;
-            B.pself[q] = self;
-            if (ovf) atomicOr(&B.ovf[slot], 1);
-            __threadfence();
-            s_last = atomicAdd(&B.cnt[slot], 1) == P - 1;
+            unsigned long long* pgw = reinterpret_cast<unsigned long long*>(B.pg + q);
+            unsigned long long* psw = reinterpret_cast<unsigned long long*>(B.pself + q);
+            const unsigned long long r0 = __hip_atomic_exchange(
+                pgw, (unsigned long long)__double_as_longlong(bg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int r1 = __hip_atomic_exchange(B.pc + q, ovf ? kOverflow : bc, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long r2 = __hip_atomic_exchange(psw, fx_lo(self), __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long r3 = __hip_atomic_exchange(
+                psw + 1, (unsigned long long)fx_hi(self), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            asm volatile("" : : "v"(r0), "v"(r1), "v"(r2), "v"(r3) : "memory");
+            s_last = __hip_atomic_fetch_add(&B.cnt[slot], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == P - 1;
         }
         __syncthreads();
         if (s_last) {                      // this workgroup finished the row last
-            __threadfence();
-            const bool any_ovf = ld_i32_agent(&B.ovf[slot]) != 0;
-            if (any_ovf) {
-                global_row_decide(A, v, B.tkey + tb, B.tval + tb, tsize, R);
-            } else if (threadIdx.x == 0) {
-                fx_t s2 = 0;
-                double g2 = -kMaxFloat;
-                int c2 = kNone;
+            fx_t s2 = 0;
+            double g2 = -kMaxFloat;
+            int c2 = kNone;
+            bool any_ovf = false;
+            if (threadIdx.x == 0) {
                 const int64_t q0 = B.poff[slot];
                 for (int i = 0; i < P; ++i) {
+                    const int ci = ld_i32_agent(B.pc + q0 + i);
+                    any_ovf = any_ovf || ci == kOverflow;
                     s2 += ld_fx_agent(B.pself + q0 + i);
                     const double gi = __longlong_as_double(
                         (long long)ld_u64_agent(reinterpret_cast<const unsigned long long*>(B.pg + q0 + i)));
-                    const int ci = ld_i32_agent(B.pc + q0 + i);
                     if (better(gi, ci, g2, c2)) {
                         g2 = gi;
                         c2 = ci;
                     }
                 }
-                A.Cn[v] = decide(A, v, own, s2, g2, c2);
+                s_nkeys = any_ovf;
+                B.cnt[slot] = 0;           // leave the counter clean for the next batch
             }
-            if (threadIdx.x == 0) {        // leave the counters clean for the next batch
-                B.cnt[slot] = 0;
-                B.ovf[slot] = 0;
-            }
+            __syncthreads();
+            if (s_nkeys) global_row_decide(A, v, B.tkey + tb, B.tval + tb, tsize, R);
+            else if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, s2, g2, c2);
         }
         __syncthreads();
     }
@@ -490,10 +580,10 @@ lm_big_kernel(LmArgs A, BigArgs B)
 
 // update_community_weights + update_community_ids of one batch, fused
 // (graph_cuda.cu:1234-1249 then 1191-1196): each thread touches only its own
-// vertex's C/C' entries, so the swap needs no second launch.
+// vertex's C/C' entries, so the swap needs no second launch.  C32 mirrors C.
 __global__ void __launch_bounds__(kBlock)
 commit_kernel(int64_t v0, int64_t v1, int64_t* __restrict__ C, int64_t* __restrict__ Cn,
-              const fx_t* __restrict__ k, fx_t* __restrict__ a)
+              int* __restrict__ C32, const fx_t* __restrict__ k, fx_t* __restrict__ a)
 {
     for (int64_t v = v0 + blockIdx.x * (int64_t)kBlock + threadIdx.x; v < v1;
          v += (int64_t)gridDim.x * kBlock) {
@@ -503,6 +593,7 @@ commit_kernel(int64_t v0, int64_t v1, int64_t* __restrict__ C, int64_t* __restri
             const fx_t kv = k[v];
             acc_add(a + src, -kv);
             acc_add(a + dst, kv);
+            C32[v] = (int)dst;
         }
         C[v] = dst;
         Cn[v] = src;
@@ -547,11 +638,11 @@ void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s)
     }
 }
 
-void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, const fx_t* k, fx_t* a,
+void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, const fx_t* k, fx_t* a,
                    hipStream_t s)
 {
     if (v1 <= v0) return;
-    commit_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(v0, v1, C, Cn, k, a);
+    commit_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(v0, v1, C, Cn, C32, k, a);
 }
 
 }  // namespace nido

@@ -1,0 +1,167 @@
+"""Host-side parts of libnido_amd.so that need no GPU: the C ABI exports,
+the reference binary format (graph.cpp:322-495), the seeded generators, and
+the argument checks of the device entry points."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import nido_amd as na
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    with open(os.path.join(ROOT, "include", "nido_amd.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z_0-9]*\s*\**\s*(nido_[a-z_0-9]+)\s*\(", text, re.M):
+        names.add(m.group(1))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) > 30
+    out = subprocess.run(["nm", "-D", "--defined-only", na.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = sorted(n for n in names if n not in exported)
+    assert not missing, missing
+    for n in names:
+        getattr(na.lib, n)
+
+
+def test_abi_version():
+    assert na.lib.nido_abi_version() == 1
+
+
+def write_reference_file(path, off, col, w, base=0, ne_header=None):
+    """Writes the reference format with numpy: header, offsets (optionally
+    shifted by `base`), then {int64 tail, double weight} records."""
+    nv = len(off) - 1
+    rec = np.zeros(len(col), dtype=[("t", "<i8"), ("w", "<f8")])
+    rec["t"], rec["w"] = col, w
+    with open(path, "wb") as f:
+        np.array([nv, len(col) if ne_header is None else ne_header], "<i8").tofile(f)
+        (np.asarray(off, np.int64) + base).astype("<i8").tofile(f)
+        rec.tofile(f)
+
+
+def test_reader_matches_numpy_writer_and_roundtrips(tmp_path):
+    off = np.array([0, 2, 3, 5, 6])
+    col = np.array([1, 2, 0, 0, 3, 2])
+    w = np.array([0.5, 1.5, 0.5, 1.5, 2.0, 2.0])
+    p = tmp_path / "g.bin"
+    write_reference_file(p, off, col, w)
+    g = na.Graph.read(str(p))
+    o2, c2, w2 = g.arrays()
+    assert g.num_vertices == 4 and g.num_edges == 6
+    assert list(o2) == list(off) and list(c2) == list(col) and list(w2) == list(w)
+    q = tmp_path / "g2.bin"
+    g.write(str(q))
+    assert p.read_bytes() == q.read_bytes()
+
+
+def test_reader_rebases_offsets_and_trusts_them_over_the_header(tmp_path):
+    # graph.cpp:376-382 and 470-472: the arc count comes from the offsets and
+    # offsets are rebased to start at 0
+    off = np.array([0, 1, 2])
+    p = tmp_path / "g.bin"
+    write_reference_file(p, off, np.array([1, 0]), np.array([1.0, 1.0]), base=0, ne_header=99)
+    g = na.Graph.read(str(p))
+    assert g.num_edges == 2
+    o2, _, _ = g.arrays()
+    assert list(o2) == [0, 1, 2]
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(na.NidoError):
+        na.Graph.read(str(tmp_path / "missing.bin"))
+    p = tmp_path / "short.bin"
+    p.write_bytes(np.array([10, 5], "<i8").tobytes())
+    with pytest.raises(na.NidoError):
+        na.Graph.read(str(p))
+
+
+def check_simple_symmetric(off, col, w):
+    nv = len(off) - 1
+    assert off[0] == 0 and np.all(np.diff(off) >= 0)
+    src = np.repeat(np.arange(nv), np.diff(off))
+    assert np.all(src != col), "self-loop"
+    for v in range(min(nv, 2000)):
+        row = col[off[v]:off[v + 1]]
+        assert np.all(np.diff(row) > 0), "unsorted or duplicate row"
+    fwd = {(int(a), int(b)): float(x) for a, b, x in zip(src, col, w)}
+    for (a, b), x in list(fwd.items())[:50000]:
+        assert fwd[(b, a)] == x
+
+
+def test_sbm_generator():
+    g = na.gen_sbm(1024, 64, 0.30, 0.005, False, 1)
+    off, col, w = g.arrays()
+    check_simple_symmetric(off, col, w)
+    assert np.all(w == 1.0)
+    exp = 2 * (16 * 64 * 63 / 2 * 0.30 + 120 * 64 * 64 * 0.005)
+    assert abs(len(col) - exp) < 0.05 * exp
+    src = np.repeat(np.arange(1024), np.diff(off))
+    intra = np.mean(src // 64 == col // 64)
+    exp_intra = (16 * 64 * 63 * 0.30) / exp          # ~0.797
+    assert abs(intra - exp_intra) < 0.02
+    off2, col2, _ = na.gen_sbm(1024, 64, 0.30, 0.005, False, 1).arrays()
+    assert np.array_equal(off, off2) and np.array_equal(col, col2)       # seeded
+    off3, _, _ = na.gen_sbm(1024, 64, 0.30, 0.005, False, 2).arrays()
+    assert not np.array_equal(off, off3)
+
+
+def test_weighted_generators_use_the_unit_interval_grid():
+    off, col, w = na.gen_sbm(500, 50, 0.2, 0.01, True, 3).arrays()
+    check_simple_symmetric(off, col, w)
+    assert np.all((w > 0) & (w <= 1))
+    assert np.all(w * 2.0 ** 53 == np.round(w * 2.0 ** 53))
+
+
+def test_rmat_generator():
+    off, col, w = na.gen_rmat(12, 8, 0.57, 0.19, 0.19, True, 7).arrays()
+    check_simple_symmetric(off, col, w)
+    assert len(off) == 4097
+    assert 0.6 * 2 * 8 * 4096 < len(col) <= 2 * 8 * 4096
+    deg = np.diff(off)
+    assert deg[:64].mean() > 4 * deg.mean()          # skew towards low ids
+
+
+def test_chung_lu_generator():
+    off, col, w = na.gen_chung_lu(20000, 2.1, 2.0, 500.0, 16.0, False, 11).arrays()
+    check_simple_symmetric(off, col, w)
+    mean = len(col) / 20000
+    assert 10 < mean < 17
+    assert np.diff(off).max() > 100
+
+
+def test_generator_argument_errors():
+    with pytest.raises(na.NidoError):
+        na.gen_sbm(-1, 64, 0.3, 0.1, False, 1)
+    with pytest.raises(na.NidoError):
+        na.gen_rmat(40, 16, 0.57, 0.19, 0.19, False, 1)
+
+
+def test_device_entry_points_validate_before_touching_the_gpu():
+    # rejected on the host: out-of-range column, NaN weight, zero batches
+    off = np.array([0, 1, 2])
+    with pytest.raises(na.NidoError) as e:
+        na.GraphGPU((off, np.array([1, 5]), np.array([1.0, 1.0])), 2)
+    assert e.value.code == -1
+    with pytest.raises(na.NidoError):
+        na.GraphGPU((off, np.array([1, 0]), np.array([1.0, np.nan])), 2)
+    with pytest.raises(na.NidoError):
+        na.GraphGPU((off, np.array([1, 0]), np.array([1.0, 1.0])), 0)
+
+
+def test_batch_bounds_host_matches_oracle():
+    import oracle
+    off, _, _ = na.gen_rmat(10, 8, 0.57, 0.19, 0.19, False, 1).arrays()
+    for nb in (1, 2, 7, 32):
+        assert np.array_equal(na.batch_bounds_host(off, nb), oracle.batch_bounds(off, nb))
