@@ -523,7 +523,7 @@ int do_lm(nido_graph* g, int b)
     // launch 1 fills the long rows' community keys that launch 2 reads
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
-        launch_lm_small(A, g->small_bins[(size_t)b], g->stream);
+        launch_lm_small(A, g->small_bins[(size_t)b], g->tuning.split_small != 0, g->stream);
     }
     if (g->big_args[(size_t)b].nitems > 0) {
         ProfScope ps(g->prof, g->stream, NIDO_K_LM_LONG, g->long_bytes[(size_t)b]);

@@ -117,7 +117,8 @@ void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s)
 // Fills L.blk0 from L.n and L.nchunks; returns the number of blocks of the
 // fused launch.
 int small_bins_blocks(SmallBins& L, unsigned cap_per_bin);
-void launch_lm_small(const LmArgs& A, const SmallBins& L, hipStream_t s);
+// split: one launch per bin (and one for the key fill) instead of one fused launch
+void launch_lm_small(const LmArgs& A, const SmallBins& L, bool split, hipStream_t s);
 void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s);
 
 void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, const fx_t* k, fx_t* a,

@@ -619,10 +619,21 @@ int small_bins_blocks(SmallBins& L, unsigned cap_per_bin)
     return total;
 }
 
-void launch_lm_small(const LmArgs& A, const SmallBins& L, hipStream_t s)
+void launch_lm_small(const LmArgs& A, const SmallBins& L, bool split, hipStream_t s)
 {
     const int blocks = L.blk0[NUM_SMALL_BINS + 1];
-    if (blocks > 0) lm_small_kernel<<<blocks, kBlock, 0, s>>>(A, L);
+    if (blocks <= 0) return;
+    if (!split) {
+        lm_small_kernel<<<blocks, kBlock, 0, s>>>(A, L);
+        return;
+    }
+    for (int b = 0; b <= NUM_SMALL_BINS; ++b) {   // bin b alone (b == NUM_SMALL_BINS: key fill)
+        SmallBins one = L;
+        for (int k = 0; k < NUM_SMALL_BINS; ++k) one.n[k] = k == b ? L.n[k] : 0;
+        if (b != NUM_SMALL_BINS) one.nchunks = 0;
+        if (small_bins_blocks(one, 4096) > 0)
+            lm_small_kernel<<<one.blk0[NUM_SMALL_BINS + 1], kBlock, 0, s>>>(A, one);
+    }
 }
 
 void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s)

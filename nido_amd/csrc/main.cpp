@@ -29,7 +29,8 @@ int main(int argc, char** argv)
         case 'p':
         case 'r':
         case 'c':
-            std::cerr << "-" << (char)opt << " is not part of this single-GPU build (see DESIGN.md)" << std::endl;
+            std::cerr << "-" << (char)opt << " is not supported: this is a single-GPU Louvain driver with the graph resident in HBM"
+                      << std::endl;
             return 2;
         case 'h':
             std::cout << "usage: nido_louvain -f <graph.bin> [-b batches] [-t threshold] [-i max-iterations] "

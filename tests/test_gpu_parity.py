@@ -124,6 +124,7 @@ KNOBS = [
     {"big_min_deg": 16, "big_part_arcs": 4},                        # many partitions per row
     {"big_min_deg": 0, "big_part_arcs": 1},                         # every row partitioned
     {"big_min_deg": 32, "big_part_arcs": 64, "big_key_limit": 2},   # forces the HBM fallback
+    {"split_small": 1},                                             # one launch per short-row bin
 ]
 
 
