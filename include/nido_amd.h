@@ -79,7 +79,7 @@ typedef struct {
     int32_t use_graph;        /* capture each iteration in a hipGraph (0/1) */
     int32_t big_slots;        /* LDS table of a long-row workgroup: 2048 or 4096 slots */
     int32_t split_small;      /* launch every short-row bin as a kernel of its own (0/1; profiling) */
-    int32_t reserved[1];
+    int32_t overlap_long;     /* run the long rows of a batch on a second stream, beside the short rows (0/1) */
 } nido_tuning;
 
 const char* nido_last_error(void);

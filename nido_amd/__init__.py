@@ -62,7 +62,7 @@ class Tuning(C.Structure):
     _fields_ = [("thread_max_deg", C.c_int32), ("big_min_deg", C.c_int32),
                 ("big_part_arcs", C.c_int32), ("big_key_limit", C.c_int32),
                 ("use_graph", C.c_int32), ("big_slots", C.c_int32), ("split_small", C.c_int32),
-                ("reserved", C.c_int32 * 1)]
+                ("overlap_long", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -58,6 +58,30 @@ __device__ __forceinline__ fx_t ld_fx_agent(const fx_t* p)
     return fx_make(ld_u64_agent(q), (int64_t)ld_u64_agent(q + 1));
 }
 
+// Workgroup-wide exclusive scan of one int per thread; *total receives the
+// sum.  `wtot` holds kBlock/64 ints of LDS; the caller synchronises before
+// wtot is used again.
+__device__ __forceinline__ int block_excl_scan_i32(int x, int* wtot, int* total)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+    }
+    if (lane == 63) wtot[wv] = inc;
+    __syncthreads();
+    int pre = 0, all = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) {
+        if (i < wv) pre += wtot[i];
+        all += wtot[i];
+    }
+    *total = all;
+    return pre + inc - x;
+}
+
 // Workgroup-wide argmax + self-weight reduction; result valid in thread 0.
 struct BlockRed {
     double g[kBlock / 64];

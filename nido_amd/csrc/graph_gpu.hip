@@ -153,6 +153,8 @@ struct ProfScope {
 struct nido_graph {
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipStream_t side = nullptr;    // long rows of a batch, beside the short rows on `stream`
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = 0;
     int nb = NIDO_DEFAULT_BATCHES;
     nido_tuning tuning{};
@@ -191,8 +193,11 @@ struct nido_graph {
     std::vector<BigArgs> big_args;       // nb
     BigItem* d_big_items = nullptr;
     KeyChunk* d_key_chunks = nullptr;
-    int* d_ckey = nullptr;
+    int* d_bkey = nullptr;         // long-row buckets: keys, weights, segment starts
+    double* d_bw = nullptr;
     int64_t key_cap = 0;
+    int* d_cpo = nullptr;
+    int64_t cpo_cap = 0;
     int64_t* d_big_poff = nullptr;
     int64_t* d_big_toff = nullptr;
     double* d_pg = nullptr;
@@ -242,6 +247,8 @@ void default_tuning(nido_tuning& t)
     t.big_key_limit = kBigLimit;
     t.use_graph = 0;
     t.big_slots = kBigSlots;
+    t.split_small = 0;
+    t.overlap_long = 1;
 }
 
 // partition_graph_edge_batch with part_on_batch=1 on one device,
@@ -302,8 +309,10 @@ int setup_phase(nido_graph* g)
     g->committed.assign((size_t)g->nb, 0);
 
     // Short rows: vertices grouped by (batch, bin), ascending id inside a
-    // group.  Long rows: per batch, P = 2^ceil(log2(d / kBigPartArcs))
-    // partition items, their partial slots and HBM fallback tables.
+    // group; isolated vertices are left out (they never move, and C' = C
+    // holds for them throughout).  Long rows: per batch, P =
+    // 2^ceil(log2(d / part_arcs)) partition items, their partial slots, HBM
+    // fallback tables, and for P > 1 the key-fill chunks and bucket space.
     std::vector<int> binv((size_t)std::max<int64_t>(nv, 1));
     std::vector<BigItem> bitems;
     std::vector<KeyChunk> chunks;
@@ -311,14 +320,15 @@ int setup_phase(nido_graph* g)
     std::vector<int64_t> item_base((size_t)g->nb), poff_base((size_t)g->nb), toff_base((size_t)g->nb),
         chunk_base((size_t)g->nb + 1);
     std::vector<int64_t> small_start((size_t)g->nb * NUM_SMALL_BINS), small_n((size_t)g->nb * NUM_SMALL_BINS);
-    int64_t pos = 0, tab_need = 0, part_need = 0, slot_need = 0, key_need = 0;
+    int64_t pos = 0, tab_need = 0, part_need = 0, slot_need = 0, key_need = 0, cpo_need = 0;
     g->short_bytes.assign((size_t)g->nb, 0.0);
     g->long_bytes.assign((size_t)g->nb, 0.0);
     for (int b = 0; b < g->nb; ++b) {
         chunk_base[(size_t)b] = (int64_t)chunks.size();
-        int64_t ksum = 0;
+        int64_t ksum = 0, csum = 0;
         int64_t cnt[NUM_BINS] = {0};
-        for (int64_t v = g->parts[b]; v < g->parts[b + 1]; ++v) cnt[bin_of(off[v + 1] - off[v], g->tuning)]++;
+        for (int64_t v = g->parts[b]; v < g->parts[b + 1]; ++v)
+            if (off[v + 1] > off[v]) cnt[bin_of(off[v + 1] - off[v], g->tuning)]++;
         int64_t cur[NUM_SMALL_BINS];
         for (int k = 0; k < NUM_SMALL_BINS; ++k) {
             small_start[(size_t)b * NUM_SMALL_BINS + k] = pos;
@@ -338,16 +348,22 @@ int setup_phase(nido_graph* g)
             // algorithmic bytes of the sweep: 24 per arc (col, weight,
             // C[col]) + 40 per vertex (offset, C, k, a_C, C')
             (k != BIN_BIG ? g->short_bytes : g->long_bytes)[(size_t)b] += 24.0 * (double)d + 40.0;
+            if (d == 0) continue;
             if (k != BIN_BIG) {
                 binv[(size_t)cur[k]++] = (int)v;
                 continue;
             }
             int P = 1;
-            while (P < 65536 && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
-            for (int p = 0; p < P; ++p) bitems.push_back(BigItem{(int)v, p, P, slot, ksum});
-            for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs)
-                chunks.push_back(KeyChunk{(int)v, e, std::min(off[v + 1], e + kKeyChunkArcs), ksum + (e - off[v])});
-            ksum += d;
+            while (P < kMaxParts && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
+            for (int p = 0; p < P; ++p) bitems.push_back(BigItem{(int)v, p, P, slot, ksum, csum});
+            if (P > 1) {
+                for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs) {
+                    chunks.push_back(KeyChunk{(int)v, P, e, std::min(off[v + 1], e + kKeyChunkArcs),
+                                              ksum + (e - off[v]), csum});
+                    csum += P + 1;
+                }
+                ksum += d;
+            }
             poff.push_back(psum);
             psum += P;
             tsum += pow2_at_least(2 * d);
@@ -363,15 +379,23 @@ int setup_phase(nido_graph* g)
         part_need = std::max(part_need, psum);
         slot_need = std::max<int64_t>(slot_need, slot);
         key_need = std::max(key_need, ksum);
+        cpo_need = std::max(cpo_need, csum);
     }
     chunk_base[(size_t)g->nb] = (int64_t)chunks.size();
     NIDO_TRY(upload(&g->d_binv, binv, g->stream));
     NIDO_TRY(upload(&g->d_big_items, bitems, g->stream));
     NIDO_TRY(upload(&g->d_key_chunks, chunks, g->stream));
     if (key_need > g->key_cap) {
-        dev_free(g->d_ckey);
-        NIDO_TRY(dev_alloc(&g->d_ckey, key_need));
+        dev_free(g->d_bkey);
+        dev_free(g->d_bw);
+        NIDO_TRY(dev_alloc(&g->d_bkey, key_need));
+        NIDO_TRY(dev_alloc(&g->d_bw, key_need));
         g->key_cap = key_need;
+    }
+    if (cpo_need > g->cpo_cap) {
+        dev_free(g->d_cpo);
+        NIDO_TRY(dev_alloc(&g->d_cpo, cpo_need));
+        g->cpo_cap = cpo_need;
     }
     NIDO_TRY(upload(&g->d_big_poff, poff, g->stream));
     NIDO_TRY(upload(&g->d_big_toff, toff, g->stream));
@@ -408,14 +432,18 @@ int setup_phase(nido_graph* g)
         }
         L.chunks = g->d_key_chunks + chunk_base[(size_t)b];
         L.nchunks = chunk_base[(size_t)b + 1] - chunk_base[(size_t)b];
-        L.ckey = g->d_ckey;
+        L.bkey = g->d_bkey;
+        L.bw = g->d_bw;
+        L.cpo = g->d_cpo;
         small_bins_blocks(L, 4096);
         BigArgs& B = g->big_args[(size_t)b];
         const int64_t i0 = item_base[(size_t)b];
         const int64_t i1 = b + 1 < g->nb ? item_base[(size_t)b + 1] : (int64_t)nbig_items;
         B.items = g->d_big_items + i0;
         B.nitems = i1 - i0;
-        B.ckey = g->d_ckey;
+        B.bkey = g->d_bkey;
+        B.bw = g->d_bw;
+        B.cpo = g->d_cpo;
         B.key_limit = g->tuning.big_key_limit;
         B.slots = g->tuning.big_slots;
         B.poff = g->d_big_poff + poff_base[(size_t)b];
@@ -520,14 +548,36 @@ int do_lm(nido_graph* g, int b)
         return NIDO_OK;
     }
     LmArgs A{g->d_off, g->d_col32, g->d_w, g->d_C32, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m};
-    // launch 1 fills the long rows' community keys that launch 2 reads
-    {
-        ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
-        launch_lm_small(A, g->small_bins[(size_t)b], g->tuning.split_small != 0, g->stream);
-    }
-    if (g->big_args[(size_t)b].nitems > 0) {
-        ProfScope ps(g->prof, g->stream, NIDO_K_LM_LONG, g->long_bytes[(size_t)b]);
-        launch_lm_big(A, g->big_args[(size_t)b], g->stream);
+    const SmallBins& L = g->small_bins[(size_t)b];
+    const BigArgs& B = g->big_args[(size_t)b];
+    const bool split = g->tuning.split_small != 0;
+    if (B.nitems > 0 && g->tuning.overlap_long) {
+        // fork: the long rows (key fill, then partitions) run on the side
+        // stream while the short rows run on the main stream; join before
+        // the commit
+        HIP_TRY(hipEventRecord(g->ev_fork, g->stream));
+        HIP_TRY(hipStreamWaitEvent(g->side, g->ev_fork, 0));
+        {
+            ProfScope ps(g->prof, g->side, NIDO_K_LM_LONG, g->long_bytes[(size_t)b]);
+            launch_lm_small(A, L, false, true, split, g->side);
+            launch_lm_big(A, B, g->side);
+        }
+        HIP_TRY(hipEventRecord(g->ev_join, g->side));
+        {
+            ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
+            launch_lm_small(A, L, true, false, split, g->stream);
+        }
+        HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_join, 0));
+    } else {
+        // launch 1 fills the long rows' buckets that launch 2 reads
+        {
+            ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
+            launch_lm_small(A, L, true, true, split, g->stream);
+        }
+        if (B.nitems > 0) {
+            ProfScope ps(g->prof, g->stream, NIDO_K_LM_LONG, g->long_bytes[(size_t)b]);
+            launch_lm_big(A, B, g->stream);
+        }
     }
     HIP_TRY(hipGetLastError());
     return NIDO_OK;
@@ -767,6 +817,10 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
             return fail(set_error(NIDO_ERR_HIP, "hipStreamCreate failed"));
         g->own_stream = true;
     }
+    if (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess)
+        return fail(set_error(NIDO_ERR_HIP, "creating the side stream or its events failed"));
     g->n0 = nv;
     g->nv = nv;
     g->ne = ne;
@@ -855,12 +909,15 @@ int nido_graph_destroy(nido_graph* g)
     dev_free(g->d_clus); dev_free(g->d_work); dev_free(g->d_acc); dev_free(g->d_q); dev_free(g->d_tot);
     dev_free(g->d_src); dev_free(g->d_part); dev_free(g->d_col32); dev_free(g->d_C32);
     dev_free(g->d_binv); dev_free(g->d_big_items); dev_free(g->d_big_poff); dev_free(g->d_big_toff);
-    dev_free(g->d_key_chunks); dev_free(g->d_ckey);
+    dev_free(g->d_key_chunks); dev_free(g->d_bkey); dev_free(g->d_bw); dev_free(g->d_cpo);
     dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt);
     dev_free(g->d_tkey); dev_free(g->d_tval);
     dev_free(g->d_small); dev_free(g->d_items);
     if (g->d_scan_tmp) (void)hipFree(g->d_scan_tmp);
     if (g->h_q) (void)hipHostFree(g->h_q);
+    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+    if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+    if (g->side) (void)hipStreamDestroy(g->side);
     if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return NIDO_OK;

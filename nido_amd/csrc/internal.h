@@ -27,36 +27,46 @@ constexpr int64_t kBigPartArcs = 1536;    // expected distinct keys per partitio
 constexpr int kBigSlots = 4096;           // LDS table of a partition workgroup (64/80 KB)
 constexpr int kBigLimit = kBigSlots / 2;  // distinct keys before the exact HBM fallback
 
-// Arcs [e0, e1) of long row v, whose community keys go to ckey[k0 ...].
+// Arcs [e0, e1) of a long row v split into P > 1 partitions.  The key fill
+// writes the community keys and weights of these arcs grouped by partition
+// into bkey/bw[k0, k0 + (e1 - e0)) (self arcs dropped) and the P + 1
+// segment starts, relative to k0, into cpo[c0, c0 + P + 1).
 struct KeyChunk {
-    int v;
-    int64_t e0, e1, k0;
+    int v, P;
+    int64_t e0, e1, k0, c0;
 };
 constexpr int64_t kKeyChunkArcs = 2048;
+constexpr int kMaxParts = 1024;           // partitions per row, at most
 
 // The first launch of a batch: the small bins plus the key fill of the
-// batch's long rows (ckey[k] = C[col] as int, -1 for a self arc).
+// batch's partitioned long rows.
 struct SmallBins {
     const int* verts[NUM_SMALL_BINS];
     int64_t n[NUM_SMALL_BINS];
     const KeyChunk* chunks;
     int64_t nchunks;
-    int* ckey;
+    int* bkey;
+    double* bw;
+    int* cpo;
     int blk0[NUM_SMALL_BINS + 2];          // first block of each bin, then of the key fill
 };
 
 // One workgroup's share of a long row: partition p of P (P a power of 2);
-// `slot` indexes the row within its batch's list of long rows; the row's
-// keys start at ckey[k0].
+// `slot` indexes the row within its batch's list of long rows.  For P > 1
+// the row's chunk j (of kKeyChunkArcs arcs) has its buckets at
+// bkey/bw[k0 + j * kKeyChunkArcs ...] and its segment starts at
+// cpo[c0 + j * (P + 1) ...]; a row with P == 1 reads its CSR row directly.
 struct BigItem {
     int v, p, P, slot;
-    int64_t k0;
+    int64_t k0, c0;
 };
 
 struct BigArgs {
     const BigItem* items;
     int64_t nitems;
-    const int* ckey;         // community keys of the batch's long rows
+    const int* bkey;         // bucketed community keys of the batch's partitioned rows
+    const double* bw;        // their weights
+    const int* cpo;          // per chunk: P + 1 segment starts
     int key_limit;           // distinct keys a partition may hold in LDS (<= slots/2)
     int narrow;              // every row's weights are >= 0 and sum below 2^31
     int slots;               // LDS table size of a partition workgroup: 2048 or 4096
@@ -82,7 +92,13 @@ struct LmArgs {
     const double* kd;        // k_v rounded to FP64
     const fx_t* k;           // k_v exact
     const fx_t* a;           // a_c exact
-    double m;                // mass, FP64
+    double m;                // mass, FP64 (one graph)
+    // Several graphs in one launch (gid null: one graph of mass m): vertex
+    // v belongs to graph gid[v], of mass mg[gid[v]], and takes part only
+    // while act[gid[v]] is set.
+    const int* gid;
+    const double* mg;
+    const unsigned char* act;
 };
 
 // A slice of one row for the degree / modularity sweeps over long rows.
@@ -117,18 +133,39 @@ void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s)
 // Fills L.blk0 from L.n and L.nchunks; returns the number of blocks of the
 // fused launch.
 int small_bins_blocks(SmallBins& L, unsigned cap_per_bin);
-// split: one launch per bin (and one for the key fill) instead of one fused launch
-void launch_lm_small(const LmArgs& A, const SmallBins& L, bool split, hipStream_t s);
+// bins / fill: launch the short-row bins / the long-row key fill (both: one
+// fused launch); split: one launch per bin instead of one fused launch.
+void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s);
 void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s);
 
 void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, const fx_t* k, fx_t* a,
                    hipStream_t s);
+// The commit over list[0, n) (or [v0, v0 + n) with a null list), restricted
+// to the vertices v whose graph has flag[gid[v]] set.
+void launch_commit_filtered(const int* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
+                            const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s);
 
 // Modularity over the arc-parallel CSR (src = row ids): writes Q to *q;
 // part holds 2 * kPartBlocks partials.
 void launch_modularity(const int* src, const int* col, const double* w, const int* C,
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
                        hipStream_t s);
+
+// ---- several graphs in one CSR ----------------------------------------------
+// Arcs [e0, e1) and vertices [v0, v1) of graph g.
+struct SegItem {
+    int g;
+    int64_t e0, e1, v0, v1;
+};
+// Per-graph modularity: acc holds 2 accumulators per graph, q[g] receives Q_g.
+void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src, const int* col,
+                           const double* w, const int* C, const fx_t* a, const double* twom_g, int G,
+                           fx_t* acc, double* q, hipStream_t s);
+// out[g] = newid[vb[g]] for g <= G (the total where vb[g] == nv).
+void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64_t* vb, int G, int64_t nv,
+                         int64_t* out, hipStream_t s);
+// gid[i] = g for i in [vb[g], vb[g+1]).
+void launch_fill_gid(const int64_t* vb, int G, int* gid, hipStream_t s);
 
 // Exclusive scan of n int64 values (out may alias in); writes the total to
 // *total (device) when total is non-null.
@@ -137,15 +174,20 @@ void launch_exclusive_scan(const int64_t* in, int64_t* out, int64_t n, int64_t* 
                            void* temp, hipStream_t s);
 
 // ---- coarsening ------------------------------------------------------------
-void launch_mark_communities(const int64_t* C, int64_t nv, int64_t* flag, hipStream_t s);
+// With gid (per current vertex) / gid0 (per original vertex) set, only the
+// vertices of graphs g with keep[g] set take part.
+void launch_mark_communities(const int64_t* C, int64_t nv, int64_t* flag, hipStream_t s,
+                             const int* gid = nullptr, const unsigned char* keep = nullptr);
 void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const int64_t* newid,
-                              hipStream_t s);
+                              hipStream_t s, const int* gid0 = nullptr, const unsigned char* keep = nullptr);
 void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t* off, int64_t nv,
-                          int64_t* cnt, int64_t* arcs, hipStream_t s);
+                          int64_t* cnt, int64_t* arcs, hipStream_t s, const int* gid = nullptr,
+                          const unsigned char* keep = nullptr);
 void launch_table_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, hipStream_t s);
 void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
                           const int64_t* newid, int64_t nv, const int64_t* toff, int* tkey,
-                          fx_t* tval, hipStream_t s);
+                          fx_t* tval, hipStream_t s, const int* gid = nullptr,
+                          const unsigned char* keep = nullptr);
 void launch_coarse_count(const int64_t* toff, const int* tkey, int64_t K, int64_t* cnt,
                          hipStream_t s);
 void launch_coarse_emit(const int64_t* toff, const int* tkey, const fx_t* tval, int64_t K,

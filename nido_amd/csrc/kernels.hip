@@ -275,29 +275,33 @@ scan_tile_apply_kernel(const int64_t* in, int64_t* out, int64_t n,
 // Coarsening (graph_gpu.cpp:1151-1304, 1382-1545), all on the device.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
-mark_kernel(const int64_t* __restrict__ C, int64_t nv, int64_t* __restrict__ flag)
+mark_kernel(const int64_t* __restrict__ C, int64_t nv, int64_t* __restrict__ flag,
+            const int* __restrict__ gid, const unsigned char* __restrict__ keep)
 {
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nv;
          i += (int64_t)gridDim.x * kBlock)
-        flag[C[i]] = 1;
+        if (!gid || keep[gid[i]]) flag[C[i]] = 1;
 }
 
 __global__ void __launch_bounds__(kBlock)
 clustering_kernel(int64_t* __restrict__ clus, int64_t n0, const int64_t* __restrict__ C,
-                  const int64_t* __restrict__ newid)
+                  const int64_t* __restrict__ newid, const int* __restrict__ gid0,
+                  const unsigned char* __restrict__ keep)
 {
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n0;
          i += (int64_t)gridDim.x * kBlock)
-        clus[i] = newid[C[clus[i]]];
+        if (!gid0 || keep[gid0[i]]) clus[i] = newid[C[clus[i]]];
 }
 
 __global__ void __launch_bounds__(kBlock)
 member_counts_kernel(const int64_t* __restrict__ C, const int64_t* __restrict__ newid,
                      const int64_t* __restrict__ off, int64_t nv, int64_t* __restrict__ cnt,
-                     int64_t* __restrict__ arcs)
+                     int64_t* __restrict__ arcs, const int* __restrict__ gid,
+                     const unsigned char* __restrict__ keep)
 {
     for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * kBlock) {
+        if (gid && !keep[gid[v]]) continue;
         const int64_t i = newid[C[v]];
         if (cnt) atomicAdd(reinterpret_cast<unsigned long long*>(cnt + i), 1ull);
         const int64_t d = off[v + 1] - off[v];
@@ -351,12 +355,14 @@ coarse_insert_kernel(const int64_t* __restrict__ off, const int64_t* __restrict_
                      const double* __restrict__ w, const int64_t* __restrict__ C,
                      const int64_t* __restrict__ newid, int64_t nv,
                      const int64_t* __restrict__ toff, int* __restrict__ tkey,
-                     fx_t* __restrict__ tval)
+                     fx_t* __restrict__ tval, const int* __restrict__ gid,
+                     const unsigned char* __restrict__ keep)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
     for (int64_t v = wid; v < nv; v += nw) {
+        if (gid && !keep[gid[v]]) continue;   // uniform in the wavefront
         const int64_t i = newid[C[v]];
         const int64_t e1 = off[v + 1];
         for (int64_t e = off[v] + lane; e < e1; e += 64)
@@ -404,6 +410,64 @@ coarse_emit_kernel(const int64_t* __restrict__ toff, const int* __restrict__ tke
             pos += __popcll(m);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Several graphs in one CSR: per-graph modularity, graph ids, vertex bases.
+// ---------------------------------------------------------------------------
+// Item = arcs [e0, e1) and vertices [v0, v1) of graph g; each workgroup sums
+// its item exactly and adds the two sums to graph g's accumulators with
+// integer atomics (order-free, so the result is exact and deterministic).
+__global__ void __launch_bounds__(kBlock)
+mod_seg_kernel(const SegItem* __restrict__ items, int64_t nitems, const int* __restrict__ src,
+               const int* __restrict__ col, const double* __restrict__ w, const int* __restrict__ C,
+               const fx_t* __restrict__ a, const double* __restrict__ twom_g, fx_t* __restrict__ acc)
+{
+    __shared__ fx_t red[kBlock / 64];
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const SegItem s = items[it];
+        fx_t I = 0, T = 0;
+        for (int64_t e = s.e0 + threadIdx.x; e < s.e1; e += kBlock)
+            if (C[src[e]] == C[col[e]]) I += fx_from_double(w[e]);    // self arcs are internal
+        const double tw = twom_g[s.g];
+        for (int64_t i = s.v0 + threadIdx.x; i < s.v1; i += kBlock) {
+            const double x = fx_to_double(a[i]) / tw;
+            T += fx_from_double(x * x);
+        }
+        I = block_sum_fx(I, red);
+        T = block_sum_fx(T, red);
+        if (threadIdx.x == 0) {
+            acc_add(acc + 2 * s.g, I);
+            acc_add(acc + 2 * s.g + 1, T);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+mod_seg_finalize_kernel(const fx_t* __restrict__ acc, const double* __restrict__ twom_g, int G,
+                        double* __restrict__ q)
+{
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock) {
+        const double tw = twom_g[g];
+        q[g] = tw == 0 ? 0.0 : fx_to_double(acc[2 * g]) / tw - fx_to_double(acc[2 * g + 1]);
+    }
+}
+
+// out[g] = newid[vb[g]] (the scan total where vb[g] == nv), g = 0..G.
+__global__ void __launch_bounds__(kBlock)
+gather_bases_kernel(const int64_t* __restrict__ newid, const int64_t* __restrict__ total,
+                    const int64_t* __restrict__ vb, int G, int64_t nv, int64_t* __restrict__ out)
+{
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g <= G; g += gridDim.x * kBlock)
+        out[g] = vb[g] < nv ? newid[vb[g]] : *total;
+}
+
+// gid[i] = g for i in [vb[g], vb[g+1]); one workgroup per graph.
+__global__ void __launch_bounds__(kBlock)
+fill_gid_kernel(const int64_t* __restrict__ vb, int G, int* __restrict__ gid)
+{
+    for (int g = blockIdx.x; g < G; g += gridDim.x)
+        for (int64_t i = vb[g] + threadIdx.x; i < vb[g + 1]; i += kBlock) gid[i] = g;
 }
 
 }  // namespace
@@ -509,23 +573,26 @@ void launch_exclusive_scan(const int64_t* in, int64_t* out, int64_t n, int64_t* 
     scan_tile_apply_kernel<<<(unsigned)tiles, kBlock, 0, s>>>(in, out, n, sums, total);
 }
 
-void launch_mark_communities(const int64_t* C, int64_t nv, int64_t* flag, hipStream_t s)
+void launch_mark_communities(const int64_t* C, int64_t nv, int64_t* flag, hipStream_t s,
+                             const int* gid, const unsigned char* keep)
 {
     launch_fill_i64(flag, 0, nv, s);
-    if (nv > 0) mark_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, nv, flag);
+    if (nv > 0) mark_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, nv, flag, gid, keep);
 }
 
 void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const int64_t* newid,
-                              hipStream_t s)
+                              hipStream_t s, const int* gid0, const unsigned char* keep)
 {
-    if (n0 > 0) clustering_kernel<<<grid_for(n0, kBlock), kBlock, 0, s>>>(clus, n0, C, newid);
+    if (n0 > 0) clustering_kernel<<<grid_for(n0, kBlock), kBlock, 0, s>>>(clus, n0, C, newid, gid0, keep);
 }
 
 void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t* off, int64_t nv,
-                          int64_t* cnt, int64_t* arcs, hipStream_t s)
+                          int64_t* cnt, int64_t* arcs, hipStream_t s, const int* gid,
+                          const unsigned char* keep)
 {
     if (nv > 0)
-        member_counts_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, newid, off, nv, cnt, arcs);
+        member_counts_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, newid, off, nv, cnt, arcs, gid,
+                                                                      keep);
 }
 
 void launch_table_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, hipStream_t s)
@@ -535,11 +602,34 @@ void launch_table_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, hipStream_
 
 void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
                           const int64_t* newid, int64_t nv, const int64_t* toff, int* tkey,
-                          fx_t* tval, hipStream_t s)
+                          fx_t* tval, hipStream_t s, const int* gid, const unsigned char* keep)
 {
     if (nv > 0)
         coarse_insert_kernel<<<grid_for(nv, kBlock / 64), kBlock, 0, s>>>(off, col, w, C, newid, nv,
-                                                                          toff, tkey, tval);
+                                                                          toff, tkey, tval, gid, keep);
+}
+
+void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src, const int* col,
+                           const double* w, const int* C, const fx_t* a, const double* twom_g, int G,
+                           fx_t* acc, double* q, hipStream_t s)
+{
+    if (G <= 0) return;
+    launch_fill_fx(acc, 2 * (int64_t)G, s);
+    if (nitems > 0)
+        mod_seg_kernel<<<grid_for(nitems, 1, 8192), kBlock, 0, s>>>(items, nitems, src, col, w, C, a,
+                                                                    twom_g, acc);
+    mod_seg_finalize_kernel<<<grid_for(G, kBlock), kBlock, 0, s>>>(acc, twom_g, G, q);
+}
+
+void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64_t* vb, int G, int64_t nv,
+                         int64_t* out, hipStream_t s)
+{
+    gather_bases_kernel<<<grid_for(G + 1, kBlock), kBlock, 0, s>>>(newid, total, vb, G, nv, out);
+}
+
+void launch_fill_gid(const int64_t* vb, int G, int* gid, hipStream_t s)
+{
+    if (G > 0) fill_gid_kernel<<<grid_for(G, 1, 8192), kBlock, 0, s>>>(vb, G, gid);
 }
 
 void launch_coarse_count(const int64_t* toff, const int* tkey, int64_t K, int64_t* cnt,

@@ -12,15 +12,15 @@
 //   d <= 8      one lane per vertex, neighbours in registers
 //   d <= 16/32/64/128   8/16/32/64 lanes per vertex, LDS table of 2d..4d slots
 //   d <= 512    one workgroup per vertex, 1024-slot LDS table
-//   d >  512    long rows.  Launch 1 stores each arc's community as a 4-byte
-//               key (ckey).  Launch 2 splits the community ids into P hash
-//               partitions: workgroup (v, p) streams the row's keys (L2
-//               resident, coalesced) and aggregates the arcs whose community
-//               is in partition p, so every e_{v,c} is complete and exact in
-//               one workgroup's LDS table; the last of the P workgroups to
-//               finish combines the P partial argmaxes.
+//   d >  512    long rows, split by a hash of the community id into
+//               P = 2^ceil(log2(d / part_arcs)) partitions, so that every
+//               e_{v,c} is complete and exact in one workgroup's LDS table;
+//               the last of the P workgroups to finish combines the P
+//               partial argmaxes.  For P > 1, launch 1 buckets the row's
+//               arcs by partition (a counting sort per 2048-arc chunk) and
+//               workgroup (v, p) of launch 2 reads only its own buckets.
 // Launch 1 = short-row bins + long-row key fill (20 KB LDS per workgroup);
-// launch 2 = long-row partitions (80 KB LDS).  Row loops keep two to eight
+// launch 2 = long-row partitions (64-80 KB LDS).  Row loops keep two to eight
 // independent loads in flight per lane.
 #include "device_common.h"
 
@@ -28,25 +28,36 @@ namespace nido {
 namespace {
 
 // g_c = (e_{v,c} - k_v a_c / (2m)) / m, graph_cuda.cu:1505-1506.
-__device__ __forceinline__ double gain_of(const LmArgs& A, double ki, fx_t e, int c)
+__device__ __forceinline__ double gain_of(const LmArgs& A, double m, double ki, fx_t e, int c)
 {
     double g = fx_to_double(e);
     const double ac = fx_to_double(A.a[c]);
-    g -= ki * ac / (2. * A.m);
-    g /= A.m;
+    g -= ki * ac / (2. * m);
+    g /= m;
     return g;
+}
+
+// Mass of v's graph; false when that graph sits out the iteration.
+__device__ __forceinline__ bool graph_mass(const LmArgs& A, int v, double& m)
+{
+    if (!A.gid) {
+        m = A.m;
+        return true;
+    }
+    const int gg = A.gid[v];
+    m = A.mg[gg];
+    return A.act[gg] != 0;
 }
 
 // Move iff g_best - s' > 0 with s' = (e_{v,own} - k_v (a_own - k_v)/(2m))/m,
 // graph_cuda.cu:1527-1536.
-__device__ __forceinline__ int64_t decide(const LmArgs& A, int v, int own, fx_t self,
+// `rest` = a_own - k_v, loaded by the caller ahead of the row scan.
+__device__ __forceinline__ int64_t decide(double m, double ki, fx_t rest, int own, fx_t self,
                                           double best_g, int best_c)
 {
     if (best_c == kNone) return own;    // gain stays -MAX_FLOAT, graph_cuda.cu:1489
-    const double m = A.m;
-    const double ki = A.kd[v];
     double s = fx_to_double(self);
-    const double ar = fx_to_double(A.a[own] - A.k[v]);
+    const double ar = fx_to_double(rest);
     s -= ki * ar / (2. * m);
     s /= m;
     double gain = best_g;
@@ -54,19 +65,40 @@ __device__ __forceinline__ int64_t decide(const LmArgs& A, int v, int own, fx_t 
     return gain > 0 ? (int64_t)best_c : (int64_t)own;
 }
 
+__device__ __forceinline__ int64_t decide(const LmArgs& A, double m, int v, int own, fx_t self,
+                                          double best_g, int best_c)
+{
+    return decide(m, A.kd[v], A.a[own] - A.k[v], own, self, best_g, best_c);
+}
+
 // Candidate update for one table entry.
-__device__ __forceinline__ void consider(const LmArgs& A, double ki, int own, int c, fx_t e,
+__device__ __forceinline__ void consider(const LmArgs& A, double m, double ki, int own, int c, fx_t e,
                                          fx_t& self, double& bg, int& bc)
 {
     if (c == own) {
         self = e;
         return;
     }
-    const double g = gain_of(A, ki, e, c);
+    const double g = gain_of(A, m, ki, e, c);
     if (better(g, c, bg, bc)) {
         bg = g;
         bc = c;
     }
+}
+
+// Makes one lane's LDS writes visible to the other lanes of its wavefront.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Set bits of `mask` below this lane.
+__device__ __forceinline__ int lane_rank(unsigned long long mask)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
 // Insert (c, w) into an LDS open-addressing table of S slots (S a power of
@@ -115,9 +147,14 @@ __device__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int6
 {
     for (int64_t i = (int64_t)bi * kBlock + threadIdx.x; i < n; i += (int64_t)nbk * kBlock) {
         const int v = verts[i];
-        const int own = (int)A.C[v];
+        double m;
+        if (!graph_mass(A, v, m)) continue;
+        const int own = A.C[v];
+        const double ki = A.kd[v];
+        const fx_t kv = A.k[v];
         const int64_t e0 = A.off[v];
         const int d = (int)(A.off[v + 1] - e0);
+        const fx_t rest = A.a[own] - kv;
         int uu[kThreadMaxDeg];
         double wd[kThreadMaxDeg];
 #pragma unroll
@@ -128,7 +165,6 @@ __device__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int6
         int cc[kThreadMaxDeg];
 #pragma unroll
         for (int j = 0; j < kThreadMaxDeg; ++j) cc[j] = uu[j] != v ? (int)A.C[uu[j]] : -1;
-        const double ki = A.kd[v];
         fx_t self = 0;
         double bg = -kMaxFloat;
         int bc = kNone;
@@ -142,9 +178,9 @@ __device__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int6
 #pragma unroll
             for (int q = j + 1; q < kThreadMaxDeg; ++q)
                 if (cc[q] == cc[j]) e += fx_from_double(wd[q]);
-            consider(A, ki, own, cc[j], e, self, bg, bc);
+            consider(A, m, ki, own, cc[j], e, self, bg, bc);
         }
-        A.Cn[v] = decide(A, v, own, self, bg, bc);
+        A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
     }
 }
 
@@ -170,21 +206,24 @@ __device__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64
             lo[s] = 0;
             hi[s] = 0;
         }
-        __syncthreads();
         const int64_t idx = base + grp;
-        const bool act = idx < n;
+        bool act = idx < n;
         const int v = act ? verts[idx] : 0;
+        double m = 0.0;
+        act = act && graph_mass(A, v, m);
+        const int own = act ? A.C[v] : 0;
+        const double ki = act ? A.kd[v] : 0.0;
+        const fx_t rest = act ? A.a[own] - A.k[v] : fx_t(0);
+        wave_lds_sync();
         if (act) scan_row<S>(A, v, lane, G, key, lo, hi);
-        __syncthreads();
+        wave_lds_sync();
         if (act) {   // uniform within the group
-            const int own = (int)A.C[v];
-            const double ki = A.kd[v];
             fx_t self = 0;
             double bg = -kMaxFloat;
             int bc = kNone;
             for (int s = lane; s < S; s += G) {
                 const int c = key[s];
-                if (c >= 0) consider(A, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+                if (c >= 0) consider(A, m, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
             }
 #pragma unroll
             for (int msk = G / 2; msk > 0; msk >>= 1) {
@@ -196,9 +235,9 @@ __device__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64
                     bc = oc;
                 }
             }
-            if (lane == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+            if (lane == 0) A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 
@@ -216,47 +255,100 @@ __device__ void block_body(const LmArgs& A, const int* __restrict__ verts, int64
             lo[s] = 0;
             hi[s] = 0;
         }
-        __syncthreads();
         const int v = verts[idx];
+        double m;
+        if (!graph_mass(A, v, m)) continue;   // uniform: the workgroup holds one vertex
+        const int own = A.C[v];
+        const double ki = A.kd[v];
+        const fx_t rest = A.a[own] - A.k[v];
+        __syncthreads();
         scan_row<S>(A, v, threadIdx.x, kBlock, key, lo, hi);
         __syncthreads();
-        const int own = (int)A.C[v];
-        const double ki = A.kd[v];
         fx_t self = 0;
         double bg = -kMaxFloat;
         int bc = kNone;
         for (int s = threadIdx.x; s < S; s += kBlock) {
             const int c = key[s];
-            if (c >= 0) consider(A, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+            if (c >= 0) consider(A, m, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
         }
         block_reduce(R, self, bg, bc);
-        if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+        if (threadIdx.x == 0) A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
     }
 }
 
-// ckey[k0 + i] = C[col[e0 + i]] for the long rows' arcs (-1 for a self arc).
+// Key fill of one chunk of a partitioned long row: an LDS counting sort of
+// the chunk's arcs by partition.  Each arc takes its rank inside its
+// partition from an LDS atomic (the order inside a bucket is immaterial: the
+// sums are exact and ties are broken by id); the exclusive scan of the P + 1
+// counters -- the last one counts self arcs, which are dropped -- gives the
+// segment starts.
 constexpr int kFillUnroll = (int)(kKeyChunkArcs / kBlock);
+constexpr int kFillScan = (kMaxParts + 1 + kBlock - 1) / kBlock;   // counters per thread in the scan
 
-__device__ void keyfill_body(const LmArgs& A, const SmallBins& L, int bi, int nbk)
+__device__ void keyfill_body(const LmArgs& A, const SmallBins& L, int bi, int nbk, char* lds)
 {
+    int* cnt = reinterpret_cast<int*>(lds);      // kMaxParts + 1 counters
+    int* wtot = cnt + kMaxParts + 1;
     for (int64_t i = bi; i < L.nchunks; i += nbk) {
         const KeyChunk ch = L.chunks[i];
+        double m;
+        if (!graph_mass(A, ch.v, m)) continue;   // uniform over the workgroup
+        const int P = ch.P;
+        const int psh = 32 - (31 - __builtin_clz((unsigned)P));   // top log2(P) bits of the hash
+        for (int s = threadIdx.x; s <= P; s += kBlock) cnt[s] = 0;
+        __syncthreads();
         int uu[kFillUnroll];
+        double ww[kFillUnroll];
 #pragma unroll
         for (int j = 0; j < kFillUnroll; ++j) {
             const int64_t e = ch.e0 + j * kBlock + threadIdx.x;
-            uu[j] = e < ch.e1 ? (int)A.col[e] : ch.v;
+            uu[j] = e < ch.e1 ? A.col[e] : -1;
+            ww[j] = e < ch.e1 ? A.w[e] : 0.0;
         }
+        int cc[kFillUnroll], pp[kFillUnroll], rk[kFillUnroll];
 #pragma unroll
         for (int j = 0; j < kFillUnroll; ++j) {
-            const int64_t e = ch.e0 + j * kBlock + threadIdx.x;
-            const int c = uu[j] != ch.v ? (int)A.C[uu[j]] : -1;
-            if (e < ch.e1) L.ckey[ch.k0 + (e - ch.e0)] = c;
+            cc[j] = uu[j] >= 0 && uu[j] != ch.v ? A.C[uu[j]] : -1;
+            pp[j] = uu[j] < 0 ? -1 : cc[j] < 0 ? P : (int)(((unsigned)cc[j] * kPartMul) >> psh);
         }
+#pragma unroll
+        for (int j = 0; j < kFillUnroll; ++j) rk[j] = pp[j] >= 0 ? atomicAdd(&cnt[pp[j]], 1) : 0;
+        __syncthreads();
+        // thread t scans counters [t * kFillScan, t * kFillScan + kFillScan)
+        int loc[kFillScan];
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < kFillScan; ++q) {
+            const int idx = threadIdx.x * kFillScan + q;
+            loc[q] = idx <= P ? cnt[idx] : 0;
+            sum += loc[q];
+        }
+        int tot;
+        int pre = block_excl_scan_i32(sum, wtot, &tot);   // synchronises after every read of cnt
+#pragma unroll
+        for (int q = 0; q < kFillScan; ++q) {
+            const int idx = threadIdx.x * kFillScan + q;
+            if (idx <= P) {
+                cnt[idx] = pre;
+                L.cpo[ch.c0 + idx] = pre;
+            }
+            pre += loc[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kFillUnroll; ++j) {
+            if (pp[j] >= 0 && pp[j] < P) {
+                const int64_t pos = ch.k0 + cnt[pp[j]] + rk[j];
+                L.bkey[pos] = cc[j];
+                L.bw[pos] = ww[j];
+            }
+        }
+        __syncthreads();          // cnt and wtot are reused by the next chunk
     }
 }
 
 constexpr int kSmallLds = 20480;   // (256/G)*S*20 for every short-row bin
+static_assert((kMaxParts + 1 + kBlock / 64) * 4 <= kSmallLds, "key-fill counters exceed the LDS buffer");
 
 __global__ void __launch_bounds__(kBlock)
 lm_small_kernel(LmArgs A, SmallBins L)
@@ -274,7 +366,7 @@ lm_small_kernel(LmArgs A, SmallBins L)
     case BIN_G32: group_body<32, 128>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
     case BIN_G64: group_body<64, 256>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
     case BIN_B256: block_body<1024>(A, L.verts[bin], L.n[bin], bi, nbk, lds, R); break;
-    default: keyfill_body(A, L, bi, nbk); break;
+    default: keyfill_body(A, L, bi, nbk, lds); break;
     }
 }
 
@@ -282,7 +374,7 @@ lm_small_kernel(LmArgs A, SmallBins L)
 
 // Whole-row aggregation in an HBM table of `size` slots (>= 2 * row length):
 // the exact fallback taken when a partition outgrows its LDS table.
-__device__ void global_row_decide(const LmArgs& A, int v, int* __restrict__ key,
+__device__ void global_row_decide(const LmArgs& A, double m, int v, int* __restrict__ key,
                                   fx_t* __restrict__ tval, int64_t size, BlockRed& R)
 {
     unsigned long long* val = reinterpret_cast<unsigned long long*>(tval);
@@ -321,13 +413,12 @@ __device__ void global_row_decide(const LmArgs& A, int v, int* __restrict__ key,
     int bc = kNone;
     for (int64_t s = threadIdx.x; s < size; s += kBlock) {
         const int c = ld_i32_agent(&key[s]);
-        if (c >= 0) consider(A, ki, own, c, ld_fx_agent(tval + s), self, bg, bc);
+        if (c >= 0) consider(A, m, ki, own, c, ld_fx_agent(tval + s), self, bg, bc);
     }
     block_reduce(R, self, bg, bc);
-    if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+    if (threadIdx.x == 0) A.Cn[v] = decide(A, m, v, own, self, bg, bc);
 }
 
-constexpr int kBigUnroll = 8;
 constexpr int kMaxProbe = 256;
 
 template <bool Narrow> struct HiWord { using type = unsigned long long; };
@@ -342,33 +433,19 @@ __device__ __forceinline__ void lds_add(unsigned long long* lo, int* hi, fx_t w)
     fx_atomic_add_narrow(lo, hi, w);
 }
 
-// Makes one lane's LDS writes visible to the other lanes of its wavefront.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Set bits of `mask` below this lane.
-__device__ __forceinline__ int lane_rank(unsigned long long mask)
-{
-    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
-                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-}
-
-constexpr int kQueue = 128;   // per-wavefront compaction queue (entries)
+constexpr int kQueue = 128;   // per-wavefront compaction queue of the sweep (entries)
 constexpr int kOverflow = -2; // partial community of a partition that outgrew its table
 
 // S-slot LDS table per workgroup.  Narrow: 32-bit high words (S*16 bytes),
 // used when every row's weights are non-negative and sum to less than 2^31.
-// Items come longest row first, so the long poles start first; partitions of
-// one row spread over all XCDs (a row's keys are small next to L2).
+// Items come longest row first, so the long poles start first.
 //
-// Only ~1/P of a row's keys belong to a partition, so the keys that do (and
-// later the occupied table slots) are first compacted into a per-wavefront
-// LDS queue and then processed 64 at a time with every lane busy; without it
-// a wavefront would run the whole exact insert for one or two live lanes.
+// A row with P == 1 is read straight from the CSR.  For P > 1 the key fill
+// has bucketed each chunk of the row by partition, so workgroup (v, p) reads
+// only its own segment of every chunk: the segment lengths of up to 256
+// chunks are scanned across the workgroup and the concatenated segments are
+// processed 256 entries at a time, each thread locating its chunk by binary
+// search in the scanned starts.  Every key a workgroup reads belongs to it.
 template <bool Narrow, int S>
 __global__ void __launch_bounds__(kBlock)
 lm_big_kernel(LmArgs A, BigArgs B)
@@ -378,28 +455,47 @@ lm_big_kernel(LmArgs A, BigArgs B)
     __shared__ unsigned long long lo[S];
     __shared__ Hi hi[S];
     __shared__ int qkey[kBlock / 64][kQueue];
-    __shared__ int qidx[kBlock / 64][kQueue];
+    __shared__ int s_pre[kBlock], s_lo[kBlock], wtot[kBlock / 64];
     __shared__ BlockRed R;
     __shared__ int s_nkeys, s_ovf, s_last;
     const int lane = threadIdx.x & 63;
     int* const qk = qkey[threadIdx.x >> 6];
-    int* const qi = qidx[threadIdx.x >> 6];
 
     for (int64_t it = blockIdx.x; it < B.nitems; it += gridDim.x) {
         const BigItem item = B.items[it];
         const int v = item.v, p = item.p, P = item.P, slot = item.slot;
-        const int psh = 32 - (31 - __builtin_clz((unsigned)P));   // top log2(P) bits select the partition
-        const int own = (int)A.C[v];
+        double m;
+        if (!graph_mass(A, v, m)) continue;   // uniform; every partition of the row skips it
+        const int own = A.C[v];
+        const double ki = A.kd[v];
         const int64_t e0 = A.off[v];
         const int64_t d = A.off[v + 1] - e0;
-        // A single-partition row holds at most d keys: its table uses the
-        // smallest power of two >= 2d slots (at least 256), so the clear and
-        // the sweep cost O(d) rather than O(S).
-        int lg = ilog2c(S);
-        if (P == 1) {
-            lg = 8;
-            while ((1ll << lg) < 2 * d && lg < ilog2c(S)) ++lg;
-        }
+        const int nch = P > 1 ? (int)((d + kKeyChunkArcs - 1) / kKeyChunkArcs) : 0;
+
+        // Segments of chunks [g * kBlock, g * kBlock + kBlock) of partition p:
+        // s_pre = exclusive scan of their lengths, s_lo = their starts.
+        auto scan_group = [&](int g) -> int {
+            const int j = g * kBlock + threadIdx.x;
+            int l0 = 0, len = 0;
+            if (j < nch) {
+                const int* cp = B.cpo + item.c0 + (int64_t)j * (P + 1) + p;
+                l0 = cp[0];
+                len = cp[1] - l0;
+            }
+            int tot;
+            const int pre = block_excl_scan_i32(len, wtot, &tot);
+            s_pre[threadIdx.x] = pre;
+            s_lo[threadIdx.x] = l0;
+            __syncthreads();
+            return tot;
+        };
+        int total = P > 1 ? scan_group(0) : (int)d;
+        // The table needs at most 2x the partition's entries (itself a bound
+        // on its distinct keys): the smallest power of two >= 2 * entries, at
+        // least 256, so the clear and the sweep cost O(entries) rather than O(S).
+        const int64_t entries = P > 1 && nch > kBlock ? (int64_t)S : (int64_t)total;
+        int lg = 8;
+        while ((1ll << lg) < 2 * entries && lg < ilog2c(S)) ++lg;
         const int ts = 1 << lg;
         for (int s = threadIdx.x; s < ts; s += kBlock) {
             key[s] = -1;
@@ -411,21 +507,17 @@ lm_big_kernel(LmArgs A, BigArgs B)
             s_ovf = 0;
         }
         __syncthreads();
-        const int* ck = B.ckey + item.k0;
-        const double* wrow = A.w + e0;
 
-        // Inserts queue entries [0, cnt) (one per lane), then moves entries
-        // [cnt, qn) to the front.  Probing gives up after kMaxProbe slots and
-        // marks the partition overflowed (the row then takes the exact HBM
-        // path), so no lane can spin on a full table.  New keys are counted
-        // once per flush; past key_limit (<= S/2) the partition is marked
-        // overflowed, and as a flush adds at most 64 keys per wavefront the
-        // table never holds more than key_limit + 256 < S keys.
-        auto flush = [&](int cnt, int qn) {
+        // Inserts (c, w) when c >= 0.  Probing gives up after kMaxProbe
+        // slots and marks the partition overflowed (the row then takes the
+        // exact HBM path), so no lane can spin on a full table.  New keys are
+        // counted per wavefront step; past key_limit (<= S/2) the partition is
+        // marked overflowed, and as at most 256 uncounted keys exist at any
+        // time the table never holds more than key_limit + 256 < S keys.
+        // Called by all lanes of the workgroup with a uniform trip count.
+        auto insert = [&](int c, double w) {
             bool fresh = false;
-            if (lane < cnt && !s_ovf) {
-                const int c = qk[lane];
-                const fx_t wf = fx_from_double(wrow[qi[lane]]);
+            if (c >= 0) {
                 unsigned h = ((unsigned)c * kHashMul) >> (32 - lg);
                 bool ok = false;
                 for (int probes = 0; probes < kMaxProbe; ++probes) {
@@ -441,51 +533,59 @@ lm_big_kernel(LmArgs A, BigArgs B)
                     }
                     h = (h + 1) & (ts - 1);
                 }
-                if (ok) lds_add(&lo[h], &hi[h], wf);
+                if (ok) lds_add(&lo[h], &hi[h], fx_from_double(w));
                 else s_ovf = 1;
             }
             const unsigned long long nf = __ballot(fresh);
             if (lane == 0 && nf) {
                 if (atomicAdd(&s_nkeys, __popcll(nf)) + __popcll(nf) > B.key_limit) s_ovf = 1;
             }
-            if (lane < qn - cnt) {       // each index is read and rewritten by one lane
-                qk[lane] = qk[cnt + lane];
-                qi[lane] = qi[cnt + lane];
-            }
-            wave_lds_sync();
         };
 
-        int qn = 0;   // queue length, uniform in the wavefront
-        for (int64_t i0 = 0; i0 < d; i0 += (int64_t)kBigUnroll * kBlock) {
-            if (s_ovf) break;
-            int cj[kBigUnroll];
-#pragma unroll
-            for (int j = 0; j < kBigUnroll; ++j) {
-                const int64_t i = i0 + j * kBlock + threadIdx.x;
-                cj[j] = i < d ? ck[i] : -1;
+        if (P == 1) {
+            for (int64_t i0 = 0; i0 < d; i0 += 2 * kBlock) {
+                if (s_ovf) break;
+                const int64_t i1 = i0 + threadIdx.x, i2 = i1 + kBlock;
+                const int u1 = i1 < d ? A.col[e0 + i1] : v;
+                const int u2 = i2 < d ? A.col[e0 + i2] : v;
+                const double w1 = i1 < d ? A.w[e0 + i1] : 0.0;
+                const double w2 = i2 < d ? A.w[e0 + i2] : 0.0;
+                const int c1 = u1 != v ? A.C[u1] : -1;     // self arcs weigh 0 (graph_cuda.cu:2312-2315)
+                const int c2 = u2 != v ? A.C[u2] : -1;
+                insert(c1, w1);
+                insert(c2, w2);
             }
-#pragma unroll
-            for (int j = 0; j < kBigUnroll; ++j) {
-                const bool mine = cj[j] >= 0 && (P == 1 || (int)(((unsigned)cj[j] * kPartMul) >> psh) == p);
-                const unsigned long long mask = __ballot(mine);
-                if (mine) {
-                    const int pos = qn + lane_rank(mask);
-                    qk[pos] = cj[j];
-                    qi[pos] = (int)(i0 + j * kBlock + threadIdx.x);
+        } else {
+            for (int g = 0; g * kBlock < nch; ++g) {
+                if (g > 0) {
+                    __syncthreads();              // every lane is done with the previous group
+                    if (s_ovf) break;             // uniform after the barrier
+                    total = scan_group(g);
                 }
-                qn += __popcll(mask);
-                if (qn >= 64) {          // qn <= 127 < kQueue here
-                    wave_lds_sync();
-                    flush(64, qn);
-                    qn -= 64;
+                const int ng = min(kBlock, nch - g * kBlock);
+                const int64_t cbase = item.k0 + (int64_t)g * kBlock * kKeyChunkArcs;
+                for (int i0 = 0; i0 < total; i0 += kBlock) {
+                    if (s_ovf) break;
+                    const int i = i0 + threadIdx.x;
+                    int c = -1;
+                    double w = 0.0;
+                    if (i < total) {
+                        int l = 0, h = ng;                   // last chunk whose start is <= i
+                        while (h - l > 1) {
+                            const int mid = (l + h) >> 1;
+                            if (s_pre[mid] <= i) l = mid;
+                            else h = mid;
+                        }
+                        const int64_t at = cbase + (int64_t)l * kKeyChunkArcs + s_lo[l] + (i - s_pre[l]);
+                        c = B.bkey[at];
+                        w = B.bw[at];
+                    }
+                    insert(c, w);
                 }
             }
         }
-        wave_lds_sync();
-        if (qn > 0) flush(qn, qn);
         __syncthreads();
         const bool ovf = s_ovf != 0;
-        const double ki = A.kd[v];
         fx_t self = 0;
         double bg = -kMaxFloat;
         int bc = kNone;
@@ -495,7 +595,7 @@ lm_big_kernel(LmArgs A, BigArgs B)
             auto score = [&](int cnt, int qn) {
                 if (lane < cnt) {
                     const int s = qk[lane];
-                    consider(A, ki, own, key[s], fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
+                    consider(A, m, ki, own, key[s], fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
                 }
                 if (lane < qn - cnt) qk[lane] = qk[cnt + lane];
                 wave_lds_sync();
@@ -521,8 +621,8 @@ lm_big_kernel(LmArgs A, BigArgs B)
         const int64_t tb = B.toff[slot];
         const int64_t tsize = B.toff[slot + 1] - tb;
         if (P == 1) {
-            if (ovf) global_row_decide(A, v, B.tkey + tb, B.tval + tb, tsize, R);
-            else if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, self, bg, bc);
+            if (ovf) global_row_decide(A, m, v, B.tkey + tb, B.tval + tb, tsize, R);
+            else if (threadIdx.x == 0) A.Cn[v] = decide(A, m, v, own, self, bg, bc);
             __syncthreads();
             continue;
         }
@@ -549,30 +649,28 @@ lm_big_kernel(LmArgs A, BigArgs B)
             s_last = __hip_atomic_fetch_add(&B.cnt[slot], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == P - 1;
         }
         __syncthreads();
-        if (s_last) {                      // this workgroup finished the row last
-            fx_t s2 = 0;
+        if (s_last) {                      // this workgroup finished the row last: combine
+            fx_t s2 = 0;                   // the P partials, one per thread
             double g2 = -kMaxFloat;
             int c2 = kNone;
-            bool any_ovf = false;
-            if (threadIdx.x == 0) {
-                const int64_t q0 = B.poff[slot];
-                for (int i = 0; i < P; ++i) {
-                    const int ci = ld_i32_agent(B.pc + q0 + i);
-                    any_ovf = any_ovf || ci == kOverflow;
-                    s2 += ld_fx_agent(B.pself + q0 + i);
-                    const double gi = __longlong_as_double(
-                        (long long)ld_u64_agent(reinterpret_cast<const unsigned long long*>(B.pg + q0 + i)));
-                    if (better(gi, ci, g2, c2)) {
-                        g2 = gi;
-                        c2 = ci;
-                    }
+            int ovf2 = 0;
+            const int64_t q0 = B.poff[slot];
+            for (int i = threadIdx.x; i < P; i += kBlock) {
+                const int ci = ld_i32_agent(B.pc + q0 + i);
+                s2 += ld_fx_agent(B.pself + q0 + i);
+                const double gi = __longlong_as_double(
+                    (long long)ld_u64_agent(reinterpret_cast<const unsigned long long*>(B.pg + q0 + i)));
+                if (ci == kOverflow) ovf2 = 1;
+                else if (better(gi, ci, g2, c2)) {
+                    g2 = gi;
+                    c2 = ci;
                 }
-                s_nkeys = any_ovf;
-                B.cnt[slot] = 0;           // leave the counter clean for the next batch
             }
-            __syncthreads();
-            if (s_nkeys) global_row_decide(A, v, B.tkey + tb, B.tval + tb, tsize, R);
-            else if (threadIdx.x == 0) A.Cn[v] = decide(A, v, own, s2, g2, c2);
+            const int any_ovf = __syncthreads_or(ovf2);
+            block_reduce(R, s2, g2, c2);
+            if (threadIdx.x == 0) B.cnt[slot] = 0;   // leave the counter clean for the next batch
+            if (any_ovf) global_row_decide(A, m, v, B.tkey + tb, B.tval + tb, tsize, R);
+            else if (threadIdx.x == 0) A.Cn[v] = decide(A, m, v, own, s2, g2, c2);
         }
         __syncthreads();
     }
@@ -581,12 +679,16 @@ lm_big_kernel(LmArgs A, BigArgs B)
 // update_community_weights + update_community_ids of one batch, fused
 // (graph_cuda.cu:1234-1249 then 1191-1196): each thread touches only its own
 // vertex's C/C' entries, so the swap needs no second launch.  C32 mirrors C.
+// The vertices are list[0, n) or, with a null list, [v0, v0 + n); with gid
+// set, only vertices of graphs whose flag is set take part.
 __global__ void __launch_bounds__(kBlock)
-commit_kernel(int64_t v0, int64_t v1, int64_t* __restrict__ C, int64_t* __restrict__ Cn,
-              int* __restrict__ C32, const fx_t* __restrict__ k, fx_t* __restrict__ a)
+commit_kernel(const int* __restrict__ list, int64_t v0, int64_t n, int64_t* __restrict__ C,
+              int64_t* __restrict__ Cn, int* __restrict__ C32, const fx_t* __restrict__ k,
+              fx_t* __restrict__ a, const int* __restrict__ gid, const unsigned char* __restrict__ flag)
 {
-    for (int64_t v = v0 + blockIdx.x * (int64_t)kBlock + threadIdx.x; v < v1;
-         v += (int64_t)gridDim.x * kBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t v = list ? (int64_t)list[i] : v0 + i;
+        if (gid && !flag[gid[v]]) continue;
         const int64_t src = C[v];
         const int64_t dst = Cn[v];
         if (src != dst) {
@@ -619,20 +721,22 @@ int small_bins_blocks(SmallBins& L, unsigned cap_per_bin)
     return total;
 }
 
-void launch_lm_small(const LmArgs& A, const SmallBins& L, bool split, hipStream_t s)
+void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s)
 {
-    const int blocks = L.blk0[NUM_SMALL_BINS + 1];
-    if (blocks <= 0) return;
-    if (!split) {
-        lm_small_kernel<<<blocks, kBlock, 0, s>>>(A, L);
+    if (bins && fill && !split) {
+        const int blocks = L.blk0[NUM_SMALL_BINS + 1];
+        if (blocks > 0) lm_small_kernel<<<blocks, kBlock, 0, s>>>(A, L);
         return;
     }
-    for (int b = 0; b <= NUM_SMALL_BINS; ++b) {   // bin b alone (b == NUM_SMALL_BINS: key fill)
+    // b < NUM_SMALL_BINS: bin b; b == NUM_SMALL_BINS: the key fill
+    const int first = bins ? 0 : NUM_SMALL_BINS, last = fill ? NUM_SMALL_BINS : NUM_SMALL_BINS - 1;
+    for (int b = first; b <= last; ++b) {
         SmallBins one = L;
-        for (int k = 0; k < NUM_SMALL_BINS; ++k) one.n[k] = k == b ? L.n[k] : 0;
+        for (int k = 0; k < NUM_SMALL_BINS; ++k) one.n[k] = (split ? k == b : bins) ? L.n[k] : 0;
         if (b != NUM_SMALL_BINS) one.nchunks = 0;
         if (small_bins_blocks(one, 4096) > 0)
             lm_small_kernel<<<one.blk0[NUM_SMALL_BINS + 1], kBlock, 0, s>>>(A, one);
+        if (!split && b < NUM_SMALL_BINS) b = NUM_SMALL_BINS - 1;   // all bins went in one launch
     }
 }
 
@@ -653,7 +757,15 @@ void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, co
                    hipStream_t s)
 {
     if (v1 <= v0) return;
-    commit_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(v0, v1, C, Cn, C32, k, a);
+    commit_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(nullptr, v0, v1 - v0, C, Cn, C32, k, a,
+                                                               nullptr, nullptr);
+}
+
+void launch_commit_filtered(const int* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
+                            const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s)
+{
+    if (n <= 0) return;
+    commit_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(list, v0, n, C, Cn, C32, k, a, gid, flag);
 }
 
 }  // namespace nido

@@ -125,6 +125,8 @@ KNOBS = [
     {"big_min_deg": 0, "big_part_arcs": 1},                         # every row partitioned
     {"big_min_deg": 32, "big_part_arcs": 64, "big_key_limit": 2},   # forces the HBM fallback
     {"split_small": 1},                                             # one launch per short-row bin
+    {"overlap_long": 0},                                            # long rows on the main stream
+    {"overlap_long": 0, "split_small": 1},
 ]
 
 
