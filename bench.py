@@ -21,6 +21,14 @@ HIP-event durations on the library's stream, against the 8 TB/s HBM peak;
 roofline_iter applies 48 B/arc + 64 B/vertex to the whole iteration.
 cpu_baseline: the oracle (C + OpenMP) timed on phase-0 iterations of the same
 graph on this host, bounded to ~20 s.
+
+--workload batched: 4,096 independent planted-partition graphs (vertex counts
+log-uniform in [256, 65536], blocks of 64, intra-degree ~12, inter-degree ~2,
+weights U(0,1], seeds 1000+i) run to completion in one handle
+(nido_batch_run); value = graphs/s end to end (upload + all phases), beside
+the plain path (one GraphGPU + LouvainGPU::run per graph) timed on a
+systematic sample of the same graphs, whose modularity and clusterings must
+equal the batched ones.
 """
 from __future__ import annotations
 
@@ -71,9 +79,123 @@ def cpu_baseline(off, col, w, nb, tol, max_iters, budget_s, gpu_first_q):
     }
 
 
+def sbm_batch(na, count, seed):
+    """The batched workload's graphs as one CSR (vptr, off, col, w)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(np.log(256), np.log(65536), count))).astype(np.int64)
+    parts = []
+    for i, n in enumerate(ns):
+        n = int(n)
+        parts.append(na.gen_sbm(n, 64, 12.0 / 63.0, 2.0 / (n - 64), True, 1000 + i))
+    nvs = [p.num_vertices for p in parts]
+    nes = [p.num_edges for p in parts]
+    vptr = np.zeros(count + 1, np.int64)
+    vptr[1:] = np.cumsum(nvs)
+    off = np.zeros(int(vptr[-1]) + 1, np.int64)
+    col = np.empty(int(sum(nes)), np.int64)
+    w = np.empty(int(sum(nes)), np.float64)
+    a = 0
+    for i, p in enumerate(parts):
+        o, c, x = p.arrays(copy=False)
+        v0, v1 = int(vptr[i]), int(vptr[i + 1])
+        off[v0:v1] = o[:-1] + a
+        col[a:a + len(c)] = c + v0
+        w[a:a + len(c)] = x
+        a += len(c)
+        parts[i] = None
+    off[-1] = a
+    return vptr, off, col, w
+
+
+def bench_batched(args):
+    import numpy as np
+    import nido_amd as na
+    t0 = time.perf_counter()
+    vptr, off, col, w = sbm_batch(na, args.graphs, args.size_seed)
+    G = len(vptr) - 1
+    log(f"batched workload: {G} graphs, {int(vptr[-1])} vertices, {len(col)} arcs "
+        f"({time.perf_counter() - t0:.1f}s to generate)")
+    nb, tol = args.batches, args.tol
+
+    def sub(i):
+        v0, v1 = int(vptr[i]), int(vptr[i + 1])
+        o = off[v0:v1 + 1] - off[v0]
+        return o, col[off[v0]:off[v1]] - v0, w[off[v0]:off[v1]]
+
+    # warm-up: a small batch through both paths (first-launch costs)
+    wb = na.BatchGPU([sub(i) for i in range(min(G, 8))], nb)
+    wb.run(500, tol)
+    wb.close()
+
+    t0 = time.perf_counter()
+    bg = na.BatchGPU(csr=(vptr, off, col, w), nbatches=nb)
+    t1 = time.perf_counter()
+    st = bg.run(500, tol)
+    t_batch = time.perf_counter() - t0
+    log(f"batched: {t_batch:.2f}s ({t1 - t0:.2f}s create), {st.phases} phases, {st.total_loops} shared iterations")
+
+    sample = list(range(0, G, max(1, G // args.plain_sample)))[:args.plain_sample]
+    t_plain, q_ok, c_ok = 0.0, True, True
+    for i in sample:
+        o, c, x = sub(i)
+        ts = time.perf_counter()
+        g = na.GraphGPU((o, c, x), nb)
+        pst = na.LouvainGPU(500, tol, nb).run(g)
+        clus = g.clustering()
+        t_plain += time.perf_counter() - ts
+        bst = bg.graph_stats(i)
+        q_ok = q_ok and abs(bst.q - pst.q) <= 1e-12 and bst.phases == pst.phases
+        c_ok = c_ok and bool(np.array_equal(bg.clustering(i), clus))
+        g.close()
+    plain_gps = len(sample) / t_plain
+    batch_gps = G / t_batch
+    log(f"plain: {len(sample)} graphs in {t_plain:.2f}s -> {plain_gps:.2f} graphs/s")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        tc, nc = 0.0, 0
+        for i in sample:
+            o, c, x = sub(i)
+            ts = time.perf_counter()
+            orc = oracle.Oracle(o, c, x, nb, threads)
+            orc.run(500, tol)
+            orc.close()
+            tc += time.perf_counter() - ts
+            nc += 1
+            if tc > args.cpu_budget:
+                break
+        cpu = {"value": nc / tc, "unit": "graphs/s", "cores": threads, "kind": "port",
+               "sample": f"{nc} graphs of the systematic sample run to completion one after another, "
+                         f"{tc:.2f} s, C/OpenMP oracle (vertex-level OpenMP inside each graph)"}
+    bg.close()
+    out = {
+        "metric": METRIC, "value": round(batch_gps, 3), "unit": "graphs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(t_batch * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{G} planted-partition graphs, n log-uniform [256, 65536], blocks of 64, "
+                               f"intra-degree ~12, inter-degree ~2, U(0,1] weights, seeds 1000+i, -b {nb} "
+                               f"-t {tol:g}, full runs (all phases)",
+                   "graphs": G, "vertices": int(vptr[-1]), "arcs": int(len(col)),
+                   "phases": st.phases, "shared_iterations": st.total_loops},
+        "plain": {"graphs_per_s": round(plain_gps, 3), "sample": len(sample),
+                  "sample_rule": f"every {max(1, G // args.plain_sample)}th graph"},
+        "speedup_vs_plain": round(batch_gps / plain_gps, 2),
+        "identical_modularity_on_sample": q_ok, "identical_clustering_on_sample": c_ok,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["rmat", "batched"], default="rmat")
+    ap.add_argument("--graphs", type=int, default=4096, help="batched workload: number of graphs")
+    ap.add_argument("--plain-sample", type=int, default=64, help="batched workload: graphs timed on the plain path")
+    ap.add_argument("--size-seed", type=int, default=4096, help="batched workload: seed of the vertex counts")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=22)
@@ -104,6 +226,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 5}))
+        return
+
+    if args.workload == "batched":
+        bench_batched(args)
         return
 
     import nido_amd as na

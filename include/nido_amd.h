@@ -221,6 +221,29 @@ int nido_run_iteration_q(const nido_graph* g, int phase, double* q);
 /* Communities at the end of a phase, phase-local ids (record=1 runs). */
 int nido_run_phase_communities(const nido_graph* g, int phase, int64_t* comm);
 
+/* ---- many independent graphs in one handle ------------------------------
+ * No counterpart in the reference, which runs one graph per process; this
+ * runs LouvainGPU::run (louvain_gpu.cpp:6-139) on every graph of a batch at
+ * once, sharing launches, and every graph's result is the one it gets alone
+ * (same phases, iterations, modularity and communities).
+ *
+ * The graphs come as one CSR: graph i owns vertices [vptr[i], vptr[i+1]) and
+ * the arcs of their rows; column ids are handle-wide (graph i's arcs stay in
+ * graph i's vertex range).  Each graph is cut into nbatches batches by its
+ * own rule (graph_gpu.cpp:303-339). */
+int nido_batch_create(int ngraphs, const int64_t* vptr, const int64_t* offsets, const int64_t* cols,
+                      const double* weights, int nbatches, void* stream, nido_graph** out);
+/* Runs every graph to completion; stats receives the handle-wide totals
+ * (phases = longest run, total_loops = shared iterations). */
+int nido_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, nido_run_stats* stats);
+int nido_batch_size(const nido_graph* g, int* ngraphs);
+/* One graph's run: q, phases, total_loops as LouvainGPU::run reports them. */
+int nido_batch_graph_stats(const nido_graph* g, int graph, nido_run_stats* out);
+int nido_batch_graph_phase(const nido_graph* g, int graph, int phase, nido_phase_stats* out);
+/* Clustering::commIds_ of one graph (clustering.cpp:12-21): original vertex
+ * (graph-local) -> final community (graph-local, dense from 0). */
+int nido_batch_clustering(const nido_graph* g, int graph, int64_t* comm);
+
 #ifdef __cplusplus
 }
 #endif

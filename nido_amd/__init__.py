@@ -122,6 +122,13 @@ KERNEL_CLASSES = ("lm_short", "lm_long", "commit", "modularity")   # NIDO_K_* or
 _sig("nido_run_phase", C.c_int, C.c_void_p, C.c_int, C.POINTER(PhaseStats))
 _sig("nido_run_iteration_q", C.c_int, C.c_void_p, C.c_int, _pdbl)
 _sig("nido_run_phase_communities", C.c_int, C.c_void_p, C.c_int, _pi64)
+_sig("nido_batch_create", C.c_int, C.c_int, _pi64, _pi64, _pi64, _pdbl, C.c_int, C.c_void_p,
+     C.POINTER(C.c_void_p))
+_sig("nido_batch_run", C.c_int, C.c_void_p, _i64, C.c_double, C.c_double, C.POINTER(RunStats))
+_sig("nido_batch_size", C.c_int, C.c_void_p, C.POINTER(C.c_int))
+_sig("nido_batch_graph_stats", C.c_int, C.c_void_p, C.c_int, C.POINTER(RunStats))
+_sig("nido_batch_graph_phase", C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(PhaseStats))
+_sig("nido_batch_clustering", C.c_int, C.c_void_p, C.c_int, _pi64)
 
 
 class NidoError(RuntimeError):
@@ -442,3 +449,79 @@ class LouvainGPU:
         _check(lib.nido_louvain_run(graph._h, self.max_loops, self.tol, self.tol_phase, int(record),
                                     int(verbose), C.byref(st)), "nido_louvain_run")
         return st
+
+
+def concat_graphs(graphs):
+    """One CSR for several graphs: (vptr, offsets, cols, weights) with
+    handle-wide column ids, graph i holding vertices [vptr[i], vptr[i+1])."""
+    vptr = [0]
+    offs, cols, ws = [], [], []
+    abase = 0
+    for g in graphs:
+        if isinstance(g, Graph):
+            off, col, w = g.arrays(copy=False)
+        else:
+            off, col, w = g
+        off = np.asarray(off, dtype=np.int64)
+        n = len(off) - 1
+        offs.append(off[:-1] - off[0] + abase)
+        cols.append(np.asarray(col, dtype=np.int64)[off[0]:off[-1]] + vptr[-1])
+        ws.append(np.asarray(w, dtype=np.float64)[off[0]:off[-1]])
+        abase += int(off[-1] - off[0])
+        vptr.append(vptr[-1] + n)
+    offs.append(np.array([abase], dtype=np.int64))
+    return (np.asarray(vptr, dtype=np.int64), np.concatenate(offs),
+            np.concatenate(cols) if cols else np.zeros(0, np.int64),
+            np.concatenate(ws) if ws else np.zeros(0))
+
+
+class BatchGPU:
+    """Many independent graphs in one device handle; run() executes
+    LouvainGPU::run on all of them at once (nido_batch_run).  Every graph's
+    result equals its result alone."""
+
+    def __init__(self, graphs=None, nbatches: int = DEFAULT_BATCHES, csr=None, stream: int | None = None):
+        vptr, off, col, w = csr if csr is not None else concat_graphs(graphs)
+        vptr, off = np.ascontiguousarray(vptr, np.int64), np.ascontiguousarray(off, np.int64)
+        col, w = np.ascontiguousarray(col, np.int64), np.ascontiguousarray(w, np.float64)
+        self._keep = (vptr, off, col, w)
+        self.vptr = vptr
+        self.ngraphs = len(vptr) - 1
+        h = C.c_void_p()
+        _check(lib.nido_batch_create(self.ngraphs, _p64(vptr), _p64(off), _p64(col), _pd(w), nbatches,
+                                     C.c_void_p(stream) if stream else None, C.byref(h)), "nido_batch_create")
+        self._h = h
+        self.nbatches = nbatches
+
+    def run(self, max_loops: int = DEFAULT_ITERATIONS, tol: float = DEFAULT_THRESHOLD,
+            tol_phase: float | None = None) -> RunStats:
+        st = RunStats()
+        _check(lib.nido_batch_run(self._h, max_loops, tol, tol if tol_phase is None else tol_phase,
+                                  C.byref(st)), "nido_batch_run")
+        return st
+
+    def graph_stats(self, i: int) -> RunStats:
+        st = RunStats()
+        _check(lib.nido_batch_graph_stats(self._h, i, C.byref(st)), "nido_batch_graph_stats")
+        return st
+
+    def graph_phase(self, i: int, phase: int) -> PhaseStats:
+        ps = PhaseStats()
+        _check(lib.nido_batch_graph_phase(self._h, i, phase, C.byref(ps)), "nido_batch_graph_phase")
+        return ps
+
+    def clustering(self, i: int) -> np.ndarray:
+        out = np.zeros(int(self.vptr[i + 1] - self.vptr[i]), dtype=np.int64)
+        _check(lib.nido_batch_clustering(self._h, i, _p64(out)), "nido_batch_clustering")
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.nido_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

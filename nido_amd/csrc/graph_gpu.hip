@@ -64,6 +64,16 @@ struct PhaseHist {
     std::vector<int64_t> comm;
 };
 
+// One graph's outcome in a batched run.
+struct BatchResult {
+    nido_run_stats st{};
+    std::vector<nido_phase_stats> phases;
+    std::vector<int64_t> clustering;   // original vertex -> final community, graph-local ids
+};
+
+constexpr int64_t kSegArcs = 65536;    // arcs per per-graph modularity item
+constexpr int64_t kSegVerts = 16384;   // vertices per per-graph modularity item
+
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -225,6 +235,28 @@ struct nido_graph {
 
     std::vector<PhaseHist> hist;
     nido_run_stats last{};
+
+    // ---- several graphs in one handle (a batched run); G == 1 otherwise ----
+    int G = 1;
+    std::vector<int64_t> vb;             // G+1 vertex bases of the current phase
+    std::vector<int64_t> vb0;            // G+1 original vertex bases
+    std::vector<int64_t> gparts;         // G * (nb+1) batch bounds (handle vertex ids)
+    std::vector<fx_t> twom_g;            // exact 2m per graph
+    std::vector<double> twom_dg;
+    int* d_gid = nullptr;                // graph of every current vertex (capacity n0)
+    int* d_gid0 = nullptr;               // graph of every original vertex
+    int64_t* d_vb = nullptr;             // G+1
+    int64_t* d_vbn = nullptr;            // G+1 (bases after renumbering)
+    double* d_mg = nullptr;              // mass per graph
+    double* d_twomg = nullptr;           // 2m per graph, FP64
+    unsigned char* d_act = nullptr;      // per graph: moves in this iteration
+    unsigned char* d_keep = nullptr;     // per graph: selected for a restore / a coarsening
+    fx_t* d_gacc = nullptr;              // 2 accumulators per graph
+    double* d_gq = nullptr;              // modularity per graph
+    SegItem* d_seg = nullptr;
+    int64_t nseg = 0;
+    std::vector<int64_t> blist0, blistn; // per batch: its slice of d_binv (non-isolated vertices)
+    std::vector<BatchResult> bres;       // per graph, from the last batched run
 };
 
 namespace {
@@ -253,7 +285,7 @@ void default_tuning(nido_tuning& t)
 
 // partition_graph_edge_batch with part_on_batch=1 on one device,
 // graph_gpu.cpp:303-339.
-void batch_bounds(const std::vector<int64_t>& off, int64_t nv, int nb, std::vector<int64_t>& parts)
+void batch_bounds(const int64_t* off, int64_t nv, int nb, std::vector<int64_t>& parts)
 {
     const int64_t ne = off[nv] - off[0];
     const int64_t ave = ne / nb;
@@ -278,6 +310,11 @@ void batch_bounds(const std::vector<int64_t>& off, int64_t nv, int nb, std::vect
         while ((int64_t)vp.size() < nb + 1) vp.push_back(nv);
     }
     parts.assign(vp.begin(), vp.begin() + nb + 1);
+}
+
+void batch_bounds(const std::vector<int64_t>& off, int64_t nv, int nb, std::vector<int64_t>& parts)
+{
+    batch_bounds(off.data(), nv, nb, parts);
 }
 
 int64_t pow2_at_least(int64_t x)
@@ -305,14 +342,37 @@ int setup_phase(nido_graph* g)
     const auto& off = g->h_off;
     g->maxdeg = 0;
     for (int64_t v = 0; v < nv; ++v) g->maxdeg = std::max(g->maxdeg, off[v + 1] - off[v]);
-    batch_bounds(off, nv, g->nb, g->parts);
     g->committed.assign((size_t)g->nb, 0);
+    // Batch b of the handle = batch b of every graph it holds (each graph
+    // cut by its own rule, graph_gpu.cpp:303-339).
+    const int G = g->G;
+    const size_t nb1 = (size_t)g->nb + 1;
+    if (G == 1) {
+        batch_bounds(off, nv, g->nb, g->parts);
+    } else {
+        g->gparts.assign((size_t)G * nb1, 0);
+        std::vector<int64_t> p;
+        for (int gg = 0; gg < G; ++gg) {
+            const int64_t v0 = g->vb[(size_t)gg], n = g->vb[(size_t)gg + 1] - v0;
+            batch_bounds(off.data() + v0, n, g->nb, p);
+            for (size_t i = 0; i < nb1; ++i) g->gparts[(size_t)gg * nb1 + i] = v0 + p[i];
+        }
+    }
+    auto for_batch = [&](int b, auto&& fn) {
+        if (G == 1) {
+            fn(g->parts[(size_t)b], g->parts[(size_t)b + 1]);
+            return;
+        }
+        for (int gg = 0; gg < G; ++gg) fn(g->gparts[(size_t)gg * nb1 + b], g->gparts[(size_t)gg * nb1 + b + 1]);
+    };
 
     // Short rows: vertices grouped by (batch, bin), ascending id inside a
-    // group; isolated vertices are left out (they never move, and C' = C
-    // holds for them throughout).  Long rows: per batch, P =
-    // 2^ceil(log2(d / part_arcs)) partition items, their partial slots, HBM
-    // fallback tables, and for P > 1 the key-fill chunks and bucket space.
+    // group, then the batch's long rows; isolated vertices are left out
+    // (they never move, and C' = C holds for them throughout), so a batch's
+    // slice of binv lists every vertex its commit has to visit.  Long rows:
+    // per batch, P = 2^ceil(log2(d / part_arcs)) partition items, their
+    // partial slots, HBM fallback tables, and for P > 1 the key-fill chunks
+    // and bucket space.
     std::vector<int> binv((size_t)std::max<int64_t>(nv, 1));
     std::vector<BigItem> bitems;
     std::vector<KeyChunk> chunks;
@@ -323,36 +383,43 @@ int setup_phase(nido_graph* g)
     int64_t pos = 0, tab_need = 0, part_need = 0, slot_need = 0, key_need = 0, cpo_need = 0;
     g->short_bytes.assign((size_t)g->nb, 0.0);
     g->long_bytes.assign((size_t)g->nb, 0.0);
+    g->blist0.assign((size_t)g->nb, 0);
+    g->blistn.assign((size_t)g->nb, 0);
     for (int b = 0; b < g->nb; ++b) {
         chunk_base[(size_t)b] = (int64_t)chunks.size();
         int64_t ksum = 0, csum = 0;
         int64_t cnt[NUM_BINS] = {0};
-        for (int64_t v = g->parts[b]; v < g->parts[b + 1]; ++v)
-            if (off[v + 1] > off[v]) cnt[bin_of(off[v + 1] - off[v], g->tuning)]++;
-        int64_t cur[NUM_SMALL_BINS];
-        for (int k = 0; k < NUM_SMALL_BINS; ++k) {
-            small_start[(size_t)b * NUM_SMALL_BINS + k] = pos;
-            small_n[(size_t)b * NUM_SMALL_BINS + k] = cnt[k];
+        for_batch(b, [&](int64_t v0, int64_t v1) {
+            for (int64_t v = v0; v < v1; ++v)
+                if (off[v + 1] > off[v]) cnt[bin_of(off[v + 1] - off[v], g->tuning)]++;
+        });
+        int64_t cur[NUM_BINS];
+        g->blist0[(size_t)b] = pos;
+        for (int k = 0; k < NUM_BINS; ++k) {
+            if (k < NUM_SMALL_BINS) {
+                small_start[(size_t)b * NUM_SMALL_BINS + k] = pos;
+                small_n[(size_t)b * NUM_SMALL_BINS + k] = cnt[k];
+            }
             cur[k] = pos;
             pos += cnt[k];
         }
+        g->blistn[(size_t)b] = pos - g->blist0[(size_t)b];
         item_base[(size_t)b] = (int64_t)bitems.size();
         poff_base[(size_t)b] = (int64_t)poff.size();
         toff_base[(size_t)b] = (int64_t)toff.size();
         int64_t psum = 0, tsum = 0;
         int slot = 0;
         toff.push_back(0);
-        for (int64_t v = g->parts[b]; v < g->parts[b + 1]; ++v) {
+        for_batch(b, [&](int64_t v0, int64_t v1) {
+          for (int64_t v = v0; v < v1; ++v) {
             const int64_t d = off[v + 1] - off[v];
             const int k = bin_of(d, g->tuning);
             // algorithmic bytes of the sweep: 24 per arc (col, weight,
             // C[col]) + 40 per vertex (offset, C, k, a_C, C')
             (k != BIN_BIG ? g->short_bytes : g->long_bytes)[(size_t)b] += 24.0 * (double)d + 40.0;
             if (d == 0) continue;
-            if (k != BIN_BIG) {
-                binv[(size_t)cur[k]++] = (int)v;
-                continue;
-            }
+            binv[(size_t)cur[k]++] = (int)v;
+            if (k != BIN_BIG) continue;
             int P = 1;
             while (P < kMaxParts && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
             for (int p = 0; p < P; ++p) bitems.push_back(BigItem{(int)v, p, P, slot, ksum, csum});
@@ -369,7 +436,8 @@ int setup_phase(nido_graph* g)
             tsum += pow2_at_least(2 * d);
             toff.push_back(tsum);
             ++slot;
-        }
+          }
+        });
         // longest rows first: their partition workgroups are the long poles
         std::stable_sort(bitems.begin() + item_base[(size_t)b], bitems.end(),
                          [&](const BigItem& x, const BigItem& y) {
@@ -492,12 +560,35 @@ int setup_phase(nido_graph* g)
     const double maxk = nido_bitsd(maxk_bits);
     const int narrow = g->nonneg && maxk < 2147483648.0 - 256.0;
     for (auto& B : g->big_args) B.narrow = narrow;
+    if (G > 1) {
+        // per-graph modularity items: arc and vertex chunks of each graph
+        std::vector<SegItem> seg;
+        for (int gg = 0; gg < G; ++gg) {
+            const int64_t v0 = g->vb[(size_t)gg], v1 = g->vb[(size_t)gg + 1];
+            const int64_t e0 = off[v0], e1 = off[v1];
+            const int64_t n = std::max((e1 - e0 + kSegArcs - 1) / kSegArcs, (v1 - v0 + kSegVerts - 1) / kSegVerts);
+            for (int64_t i = 0; i < n; ++i)
+                seg.push_back(SegItem{gg, std::min(e1, e0 + i * kSegArcs), std::min(e1, e0 + (i + 1) * kSegArcs),
+                                      std::min(v1, v0 + i * kSegVerts), std::min(v1, v0 + (i + 1) * kSegVerts)});
+        }
+        g->nseg = (int64_t)seg.size();
+        NIDO_TRY(upload(&g->d_seg, seg, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    return NIDO_OK;
+}
+
+// The GraphGPU method set works on handles that hold one graph.
+int single_graph(const nido_graph* g)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (g->G != 1) return set_error(NIDO_ERR_STATE, "this handle holds %d graphs: use nido_batch_run", g->G);
     return NIDO_OK;
 }
 
 int check_batch(const nido_graph* g, int batch)
 {
-    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    NIDO_TRY(single_graph(g));
     if (batch < 0 || batch >= g->nb) return set_error(NIDO_ERR_INVALID, "batch %d out of range", batch);
     return NIDO_OK;
 }
@@ -539,15 +630,23 @@ int do_modularity(nido_graph* g, double* q)
 
 int do_lm(nido_graph* g, int b)
 {
-    const int64_t v0 = g->parts[(size_t)b], v1 = g->parts[(size_t)b + 1];
     g->committed[(size_t)b] = 0;
-    if (v1 <= v0) return NIDO_OK;
-    if (g->twom == 0) {
-        HIP_TRY(hipMemcpyAsync(g->d_Cn + v0, g->d_C + v0, sizeof(int64_t) * (size_t)(v1 - v0),
-                               hipMemcpyDeviceToDevice, g->stream));
-        return NIDO_OK;
-    }
     LmArgs A{g->d_off, g->d_col32, g->d_w, g->d_C32, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m};
+    if (g->G == 1) {
+        const int64_t v0 = g->parts[(size_t)b], v1 = g->parts[(size_t)b + 1];
+        if (v1 <= v0) return NIDO_OK;
+        if (g->twom == 0) {
+            HIP_TRY(hipMemcpyAsync(g->d_Cn + v0, g->d_C + v0, sizeof(int64_t) * (size_t)(v1 - v0),
+                                   hipMemcpyDeviceToDevice, g->stream));
+            return NIDO_OK;
+        }
+    } else {
+        // graphs without weight never take part (their act flag stays 0)
+        if (g->blistn[(size_t)b] == 0) return NIDO_OK;
+        A.gid = g->d_gid;
+        A.mg = g->d_mg;
+        A.act = g->d_act;
+    }
     const SmallBins& L = g->small_bins[(size_t)b];
     const BigArgs& B = g->big_args[(size_t)b];
     const bool split = g->tuning.split_small != 0;
@@ -587,8 +686,12 @@ int do_commit(nido_graph* g, int b)
 {
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
-        launch_commit(g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_C, g->d_Cn, g->d_C32, g->d_k,
-                      g->d_a, g->stream);
+        if (g->G == 1)
+            launch_commit(g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_C, g->d_Cn, g->d_C32, g->d_k,
+                          g->d_a, g->stream);
+        else
+            launch_commit_filtered(g->d_binv + g->blist0[(size_t)b], 0, g->blistn[(size_t)b], g->d_C, g->d_Cn,
+                                   g->d_C32, g->d_k, g->d_a, g->d_gid, g->d_act, g->stream);
     }
     HIP_TRY(hipGetLastError());
     g->committed[(size_t)b] = 1;
@@ -619,26 +722,13 @@ int do_iteration(nido_graph* g, double* q, double* dq, double* qt_out = nullptr)
     return NIDO_OK;
 }
 
-// aggregation, graph_gpu.cpp:1382-1545, on the device.
-int do_aggregation(nido_graph* g, int* done)
+// The coarse graph of communities (graph_gpu.cpp:1396-1516): coarse vertex
+// newid[c] for every community c, arcs merged by target with exact sums.
+// With gid set, only the graphs g with keep[g] set are carried over.
+int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, const unsigned char* keep)
 {
     hipStream_t s = g->stream;
     const int64_t nv = g->nv;
-    int64_t* newid = g->d_work;
-    launch_mark_communities(g->d_C, nv, newid, s);
-    launch_exclusive_scan(newid, newid, nv, g->d_tot, g->d_scan_tmp, s);
-    HIP_TRY(hipGetLastError());
-    int64_t K = 0;
-    HIP_TRY(hipMemcpyAsync(&K, g->d_tot, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    launch_update_clustering(g->d_clus, g->n0, g->d_C, newid, s);   // graph_gpu.cpp:1389
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s));
-    if (K == nv || K == 1) {                                          // graph_gpu.cpp:1391-1392
-        *done = 1;
-        return NIDO_OK;
-    }
-    *done = 0;
-
     int64_t *arcs = nullptr, *toff = nullptr, *cnt = nullptr, *noff = nullptr, *ncol = nullptr;
     double* nw = nullptr;
     int* tkey = nullptr;
@@ -660,7 +750,7 @@ int do_aggregation(nido_graph* g, int* done)
             break;
         }
         launch_fill_i64(arcs, 0, K + 1, s);
-        launch_member_counts(g->d_C, newid, g->d_off, nv, nullptr, arcs, s);
+        launch_member_counts(g->d_C, newid, g->d_off, nv, nullptr, arcs, s, gid, keep);
         launch_table_sizes(arcs, K, arcs, s);                      // in place: arcs -> table size
         launch_exclusive_scan(arcs, toff, K, toff + K, tmp, s);
         int64_t T = 0;
@@ -669,7 +759,7 @@ int do_aggregation(nido_graph* g, int* done)
         if ((rc = dev_alloc(&tkey, T)) || (rc = dev_alloc(&tval, T))) break;
         launch_fill_i32(tkey, -1, T, s);
         launch_fill_fx(tval, T, s);
-        launch_coarse_insert(g->d_off, g->d_col, g->d_w, g->d_C, newid, nv, toff, tkey, tval, s);
+        launch_coarse_insert(g->d_off, g->d_col, g->d_w, g->d_C, newid, nv, toff, tkey, tval, s, gid, keep);
         launch_coarse_count(toff, tkey, K, cnt, s);
         launch_exclusive_scan(cnt, noff, K, noff + K, tmp, s);
         HIP_TRY(hipGetLastError());
@@ -690,7 +780,29 @@ int do_aggregation(nido_graph* g, int* done)
         g->h_off.swap(hoff);
     } while (0);
     cleanup();
-    if (rc) return rc;
+    return rc;
+}
+
+// aggregation, graph_gpu.cpp:1382-1545, on the device.
+int do_aggregation(nido_graph* g, int* done)
+{
+    hipStream_t s = g->stream;
+    const int64_t nv = g->nv;
+    int64_t* newid = g->d_work;
+    launch_mark_communities(g->d_C, nv, newid, s);
+    launch_exclusive_scan(newid, newid, nv, g->d_tot, g->d_scan_tmp, s);
+    HIP_TRY(hipGetLastError());
+    int64_t K = 0;
+    HIP_TRY(hipMemcpyAsync(&K, g->d_tot, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    launch_update_clustering(g->d_clus, g->n0, g->d_C, newid, s);   // graph_gpu.cpp:1389
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    if (K == nv || K == 1) {                                          // graph_gpu.cpp:1391-1392
+        *done = 1;
+        return NIDO_OK;
+    }
+    *done = 0;
+    NIDO_TRY(coarsen(g, newid, K, nullptr, nullptr));
     return setup_phase(g);
 }
 
@@ -774,8 +886,215 @@ int do_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, int r
     return NIDO_OK;
 }
 
+// ---- many graphs in one handle ---------------------------------------------
+// Every graph follows exactly the steps LouvainGPU::run would take on it
+// alone (louvain_gpu.cpp:6-139); the graphs only share launches.  A graph
+// whose iteration loop has ended sits out (its act flag is 0) until the
+// phase ends for all; a graph whose run has ended keeps no vertices in the
+// next phase's coarse graph.
+
+int batch_modularity(nido_graph* g, std::vector<double>& q)
+{
+    {
+        ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
+        launch_modularity_seg(g->d_seg, g->nseg, g->d_src, g->d_col32, g->d_w, g->d_C32, g->d_a, g->d_twomg,
+                              g->G, g->d_gacc, g->d_gq, g->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    q.resize((size_t)g->G);
+    HIP_TRY(hipMemcpyAsync(q.data(), g->d_gq, sizeof(double) * (size_t)g->G, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->prof.harvest();
+    return NIDO_OK;
+}
+
+int upload_flags(nido_graph* g, unsigned char* d, const std::vector<unsigned char>& h)
+{
+    HIP_TRY(hipMemcpyAsync(d, h.data(), h.size(), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));   // h may change right after
+    return NIDO_OK;
+}
+
+// Renumbers the communities of the graphs with keep[g] set (ascending, per
+// graph contiguous because graphs hold contiguous vertex ranges); returns
+// the new vertex bases in nvb (G+1).
+int batch_renumber(nido_graph* g, const std::vector<unsigned char>& keep, std::vector<int64_t>& nvb)
+{
+    hipStream_t s = g->stream;
+    int64_t* newid = g->d_work;
+    NIDO_TRY(upload_flags(g, g->d_keep, keep));
+    launch_mark_communities(g->d_C, g->nv, newid, s, g->d_gid, g->d_keep);
+    launch_exclusive_scan(newid, newid, g->nv, g->d_tot, g->d_scan_tmp, s);
+    launch_gather_bases(newid, g->d_tot, g->d_vb, g->G, g->nv, g->d_vbn, s);
+    HIP_TRY(hipGetLastError());
+    nvb.resize((size_t)g->G + 1);
+    HIP_TRY(hipMemcpyAsync(nvb.data(), g->d_vbn, sizeof(int64_t) * nvb.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return NIDO_OK;
+}
+
+int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, nido_run_stats* st)
+{
+    const int G = g->G;
+    const double t0 = now_s();
+    g->bres.assign((size_t)G, BatchResult{});
+    std::vector<unsigned char> alive((size_t)G, 1), act((size_t)G), flag((size_t)G);
+    std::vector<int64_t> base((size_t)G, 0);   // subtracted from clus to give graph-local ids
+    std::vector<double> Q((size_t)G), Q0((size_t)G), dQ((size_t)G), qt;
+    std::vector<int64_t> loops((size_t)G);
+    nido_run_stats S{};
+    int nphase = 0;
+    auto any = [](const std::vector<unsigned char>& f) {
+        for (unsigned char x : f)
+            if (x) return true;
+        return false;
+    };
+    while (any(alive) && nphase < NIDO_MAX_PHASES) {
+        NIDO_TRY(do_singleton(g));
+        NIDO_TRY(batch_modularity(g, Q));
+        for (int i = 0; i < G; ++i) {
+            Q0[(size_t)i] = Q[(size_t)i];
+            dQ[(size_t)i] = 1.0e308;
+            loops[(size_t)i] = 0;
+            act[(size_t)i] = alive[(size_t)i] && g->twom_g[(size_t)i] != 0;
+            if (alive[(size_t)i] && !act[(size_t)i]) {
+                // no weight: one pass with no moves and Q = 0 (the GraphGPU
+                // path short-circuits the same way)
+                loops[(size_t)i] = 1;
+                Q[(size_t)i] = 0.0;
+                dQ[(size_t)i] = 0.0;
+            }
+        }
+        const double ts = now_s();
+        int64_t iters = 0;
+        while (any(act)) {
+            NIDO_TRY(upload_flags(g, g->d_act, act));
+            for (int b = 0; b < g->nb; ++b) {
+                NIDO_TRY(do_lm(g, b));
+                NIDO_TRY(do_commit(g, b));
+            }
+            NIDO_TRY(batch_modularity(g, qt));
+            bool rollback = false;
+            for (int i = 0; i < G; ++i) {
+                flag[(size_t)i] = 0;
+                if (!act[(size_t)i]) continue;
+                dQ[(size_t)i] = qt[(size_t)i] - Q[(size_t)i];
+                loops[(size_t)i]++;
+                if (dQ[(size_t)i] < 0) {
+                    flag[(size_t)i] = 1;            // graph_gpu.cpp:822-842
+                    rollback = true;
+                } else {
+                    Q[(size_t)i] = qt[(size_t)i];
+                }
+                act[(size_t)i] = tol < dQ[(size_t)i] && loops[(size_t)i] < max_loops;   // louvain_gpu.cpp:53
+            }
+            if (rollback) {
+                NIDO_TRY(upload_flags(g, g->d_keep, flag));
+                launch_commit_filtered(nullptr, 0, g->nv, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->d_gid,
+                                       g->d_keep, g->stream);
+                HIP_TRY(hipGetLastError());
+            }
+            ++iters;
+        }
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        const double te = now_s();
+        S.t_loop += te - ts;
+        S.total_loops += iters;
+        // phase end, per graph (louvain_gpu.cpp:107-118)
+        std::vector<unsigned char> cand((size_t)G, 0);
+        for (int i = 0; i < G; ++i) {
+            if (!alive[(size_t)i]) continue;
+            BatchResult& R = g->bres[(size_t)i];
+            nido_phase_stats ps{};
+            ps.nv = g->vb[(size_t)i + 1] - g->vb[(size_t)i];
+            ps.ne = g->h_off[(size_t)g->vb[(size_t)i + 1]] - g->h_off[(size_t)g->vb[(size_t)i]];
+            ps.q0 = Q0[(size_t)i];
+            ps.q = Q[(size_t)i];
+            ps.loops = loops[(size_t)i];
+            ps.t_loop = te - ts;
+            R.phases.push_back(ps);
+            R.st.total_loops += loops[(size_t)i];
+            R.st.q = Q[(size_t)i];
+            R.st.phases = (int)R.phases.size();
+            if (Q[(size_t)i] - Q0[(size_t)i] > tol_phase) {
+                cand[(size_t)i] = 1;
+            } else {
+                alive[(size_t)i] = 0;                  // ends with the clustering of the phase before
+                base[(size_t)i] = g->vb[(size_t)i];
+                R.st.done = 1;
+            }
+        }
+        ++nphase;
+        if (!any(cand)) break;
+        // aggregation of the candidates: renumber, update their clustering
+        // (graph_gpu.cpp:1389), then stop those with K == N or K == 1
+        // (graph_gpu.cpp:1391-1392) and coarsen the rest
+        const double a0 = now_s();
+        std::vector<int64_t> nvb;
+        NIDO_TRY(batch_renumber(g, cand, nvb));
+        std::vector<unsigned char> fin((size_t)G, 0), cont((size_t)G, 0);
+        for (int i = 0; i < G; ++i) {
+            if (!cand[(size_t)i]) continue;
+            const int64_t K = nvb[(size_t)i + 1] - nvb[(size_t)i];
+            const int64_t n = g->vb[(size_t)i + 1] - g->vb[(size_t)i];
+            if (K == n || K == 1) {
+                fin[(size_t)i] = 1;
+                alive[(size_t)i] = 0;
+                base[(size_t)i] = nvb[(size_t)i];
+                g->bres[(size_t)i].st.done = 1;
+            } else {
+                cont[(size_t)i] = 1;
+            }
+        }
+        if (any(fin)) {
+            NIDO_TRY(upload_flags(g, g->d_keep, fin));
+            launch_update_clustering(g->d_clus, g->n0, g->d_C, g->d_work, g->stream, g->d_gid0, g->d_keep);
+            HIP_TRY(hipGetLastError());
+        }
+        if (any(cont)) {
+            if (any(fin)) NIDO_TRY(batch_renumber(g, cont, nvb));
+            NIDO_TRY(upload_flags(g, g->d_keep, cont));
+            launch_update_clustering(g->d_clus, g->n0, g->d_C, g->d_work, g->stream, g->d_gid0, g->d_keep);
+            HIP_TRY(hipGetLastError());
+            NIDO_TRY(coarsen(g, g->d_work, nvb[(size_t)G], g->d_gid, g->d_keep));
+            g->vb = nvb;
+            HIP_TRY(hipMemcpyAsync(g->d_vb, g->vb.data(), sizeof(int64_t) * g->vb.size(), hipMemcpyHostToDevice,
+                                   g->stream));
+            launch_fill_gid(g->d_vb, G, g->d_gid, g->stream);
+            HIP_TRY(hipGetLastError());
+            NIDO_TRY(setup_phase(g));
+        }
+        const double at = now_s() - a0;
+        S.t_agg += at;
+        for (int i = 0; i < G; ++i)
+            if (cand[(size_t)i]) g->bres[(size_t)i].phases.back().t_agg = at;
+        if (!any(cont)) break;
+    }
+    // clusterings: graph-local ids
+    std::vector<int64_t> clus((size_t)g->n0);
+    if (g->n0)
+        HIP_TRY(hipMemcpy(clus.data(), g->d_clus, sizeof(int64_t) * (size_t)g->n0, hipMemcpyDeviceToHost));
+    for (int i = 0; i < G; ++i) {
+        BatchResult& R = g->bres[(size_t)i];
+        const int64_t o0 = g->vb0[(size_t)i], o1 = g->vb0[(size_t)i + 1];
+        R.clustering.resize((size_t)(o1 - o0));
+        for (int64_t v = o0; v < o1; ++v) R.clustering[(size_t)(v - o0)] = clus[(size_t)v] - base[(size_t)i];
+    }
+    S.t_total = now_s() - t0;
+    S.phases = nphase;
+    S.done = 1;
+    for (int i = 0; i < G; ++i) {
+        BatchResult& R = g->bres[(size_t)i];
+        R.st.t_total = S.t_total;
+        S.q += R.st.q;
+    }
+    g->last = S;
+    if (st) *st = S;
+    return NIDO_OK;
+}
+
 int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int nb,
-                void* stream, nido_graph** out)
+                void* stream, nido_graph** out, int G = 1, const int64_t* vptr = nullptr)
 {
     if (!out) return set_error(NIDO_ERR_INVALID, "null output handle");
     *out = nullptr;
@@ -853,6 +1172,36 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     if (nv > 0 && hipMemcpyAsync(g->d_clus, ident.data(), sizeof(int64_t) * (size_t)nv, hipMemcpyHostToDevice, s) != hipSuccess)
         return fail(set_error(NIDO_ERR_HIP, "upload failed"));
     if (hipStreamSynchronize(s) != hipSuccess) return fail(set_error(NIDO_ERR_HIP, "upload sync failed"));
+    if (G > 1) {
+        g->G = G;
+        g->vb.assign(vptr, vptr + G + 1);
+        g->vb0 = g->vb;
+        g->twom_g.assign((size_t)G, 0);
+        g->twom_dg.assign((size_t)G, 0.0);
+        std::vector<double> mg((size_t)G), tw((size_t)G);
+        for (int i = 0; i < G; ++i) {
+            fx_t sum = 0;   // 2m of graph i, exactly (graph_cuda.cu:1855 sums the same k_v)
+            for (int64_t e = g->h_off[(size_t)vptr[i]]; e < g->h_off[(size_t)vptr[i + 1]]; ++e)
+                sum += fx_from_double(w[base + e]);
+            g->twom_g[(size_t)i] = sum;
+            tw[(size_t)i] = g->twom_dg[(size_t)i] = fx_to_double(sum);
+            mg[(size_t)i] = tw[(size_t)i] * 0.5;
+        }
+        if ((rc = dev_alloc(&g->d_gid, nv)) || (rc = dev_alloc(&g->d_gid0, nv)) ||
+            (rc = dev_alloc(&g->d_vb, G + 1)) || (rc = dev_alloc(&g->d_vbn, G + 1)) ||
+            (rc = dev_alloc(&g->d_mg, G)) || (rc = dev_alloc(&g->d_twomg, G)) || (rc = dev_alloc(&g->d_act, G)) ||
+            (rc = dev_alloc(&g->d_keep, G)) || (rc = dev_alloc(&g->d_gacc, 2 * (int64_t)G)) ||
+            (rc = dev_alloc(&g->d_gq, G)))
+            return fail(rc);
+        if (hipMemcpyAsync(g->d_vb, g->vb.data(), sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(g->d_mg, mg.data(), sizeof(double) * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(g->d_twomg, tw.data(), sizeof(double) * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail(set_error(NIDO_ERR_HIP, "upload failed"));
+        launch_fill_gid(g->d_vb, G, g->d_gid, s);
+        launch_fill_gid(g->d_vb, G, g->d_gid0, s);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            return fail(set_error(NIDO_ERR_HIP, "graph id set-up failed"));
+    }
     if ((rc = setup_phase(g))) return fail(rc);
     if ((rc = do_singleton(g))) return fail(rc);
     *out = g;
@@ -908,6 +1257,9 @@ int nido_graph_destroy(nido_graph* g)
     dev_free(g->d_C); dev_free(g->d_Cn); dev_free(g->d_k); dev_free(g->d_a); dev_free(g->d_kd);
     dev_free(g->d_clus); dev_free(g->d_work); dev_free(g->d_acc); dev_free(g->d_q); dev_free(g->d_tot);
     dev_free(g->d_src); dev_free(g->d_part); dev_free(g->d_col32); dev_free(g->d_C32);
+    dev_free(g->d_gid); dev_free(g->d_gid0); dev_free(g->d_vb); dev_free(g->d_vbn); dev_free(g->d_mg);
+    dev_free(g->d_twomg); dev_free(g->d_act); dev_free(g->d_keep); dev_free(g->d_gacc); dev_free(g->d_gq);
+    dev_free(g->d_seg);
     dev_free(g->d_binv); dev_free(g->d_big_items); dev_free(g->d_big_poff); dev_free(g->d_big_toff);
     dev_free(g->d_key_chunks); dev_free(g->d_bkey); dev_free(g->d_bw); dev_free(g->d_cpo);
     dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt);
@@ -972,13 +1324,14 @@ int nido_graph_sync(nido_graph* g)
 
 int nido_singleton_partition(nido_graph* g)
 {
-    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    NIDO_TRY(single_graph(g));
     return do_singleton(g);
 }
 
 int nido_compute_modularity(nido_graph* g, double* q)
 {
-    if (!g || !q) return set_error(NIDO_ERR_INVALID, "null argument");
+    NIDO_TRY(single_graph(g));
+    if (!q) return set_error(NIDO_ERR_INVALID, "null argument");
     return do_modularity(g, q);
 }
 
@@ -1004,13 +1357,14 @@ int nido_update_community_ids(nido_graph* g, int batch)
 
 int nido_restore_community(nido_graph* g)
 {
-    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    NIDO_TRY(single_graph(g));
     return do_restore(g);
 }
 
 int nido_aggregation(nido_graph* g, int* done)
 {
-    if (!g || !done) return set_error(NIDO_ERR_INVALID, "null argument");
+    NIDO_TRY(single_graph(g));
+    if (!done) return set_error(NIDO_ERR_INVALID, "null argument");
     API_GUARD(return do_aggregation(g, done);)
 }
 
@@ -1072,7 +1426,8 @@ int nido_dump_partition(nido_graph* g, const char* path)
 
 int nido_louvain_iteration(nido_graph* g, double* q, double* dq, double* q_eval)
 {
-    if (!g || !q || !dq) return set_error(NIDO_ERR_INVALID, "null argument");
+    NIDO_TRY(single_graph(g));
+    if (!q || !dq) return set_error(NIDO_ERR_INVALID, "null argument");
     API_GUARD(return do_iteration(g, q, dq, q_eval);)
 }
 
@@ -1101,8 +1456,91 @@ int nido_graph_kernel_stats(nido_graph* g, int cls, int64_t* launches, double* m
 int nido_louvain_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, int record,
                      int verbose, nido_run_stats* stats)
 {
-    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    NIDO_TRY(single_graph(g));
     API_GUARD(return do_run(g, max_loops, tol, tol_phase, record, verbose, stats);)
+}
+
+int nido_batch_create(int ngraphs, const int64_t* vptr, const int64_t* off, const int64_t* col,
+                      const double* w, int nbatches, void* stream, nido_graph** out)
+{
+    if (!out) return set_error(NIDO_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (ngraphs < 1 || !vptr || !off) return set_error(NIDO_ERR_INVALID, "need ngraphs >= 1, vptr and offsets");
+    if (vptr[0] != 0) return set_error(NIDO_ERR_INVALID, "vptr[0] must be 0");
+    for (int i = 0; i < ngraphs; ++i)
+        if (vptr[i + 1] < vptr[i]) return set_error(NIDO_ERR_INVALID, "vptr decreases at graph %d", i);
+    const int64_t nv = vptr[ngraphs];
+    if (nv >= (1ll << 31) - 1) return set_error(NIDO_ERR_CAPACITY, "total vertices must be below 2^31-1");
+    // graphs are disjoint: graph i's arcs stay inside [vptr[i], vptr[i+1])
+    for (int i = 0; i < ngraphs; ++i)
+        for (int64_t v = vptr[i]; v < vptr[i + 1]; ++v) {
+            if (off[v + 1] < off[v]) return set_error(NIDO_ERR_INVALID, "offsets decrease at %lld", (long long)v);
+            if (!col) continue;
+            for (int64_t e = off[v]; e < off[v + 1]; ++e)
+                if (col[e] < vptr[i] || col[e] >= vptr[i + 1])
+                    return set_error(NIDO_ERR_INVALID, "arc %lld of graph %d leaves the graph", (long long)e, i);
+        }
+    API_GUARD(return create_impl(nv, off, col, w, nbatches, stream, out, ngraphs == 1 ? 1 : ngraphs, vptr);)
+}
+
+int nido_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, nido_run_stats* stats)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (g->G == 1) {
+        // one graph: the plain path, with its result kept in the batched form
+        nido_run_stats S{};
+        API_GUARD(NIDO_TRY(do_run(g, max_loops, tol, tol_phase, 0, 0, &S));
+                  g->bres.assign(1, BatchResult{});
+                  g->bres[0].st = S;
+                  for (const auto& h : g->hist) g->bres[0].phases.push_back(h.st);
+                  g->bres[0].clustering.resize((size_t)g->n0);
+                  if (g->n0) HIP_TRY(hipMemcpy(g->bres[0].clustering.data(), g->d_clus,
+                                               sizeof(int64_t) * (size_t)g->n0, hipMemcpyDeviceToHost));
+                  if (stats) *stats = S;
+                  return NIDO_OK;)
+    }
+    API_GUARD(return do_batch_run(g, max_loops, tol, tol_phase, stats);)
+}
+
+int nido_batch_size(const nido_graph* g, int* ngraphs)
+{
+    if (!g || !ngraphs) return set_error(NIDO_ERR_INVALID, "null argument");
+    *ngraphs = g->G;
+    return NIDO_OK;
+}
+
+static int check_result(const nido_graph* g, int graph)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (graph < 0 || graph >= g->G) return set_error(NIDO_ERR_INVALID, "graph %d out of range", graph);
+    if (g->bres.size() != (size_t)g->G) return set_error(NIDO_ERR_STATE, "no batched run yet");
+    return NIDO_OK;
+}
+
+int nido_batch_graph_stats(const nido_graph* g, int graph, nido_run_stats* out)
+{
+    NIDO_TRY(check_result(g, graph));
+    if (!out) return set_error(NIDO_ERR_INVALID, "null argument");
+    *out = g->bres[(size_t)graph].st;
+    return NIDO_OK;
+}
+
+int nido_batch_graph_phase(const nido_graph* g, int graph, int phase, nido_phase_stats* out)
+{
+    NIDO_TRY(check_result(g, graph));
+    const auto& P = g->bres[(size_t)graph].phases;
+    if (!out || phase < 0 || phase >= (int)P.size()) return set_error(NIDO_ERR_INVALID, "phase %d out of range", phase);
+    *out = P[(size_t)phase];
+    return NIDO_OK;
+}
+
+int nido_batch_clustering(const nido_graph* g, int graph, int64_t* comm)
+{
+    NIDO_TRY(check_result(g, graph));
+    if (!comm) return set_error(NIDO_ERR_INVALID, "null argument");
+    const auto& c = g->bres[(size_t)graph].clustering;
+    std::copy(c.begin(), c.end(), comm);
+    return NIDO_OK;
 }
 
 int nido_run_phase(const nido_graph* g, int phase, nido_phase_stats* out)

@@ -1,0 +1,75 @@
+"""Many graphs in one handle (nido_batch_run): every graph's run must equal
+its run alone -- same phases, iterations per phase, modularity and final
+clustering -- checked against the oracle, graph by graph."""
+import numpy as np
+import pytest
+
+from parity import Q_TOL, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _na():
+    import nido_amd as na
+    assert na.device_count() >= 1, "no HIP device visible"
+    return na
+
+
+def _mixed_graphs(na):
+    gs = []
+    # planted partitions of different sizes and densities, weighted and not
+    for i, (n, blk, pin, pout, wtd) in enumerate([(256, 64, 12 / 63, 2 / 192, True), (700, 64, 0.2, 0.01, False),
+                                                   (1500, 50, 0.25, 0.004, True), (3000, 64, 12 / 63, 2 / 2936, True),
+                                                   (330, 30, 0.4, 0.05, True)]):
+        gs.append(na.gen_sbm(n, blk, pin, pout, wtd, 1000 + i).arrays())
+    gs.append(na.gen_rmat(10, 8, 0.57, 0.19, 0.19, True, 3).arrays())        # skewed, has long rows
+    gs.append((np.array([0, 1, 3, 4]), np.array([1, 0, 2, 1]), np.ones(4)))   # path: rolls back
+    gs.append((np.array([0, 1, 2]), np.array([1, 0]), np.array([1.0, 1.0])))  # one edge
+    gs.append((np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0)))    # no vertices
+    gs.append((np.zeros(6, np.int64), np.zeros(0, np.int64), np.zeros(0)))    # isolated vertices
+    gs.append((np.array([0, 1, 2]), np.array([1, 0]), np.array([0.0, 0.0])))  # arcs of weight 0
+    return gs
+
+
+def _check(na, graphs, nb, **tuning):
+    ref = [run_oracle(off, col, w, nb) for off, col, w in graphs]
+    bg = na.BatchGPU(graphs, nb)
+    st = bg.run(500, 1e-6)
+    assert st.phases == max(len(r["phases"]) for r in ref)
+    for i, r in enumerate(ref):
+        gs = bg.graph_stats(i)
+        assert gs.phases == len(r["phases"]), f"graph {i}: phases {gs.phases} vs {len(r['phases'])}"
+        for p, rp in enumerate(r["phases"]):
+            ps = bg.graph_phase(i, p)
+            assert ps.nv == rp["nv"] and ps.ne == rp["ne"], f"graph {i} phase {p}"
+            assert ps.loops == rp["loops"], f"graph {i} phase {p}: loops {ps.loops} vs {rp['loops']}"
+            assert abs(ps.q - rp["q"]) <= Q_TOL and abs(ps.q0 - rp["q0"]) <= Q_TOL, f"graph {i} phase {p}"
+        assert abs(gs.q - r["q"]) <= Q_TOL, f"graph {i}: Q {gs.q} vs {r['q']}"
+        assert np.array_equal(bg.clustering(i), r["clustering"]), f"graph {i}: clustering differs"
+    bg.close()
+
+
+@pytest.mark.parametrize("nb", [1, 4, 32])
+def test_batch_equals_each_graph_alone(nb):
+    na = _na()
+    print(f"case: mixed batch nb={nb}", flush=True)
+    _check(na, _mixed_graphs(na), nb)
+
+
+def test_batch_of_one_graph_is_the_plain_run():
+    na = _na()
+    off, col, w = na.gen_sbm(2000, 64, 0.2, 0.003, True, 5).arrays()
+    print("case: batch of one", flush=True)
+    _check(na, [(off, col, w)], 8)
+
+
+def test_batch_many_small_graphs():
+    na = _na()
+    rng = np.random.default_rng(7)
+    graphs = []
+    for i in range(64):
+        n = int(np.exp(rng.uniform(np.log(128), np.log(2048))))
+        graphs.append(na.gen_sbm(n, 64, 12 / 63, 2 / max(n - 64, 1), True, 2000 + i).arrays())
+    print("case: 64 small sbms nb=32", flush=True)
+    _check(na, graphs, 32)
+

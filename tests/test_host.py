@@ -165,3 +165,25 @@ def test_batch_bounds_host_matches_oracle():
     off, _, _ = na.gen_rmat(10, 8, 0.57, 0.19, 0.19, False, 1).arrays()
     for nb in (1, 2, 7, 32):
         assert np.array_equal(na.batch_bounds_host(off, nb), oracle.batch_bounds(off, nb))
+
+
+def test_batch_create_validates_on_the_host():
+    # arcs must stay inside their graph's vertex range; checked before any device work
+    vptr = np.array([0, 2, 4])
+    off = np.array([0, 1, 2, 3, 4])
+    with pytest.raises(na.NidoError) as e:
+        na.BatchGPU(csr=(vptr, off, np.array([1, 2, 3, 2]), np.ones(4)), nbatches=2)
+    assert e.value.code == -1
+    with pytest.raises(na.NidoError):
+        na.BatchGPU(csr=(np.array([0, 3, 2]), np.array([0, 0, 0, 0]), np.zeros(0, np.int64), np.zeros(0)),
+                    nbatches=2)
+
+
+def test_concat_graphs_layout():
+    g1 = (np.array([0, 1, 2]), np.array([1, 0]), np.array([0.5, 0.5]))
+    g2 = (np.array([0, 2, 3, 4]), np.array([1, 2, 0, 0]), np.array([1.0, 2.0, 1.0, 2.0]))
+    vptr, off, col, w = na.concat_graphs([g1, g2])
+    assert list(vptr) == [0, 2, 5]
+    assert list(off) == [0, 1, 2, 4, 5, 6]
+    assert list(col) == [1, 0, 3, 4, 2, 2]
+    assert list(w) == [0.5, 0.5, 1.0, 2.0, 1.0, 2.0]
