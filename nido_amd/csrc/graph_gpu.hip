@@ -280,7 +280,7 @@ void default_tuning(nido_tuning& t)
     t.use_graph = 0;
     t.big_slots = kBigSlots;
     t.split_small = 0;
-    t.overlap_long = 1;
+    t.overlap_long = 0;
 }
 
 // partition_graph_edge_batch with part_on_batch=1 on one device,

@@ -37,10 +37,13 @@ __device__ __forceinline__ double gain_of(const LmArgs& A, double m, double ki, 
     return g;
 }
 
-// Mass of v's graph; false when that graph sits out the iteration.
+// Mass of v's graph; false when that graph sits out the iteration.  The
+// kernels are compiled twice: Multi = false (one graph) leaves out the
+// graph lookups altogether.
+template <bool Multi>
 __device__ __forceinline__ bool graph_mass(const LmArgs& A, int v, double& m)
 {
-    if (!A.gid) {
+    if constexpr (!Multi) {
         m = A.m;
         return true;
     }
@@ -143,12 +146,13 @@ __device__ __forceinline__ void scan_row(const LmArgs& A, int v, int lane, int l
 
 // ---- launch 1 bodies (blocks [bi, bi+nbk) of the fused launch) -------------
 
-__device__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk)
+template <bool Multi>
+__device__ __forceinline__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk)
 {
     for (int64_t i = (int64_t)bi * kBlock + threadIdx.x; i < n; i += (int64_t)nbk * kBlock) {
         const int v = verts[i];
         double m;
-        if (!graph_mass(A, v, m)) continue;
+        if (!graph_mass<Multi>(A, v, m)) continue;
         const int own = A.C[v];
         const double ki = A.kd[v];
         const fx_t kv = A.k[v];
@@ -186,8 +190,8 @@ __device__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int6
 
 // G lanes per vertex (G <= 64), S-slot LDS table per vertex; 256/G vertices
 // per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
-template <int G, int S>
-__device__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
+template <int G, int S, bool Multi>
+__device__ __forceinline__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
                            char* lds)
 {
     constexpr int NG = kBlock / G;
@@ -210,7 +214,7 @@ __device__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64
         bool act = idx < n;
         const int v = act ? verts[idx] : 0;
         double m = 0.0;
-        act = act && graph_mass(A, v, m);
+        act = act && graph_mass<Multi>(A, v, m);
         const int own = act ? A.C[v] : 0;
         const double ki = act ? A.kd[v] : 0.0;
         const fx_t rest = act ? A.a[own] - A.k[v] : fx_t(0);
@@ -242,8 +246,8 @@ __device__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64
 }
 
 // One workgroup per vertex, S-slot LDS table.
-template <int S>
-__device__ void block_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
+template <int S, bool Multi>
+__device__ __forceinline__ void block_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
                            char* lds, BlockRed& R)
 {
     int* key = reinterpret_cast<int*>(lds);
@@ -257,7 +261,7 @@ __device__ void block_body(const LmArgs& A, const int* __restrict__ verts, int64
         }
         const int v = verts[idx];
         double m;
-        if (!graph_mass(A, v, m)) continue;   // uniform: the workgroup holds one vertex
+        if (!graph_mass<Multi>(A, v, m)) continue;   // uniform: the workgroup holds one vertex
         const int own = A.C[v];
         const double ki = A.kd[v];
         const fx_t rest = A.a[own] - A.k[v];
@@ -285,34 +289,36 @@ __device__ void block_body(const LmArgs& A, const int* __restrict__ verts, int64
 constexpr int kFillUnroll = (int)(kKeyChunkArcs / kBlock);
 constexpr int kFillScan = (kMaxParts + 1 + kBlock - 1) / kBlock;   // counters per thread in the scan
 
-__device__ void keyfill_body(const LmArgs& A, const SmallBins& L, int bi, int nbk, char* lds)
+template <bool Multi>
+__device__ __forceinline__ void keyfill_body(const LmArgs& A, const SmallBins& L, int bi, int nbk, char* lds)
 {
     int* cnt = reinterpret_cast<int*>(lds);      // kMaxParts + 1 counters
     int* wtot = cnt + kMaxParts + 1;
     for (int64_t i = bi; i < L.nchunks; i += nbk) {
         const KeyChunk ch = L.chunks[i];
         double m;
-        if (!graph_mass(A, ch.v, m)) continue;   // uniform over the workgroup
+        if (!graph_mass<Multi>(A, ch.v, m)) continue;   // uniform over the workgroup
         const int P = ch.P;
         const int psh = 32 - (31 - __builtin_clz((unsigned)P));   // top log2(P) bits of the hash
         for (int s = threadIdx.x; s <= P; s += kBlock) cnt[s] = 0;
         __syncthreads();
-        int uu[kFillUnroll];
-        double ww[kFillUnroll];
+        // cc: community key (-1 self arc, -2 past the chunk); rk: rank in
+        // its partition.  Weights are read again after the scan (L2-warm)
+        // rather than held in registers across it.
+        int cc[kFillUnroll], rk[kFillUnroll];
 #pragma unroll
         for (int j = 0; j < kFillUnroll; ++j) {
             const int64_t e = ch.e0 + j * kBlock + threadIdx.x;
-            uu[j] = e < ch.e1 ? A.col[e] : -1;
-            ww[j] = e < ch.e1 ? A.w[e] : 0.0;
+            cc[j] = e < ch.e1 ? A.col[e] : -2;
         }
-        int cc[kFillUnroll], pp[kFillUnroll], rk[kFillUnroll];
+#pragma unroll
+        for (int j = 0; j < kFillUnroll; ++j) cc[j] = cc[j] >= 0 ? (cc[j] != ch.v ? A.C[cc[j]] : -1) : -2;
+        auto part_of = [&](int c) { return c == -2 ? -1 : c < 0 ? P : (int)(((unsigned)c * kPartMul) >> psh); };
 #pragma unroll
         for (int j = 0; j < kFillUnroll; ++j) {
-            cc[j] = uu[j] >= 0 && uu[j] != ch.v ? A.C[uu[j]] : -1;
-            pp[j] = uu[j] < 0 ? -1 : cc[j] < 0 ? P : (int)(((unsigned)cc[j] * kPartMul) >> psh);
+            const int pj = part_of(cc[j]);
+            rk[j] = pj >= 0 ? atomicAdd(&cnt[pj], 1) : 0;
         }
-#pragma unroll
-        for (int j = 0; j < kFillUnroll; ++j) rk[j] = pp[j] >= 0 ? atomicAdd(&cnt[pp[j]], 1) : 0;
         __syncthreads();
         // thread t scans counters [t * kFillScan, t * kFillScan + kFillScan)
         int loc[kFillScan];
@@ -337,10 +343,11 @@ __device__ void keyfill_body(const LmArgs& A, const SmallBins& L, int bi, int nb
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kFillUnroll; ++j) {
-            if (pp[j] >= 0 && pp[j] < P) {
-                const int64_t pos = ch.k0 + cnt[pp[j]] + rk[j];
+            const int pj = part_of(cc[j]);
+            if (pj >= 0 && pj < P) {
+                const int64_t pos = ch.k0 + cnt[pj] + rk[j];
                 L.bkey[pos] = cc[j];
-                L.bw[pos] = ww[j];
+                L.bw[pos] = A.w[ch.e0 + j * kBlock + threadIdx.x];
             }
         }
         __syncthreads();          // cnt and wtot are reused by the next chunk
@@ -350,31 +357,39 @@ __device__ void keyfill_body(const LmArgs& A, const SmallBins& L, int bi, int nb
 constexpr int kSmallLds = 20480;   // (256/G)*S*20 for every short-row bin
 static_assert((kMaxParts + 1 + kBlock / 64) * 4 <= kSmallLds, "key-fill counters exceed the LDS buffer");
 
+template <bool Multi>
 __global__ void __launch_bounds__(kBlock)
 lm_small_kernel(LmArgs A, SmallBins L)
 {
     __shared__ __align__(16) char lds[kSmallLds];
     __shared__ BlockRed R;
+    // Every index into L is a constant (a dynamically indexed kernel
+    // argument would be copied to scratch memory).
     int bin = 0;
-    while (bin < NUM_SMALL_BINS && (int)blockIdx.x >= L.blk0[bin + 1]) ++bin;
-    const int bi = (int)blockIdx.x - L.blk0[bin];
-    const int nbk = L.blk0[bin + 1] - L.blk0[bin];
+#pragma unroll
+    for (int k = 1; k <= NUM_SMALL_BINS; ++k) bin += (int)blockIdx.x >= L.blk0[k];
+    const int b = (int)blockIdx.x;
+#define NIDO_BIN_ARGS(K) L.verts[K], L.n[K], b - L.blk0[K], L.blk0[K + 1] - L.blk0[K]
     switch (bin) {
-    case BIN_THREAD: thread_body(A, L.verts[bin], L.n[bin], bi, nbk); break;
-    case BIN_G8: group_body<8, 32>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
-    case BIN_G16: group_body<16, 64>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
-    case BIN_G32: group_body<32, 128>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
-    case BIN_G64: group_body<64, 256>(A, L.verts[bin], L.n[bin], bi, nbk, lds); break;
-    case BIN_B256: block_body<1024>(A, L.verts[bin], L.n[bin], bi, nbk, lds, R); break;
-    default: keyfill_body(A, L, bi, nbk, lds); break;
+    case BIN_THREAD: thread_body<Multi>(A, NIDO_BIN_ARGS(BIN_THREAD)); break;
+    case BIN_G8: group_body<8, 32, Multi>(A, NIDO_BIN_ARGS(BIN_G8), lds); break;
+    case BIN_G16: group_body<16, 64, Multi>(A, NIDO_BIN_ARGS(BIN_G16), lds); break;
+    case BIN_G32: group_body<32, 128, Multi>(A, NIDO_BIN_ARGS(BIN_G32), lds); break;
+    case BIN_G64: group_body<64, 256, Multi>(A, NIDO_BIN_ARGS(BIN_G64), lds); break;
+    case BIN_B256: block_body<1024, Multi>(A, NIDO_BIN_ARGS(BIN_B256), lds, R); break;
+    default:
+        keyfill_body<Multi>(A, L, b - L.blk0[NUM_SMALL_BINS], L.blk0[NUM_SMALL_BINS + 1] - L.blk0[NUM_SMALL_BINS],
+                            lds);
+        break;
     }
+#undef NIDO_BIN_ARGS
 }
 
 // ---- launch 2: partitioned long rows ---------------------------------------
 
 // Whole-row aggregation in an HBM table of `size` slots (>= 2 * row length):
 // the exact fallback taken when a partition outgrows its LDS table.
-__device__ void global_row_decide(const LmArgs& A, double m, int v, int* __restrict__ key,
+__device__ __forceinline__ void global_row_decide(const LmArgs& A, double m, int v, int* __restrict__ key,
                                   fx_t* __restrict__ tval, int64_t size, BlockRed& R)
 {
     unsigned long long* val = reinterpret_cast<unsigned long long*>(tval);
@@ -446,7 +461,7 @@ constexpr int kOverflow = -2; // partial community of a partition that outgrew i
 // chunks are scanned across the workgroup and the concatenated segments are
 // processed 256 entries at a time, each thread locating its chunk by binary
 // search in the scanned starts.  Every key a workgroup reads belongs to it.
-template <bool Narrow, int S>
+template <bool Narrow, int S, bool Multi>
 __global__ void __launch_bounds__(kBlock)
 lm_big_kernel(LmArgs A, BigArgs B)
 {
@@ -465,7 +480,7 @@ lm_big_kernel(LmArgs A, BigArgs B)
         const BigItem item = B.items[it];
         const int v = item.v, p = item.p, P = item.P, slot = item.slot;
         double m;
-        if (!graph_mass(A, v, m)) continue;   // uniform; every partition of the row skips it
+        if (!graph_mass<Multi>(A, v, m)) continue;   // uniform; every partition of the row skips it
         const int own = A.C[v];
         const double ki = A.kd[v];
         const int64_t e0 = A.off[v];
@@ -721,11 +736,17 @@ int small_bins_blocks(SmallBins& L, unsigned cap_per_bin)
     return total;
 }
 
+static void small_launch(const LmArgs& A, const SmallBins& L, int blocks, hipStream_t s)
+{
+    if (A.gid) lm_small_kernel<true><<<blocks, kBlock, 0, s>>>(A, L);
+    else lm_small_kernel<false><<<blocks, kBlock, 0, s>>>(A, L);
+}
+
 void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s)
 {
     if (bins && fill && !split) {
         const int blocks = L.blk0[NUM_SMALL_BINS + 1];
-        if (blocks > 0) lm_small_kernel<<<blocks, kBlock, 0, s>>>(A, L);
+        if (blocks > 0) small_launch(A, L, blocks, s);
         return;
     }
     // b < NUM_SMALL_BINS: bin b; b == NUM_SMALL_BINS: the key fill
@@ -735,7 +756,7 @@ void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, 
         for (int k = 0; k < NUM_SMALL_BINS; ++k) one.n[k] = (split ? k == b : bins) ? L.n[k] : 0;
         if (b != NUM_SMALL_BINS) one.nchunks = 0;
         if (small_bins_blocks(one, 4096) > 0)
-            lm_small_kernel<<<one.blk0[NUM_SMALL_BINS + 1], kBlock, 0, s>>>(A, one);
+            small_launch(A, one, one.blk0[NUM_SMALL_BINS + 1], s);
         if (!split && b < NUM_SMALL_BINS) b = NUM_SMALL_BINS - 1;   // all bins went in one launch
     }
 }
@@ -744,12 +765,16 @@ void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s)
 {
     if (B.nitems <= 0) return;
     const unsigned g = grid_for(B.nitems, 1, 2048);
-    if (B.slots == 2048) {
-        if (B.narrow) lm_big_kernel<true, 2048><<<g, kBlock, 0, s>>>(A, B);
-        else lm_big_kernel<false, 2048><<<g, kBlock, 0, s>>>(A, B);
-    } else {
-        if (B.narrow) lm_big_kernel<true, 4096><<<g, kBlock, 0, s>>>(A, B);
-        else lm_big_kernel<false, 4096><<<g, kBlock, 0, s>>>(A, B);
+    const int which = (B.slots == 2048 ? 0 : 4) + (B.narrow ? 0 : 2) + (A.gid ? 1 : 0);
+    switch (which) {
+    case 0: lm_big_kernel<true, 2048, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 1: lm_big_kernel<true, 2048, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 2: lm_big_kernel<false, 2048, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 3: lm_big_kernel<false, 2048, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 4: lm_big_kernel<true, 4096, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 5: lm_big_kernel<true, 4096, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 6: lm_big_kernel<false, 4096, false><<<g, kBlock, 0, s>>>(A, B); break;
+    default: lm_big_kernel<false, 4096, true><<<g, kBlock, 0, s>>>(A, B); break;
     }
 }
 
