@@ -244,6 +244,38 @@ int nido_batch_graph_phase(const nido_graph* g, int graph, int phase, nido_phase
  * (graph-local) -> final community (graph-local, dense from 0). */
 int nido_batch_clustering(const nido_graph* g, int graph, int64_t* comm);
 
+/* ---- request loop (continuous batching on one GPU) -----------------------
+ * No counterpart in the reference.  Requests (one graph each, copied at
+ * submission) are admitted in arrival order as soon as one of `slots`
+ * workers is free and the request's estimated HBM footprint fits in the
+ * unused part of hbm_budget; each worker runs LouvainGPU::run on its own
+ * stream, so admitted requests overlap on the device.  A request's result
+ * is the one it gets alone. */
+typedef struct nido_server nido_server;
+typedef struct {
+    int32_t slots;         /* concurrent requests (1..64) */
+    int32_t nbatches;      /* -b */
+    int64_t hbm_budget;    /* bytes of device memory the admitted requests may use together */
+    int64_t max_loops;     /* -i */
+    double tol;            /* -t */
+    double tol_phase;
+} nido_serve_config;
+int nido_serve_create(const nido_serve_config* cfg, nido_server** out);
+int64_t nido_serve_footprint(int64_t nv, int64_t ne);   /* admission estimate, bytes */
+/* Copies the graph unless flags has NIDO_SERVE_BORROW, in which case the
+ * caller keeps the arrays alive and unchanged until the request finishes. */
+#define NIDO_SERVE_BORROW 1
+int nido_serve_submit(nido_server* s, int64_t nv, const int64_t* offsets, const int64_t* cols,
+                      const double* weights, int flags, int64_t* id);
+/* Blocks until request `id` has finished; latency_s = finish - submission. */
+int nido_serve_wait(nido_server* s, int64_t id, nido_run_stats* stats, double* latency_s);
+int nido_serve_clustering(nido_server* s, int64_t id, int64_t* comm);   /* nv entries */
+int nido_serve_release(nido_server* s, int64_t id);                      /* forget a finished request */
+int nido_serve_status(nido_server* s, int64_t* queued, int64_t* running, int64_t* completed,
+                      int64_t* peak_running);
+/* Drains the queue, then stops the workers. */
+int nido_serve_destroy(nido_server* s);
+
 #ifdef __cplusplus
 }
 #endif

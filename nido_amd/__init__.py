@@ -131,6 +131,21 @@ _sig("nido_batch_graph_phase", C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(
 _sig("nido_batch_clustering", C.c_int, C.c_void_p, C.c_int, _pi64)
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("nbatches", C.c_int32), ("hbm_budget", C.c_int64),
+                ("max_loops", C.c_int64), ("tol", C.c_double), ("tol_phase", C.c_double)]
+
+
+_sig("nido_serve_create", C.c_int, C.POINTER(ServeConfig), C.POINTER(C.c_void_p))
+_sig("nido_serve_footprint", C.c_int64, _i64, _i64)
+_sig("nido_serve_submit", C.c_int, C.c_void_p, _i64, _pi64, _pi64, _pdbl, C.c_int, _pi64)
+_sig("nido_serve_wait", C.c_int, C.c_void_p, _i64, C.POINTER(RunStats), _pdbl)
+_sig("nido_serve_clustering", C.c_int, C.c_void_p, _i64, _pi64)
+_sig("nido_serve_release", C.c_int, C.c_void_p, _i64)
+_sig("nido_serve_status", C.c_int, C.c_void_p, _pi64, _pi64, _pi64, _pi64)
+_sig("nido_serve_destroy", C.c_int, C.c_void_p)
+
+
 class NidoError(RuntimeError):
     """A negative status from libnido_amd, with the library's message."""
 
@@ -518,6 +533,69 @@ class BatchGPU:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib.nido_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (nido_serve_*): submit() copies
+    the graph and returns at once; wait() blocks for that request."""
+
+    def __init__(self, slots: int = 8, nbatches: int = DEFAULT_BATCHES, hbm_budget: int = 200 << 30,
+                 max_loops: int = DEFAULT_ITERATIONS, tol: float = DEFAULT_THRESHOLD, tol_phase: float | None = None):
+        cfg = ServeConfig(slots, nbatches, hbm_budget, max_loops, tol, tol if tol_phase is None else tol_phase)
+        h = C.c_void_p()
+        _check(lib.nido_serve_create(C.byref(cfg), C.byref(h)), "nido_serve_create")
+        self._h = h
+        self._n = {}
+
+    def submit(self, graph, borrow: bool = False) -> int:
+        """borrow=True: the arrays are not copied; keep `graph` alive and
+        unchanged until wait() returns for this request."""
+        if isinstance(graph, Graph):
+            off, col, w = graph.arrays(copy=False)
+        else:
+            off, col, w = graph
+        off = np.ascontiguousarray(off, np.int64)
+        col = np.ascontiguousarray(col, np.int64)
+        w = np.ascontiguousarray(w, np.float64)
+        rid = C.c_int64()
+        _check(lib.nido_serve_submit(self._h, len(off) - 1, _p64(off), _p64(col), _pd(w), 1 if borrow else 0,
+                                     C.byref(rid)), "nido_serve_submit")
+        self._n[rid.value] = len(off) - 1
+        if borrow:
+            self._held = getattr(self, "_held", {})
+            self._held[rid.value] = (graph, off, col, w)
+        return rid.value
+
+    def wait(self, rid: int):
+        st, lat = RunStats(), C.c_double()
+        _check(lib.nido_serve_wait(self._h, rid, C.byref(st), C.byref(lat)), "nido_serve_wait")
+        return st, lat.value
+
+    def clustering(self, rid: int) -> np.ndarray:
+        out = np.zeros(max(self._n[rid], 1), dtype=np.int64)
+        _check(lib.nido_serve_clustering(self._h, rid, _p64(out)), "nido_serve_clustering")
+        return out[:self._n[rid]]
+
+    def release(self, rid: int) -> None:
+        _check(lib.nido_serve_release(self._h, rid), "nido_serve_release")
+        self._n.pop(rid, None)
+        getattr(self, "_held", {}).pop(rid, None)
+
+    def status(self) -> dict:
+        q, r, c, p = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib.nido_serve_status(self._h, C.byref(q), C.byref(r), C.byref(c), C.byref(p)), "nido_serve_status")
+        return {"queued": q.value, "running": r.value, "completed": c.value, "peak_running": p.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.nido_serve_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -6,6 +6,7 @@
 // and the host only reads back one scalar per modularity evaluation.
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -36,25 +37,86 @@ namespace {
 constexpr int64_t kSmallRow = 32;      // rows handled by one lane in the degree/modularity sweeps
 constexpr int64_t kItemArcs = 1024;    // longer rows are cut into slices of this many arcs
 
-template <typename T>
-int dev_alloc(T** p, int64_t count)
+// Device memory is stream-ordered: allocations and frees made while a
+// handle's AllocScope is active go through a library-owned memory pool on
+// that handle's stream (hipMallocFromPoolAsync / hipFreeAsync), so a free
+// never waits for the whole device -- handles running side by side on
+// their own streams (a serving loop) do not stall each other.  The pool
+// keeps freed memory for reuse.
+thread_local hipStream_t t_alloc_stream = nullptr;
+
+struct AllocScope {
+    hipStream_t prev;
+    explicit AllocScope(hipStream_t s) : prev(t_alloc_stream) { t_alloc_stream = s; }
+    ~AllocScope() { t_alloc_stream = prev; }
+};
+
+hipMemPool_t device_pool()
+{
+    static std::mutex mu;
+    static std::vector<hipMemPool_t> pools;
+    static std::vector<char> tried;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)pools.size() <= dev) {
+        pools.resize((size_t)dev + 1, nullptr);
+        tried.resize((size_t)dev + 1, 0);
+    }
+    if (!tried[(size_t)dev]) {
+        tried[(size_t)dev] = 1;
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            pools[(size_t)dev] = pool;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    return pools[(size_t)dev];
+}
+
+int raw_alloc(void** p, size_t bytes)
 {
     *p = nullptr;
-    if (count <= 0) count = 1;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)count);
+    if (bytes == 0) bytes = 16;
+    hipMemPool_t pool = t_alloc_stream ? device_pool() : nullptr;
+    hipError_t e = pool ? hipMallocFromPoolAsync(p, bytes, pool, t_alloc_stream) : hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
-        return set_error(NIDO_ERR_NOMEM, "hipMalloc of %lld bytes failed: %s",
-                         (long long)(sizeof(T) * (size_t)count), hipGetErrorString(e));
+        return set_error(NIDO_ERR_NOMEM, "device allocation of %lld bytes failed: %s", (long long)bytes,
+                         hipGetErrorString(e));
     }
     return NIDO_OK;
+}
+
+void raw_free(void* p)
+{
+    if (!p) return;
+    if (t_alloc_stream && device_pool()) (void)hipFreeAsync(p, t_alloc_stream);
+    else (void)hipFree(p);
+}
+
+template <typename T>
+int dev_alloc(T** p, int64_t count)
+{
+    void* q = nullptr;
+    const int rc = raw_alloc(&q, sizeof(T) * (size_t)(count > 0 ? count : 1));
+    *p = static_cast<T*>(q);
+    return rc;
 }
 
 template <typename T>
 void dev_free(T*& p)
 {
-    if (p) (void)hipFree(p);
+    raw_free(p);
     p = nullptr;
 }
 
@@ -737,7 +799,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
     auto cleanup = [&]() {
         dev_free(arcs); dev_free(toff); dev_free(cnt); dev_free(noff);
         dev_free(ncol); dev_free(nw); dev_free(tkey); dev_free(tval);
-        if (tmp) (void)hipFree(tmp);
+        raw_free(tmp);
         tmp = nullptr;
     };
     int rc = NIDO_OK;
@@ -745,10 +807,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         if ((rc = dev_alloc(&arcs, K + 1)) || (rc = dev_alloc(&toff, K + 1)) ||
             (rc = dev_alloc(&cnt, K + 1)) || (rc = dev_alloc(&noff, K + 1)))
             break;
-        if (hipMalloc(&tmp, scan_temp_bytes(K + 1)) != hipSuccess) {
-            rc = set_error(NIDO_ERR_NOMEM, "scan scratch allocation failed");
-            break;
-        }
+        if ((rc = raw_alloc(&tmp, scan_temp_bytes(K + 1)))) break;
         launch_fill_i64(arcs, 0, K + 1, s);
         launch_member_counts(g->d_C, newid, g->d_off, nv, nullptr, arcs, s, gid, keep);
         launch_table_sizes(arcs, K, arcs, s);                      // in place: arcs -> table size
@@ -853,7 +912,9 @@ int do_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, int r
         if (record) {
             h.comm.resize((size_t)g->nv);
             if (g->nv)
-                HIP_TRY(hipMemcpy(h.comm.data(), g->d_C, sizeof(int64_t) * (size_t)g->nv, hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpyAsync(h.comm.data(), g->d_C, sizeof(int64_t) * (size_t)g->nv, hipMemcpyDeviceToHost,
+                                       g->stream));
+            HIP_TRY(hipStreamSynchronize(g->stream));
         }
         if (Q - Q_old > tol_phase) {                    // louvain_gpu.cpp:108
             const double a0 = now_s();
@@ -1071,9 +1132,11 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
         if (!any(cont)) break;
     }
     // clusterings: graph-local ids
-    std::vector<int64_t> clus((size_t)g->n0);
+    std::vector<int64_t> clus((size_t)std::max<int64_t>(g->n0, 1));
     if (g->n0)
-        HIP_TRY(hipMemcpy(clus.data(), g->d_clus, sizeof(int64_t) * (size_t)g->n0, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(clus.data(), g->d_clus, sizeof(int64_t) * (size_t)g->n0, hipMemcpyDeviceToHost,
+                               g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
     for (int i = 0; i < G; ++i) {
         BatchResult& R = g->bres[(size_t)i];
         const int64_t o0 = g->vb0[(size_t)i], o1 = g->vb0[(size_t)i + 1];
@@ -1140,6 +1203,7 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess)
         return fail(set_error(NIDO_ERR_HIP, "creating the side stream or its events failed"));
+    AllocScope scope(g->stream);
     g->n0 = nv;
     g->nv = nv;
     g->ne = ne;
@@ -1156,8 +1220,7 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         (rc = dev_alloc(&g->d_col32, ne)) || (rc = dev_alloc(&g->d_C32, nv)) ||
         (rc = dev_alloc(&g->d_part, 2 * (int64_t)kPartBlocks)))
         return fail(rc);
-    if (hipMalloc(&g->d_scan_tmp, scan_temp_bytes(nv + 1)) != hipSuccess)
-        return fail(set_error(NIDO_ERR_NOMEM, "scan scratch allocation failed"));
+    if ((rc = raw_alloc(&g->d_scan_tmp, scan_temp_bytes(nv + 1)))) return fail(rc);
     if (hipHostMalloc(reinterpret_cast<void**>(&g->h_q), sizeof(double)) != hipSuccess)
         return fail(set_error(NIDO_ERR_NOMEM, "pinned allocation failed"));
     hipStream_t s = g->stream;
@@ -1253,6 +1316,7 @@ int nido_graph_destroy(nido_graph* g)
 {
     if (!g) return NIDO_OK;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
+    AllocScope scope(g->stream);
     free_graph_arrays(g);
     dev_free(g->d_C); dev_free(g->d_Cn); dev_free(g->d_k); dev_free(g->d_a); dev_free(g->d_kd);
     dev_free(g->d_clus); dev_free(g->d_work); dev_free(g->d_acc); dev_free(g->d_q); dev_free(g->d_tot);
@@ -1265,7 +1329,7 @@ int nido_graph_destroy(nido_graph* g)
     dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt);
     dev_free(g->d_tkey); dev_free(g->d_tval);
     dev_free(g->d_small); dev_free(g->d_items);
-    if (g->d_scan_tmp) (void)hipFree(g->d_scan_tmp);
+    raw_free(g->d_scan_tmp);
     if (g->h_q) (void)hipHostFree(g->h_q);
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
@@ -1305,6 +1369,7 @@ int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
         return set_error(NIDO_ERR_INVALID, "big_slots must be 2048 or 4096");
     if (t->big_key_limit < 1 || t->big_key_limit > t->big_slots / 2)
         return set_error(NIDO_ERR_INVALID, "big_key_limit must be in [1, big_slots/2]");
+    AllocScope scope(g->stream);
     API_GUARD(g->tuning = *t; return setup_phase(g);)
 }
 
@@ -1365,6 +1430,7 @@ int nido_aggregation(nido_graph* g, int* done)
 {
     NIDO_TRY(single_graph(g));
     if (!done) return set_error(NIDO_ERR_INVALID, "null argument");
+    AllocScope scope(g->stream);
     API_GUARD(return do_aggregation(g, done);)
 }
 
@@ -1457,6 +1523,7 @@ int nido_louvain_run(nido_graph* g, int64_t max_loops, double tol, double tol_ph
                      int verbose, nido_run_stats* stats)
 {
     NIDO_TRY(single_graph(g));
+    AllocScope scope(g->stream);
     API_GUARD(return do_run(g, max_loops, tol, tol_phase, record, verbose, stats);)
 }
 
@@ -1486,6 +1553,7 @@ int nido_batch_create(int ngraphs, const int64_t* vptr, const int64_t* off, cons
 int nido_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, nido_run_stats* stats)
 {
     if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    AllocScope scope(g->stream);
     if (g->G == 1) {
         // one graph: the plain path, with its result kept in the batched form
         nido_run_stats S{};
@@ -1494,8 +1562,7 @@ int nido_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phas
                   g->bres[0].st = S;
                   for (const auto& h : g->hist) g->bres[0].phases.push_back(h.st);
                   g->bres[0].clustering.resize((size_t)g->n0);
-                  if (g->n0) HIP_TRY(hipMemcpy(g->bres[0].clustering.data(), g->d_clus,
-                                               sizeof(int64_t) * (size_t)g->n0, hipMemcpyDeviceToHost));
+                  NIDO_TRY(d2h(g, g->bres[0].clustering.data(), g->d_clus, g->n0));
                   if (stats) *stats = S;
                   return NIDO_OK;)
     }

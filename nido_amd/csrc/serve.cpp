@@ -1,0 +1,266 @@
+// serve.cpp -- a Louvain request loop over the C ABI: continuous batching on
+// one GPU.
+//
+// Requests (one graph each) queue in arrival order.  A request is admitted
+// when one of `slots` workers is free and its estimated HBM footprint fits
+// in what is left of the budget; each worker owns a HIP stream and runs the
+// whole LouvainGPU::run (louvain_gpu.cpp:6-139) for its request there, with
+// every phase's state resident on the device, so admitted requests overlap
+// on the GPU.  A finished request frees its slot and its share of the
+// budget, and the next ones are admitted at once.  A request's result does
+// not depend on what runs beside it.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/nido_amd.h"
+#include "errors.h"
+
+using nido::set_error;
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    int64_t id = 0;
+    int64_t nv = 0;
+    std::vector<int64_t> off, col;
+    std::vector<double> w;
+    const int64_t* boff = nullptr;   // borrowed input (NIDO_SERVE_BORROW)
+    const int64_t* bcol = nullptr;
+    const double* bw = nullptr;
+    int64_t ne = 0;
+    int64_t footprint = 0;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    int state = 0;            // 0 queued, 1 running, 2 done
+    int rc = NIDO_OK;
+    std::string err;
+    nido_run_stats st{};
+    std::vector<int64_t> clustering;
+};
+
+}  // namespace
+
+struct nido_server {
+    nido_serve_config cfg{};
+    int device = 0;
+    std::vector<std::thread> workers;
+    std::vector<hipStream_t> streams;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<int64_t, Request*> reqs;
+    int64_t next_id = 1;
+    int64_t budget_free = 0;
+    int64_t running = 0, completed = 0, peak_running = 0;
+    bool stop = false;
+
+    void worker(int slot);
+};
+
+void nido_server::worker(int slot)
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            // a request fits once enough running ones finish (its footprint
+            // is checked against the whole budget at submission); on stop
+            // the queue is still drained
+            cv_work.wait(lock, [&] {
+                return (!queue.empty() && queue.front()->footprint <= budget_free) || (stop && queue.empty());
+            });
+            if (queue.empty()) return;
+            r = queue.front();
+            queue.pop_front();
+            budget_free -= r->footprint;
+            r->state = 1;
+            ++running;
+            if (running > peak_running) peak_running = running;
+        }
+        r->t_start = now_s();
+        nido_graph* g = nullptr;
+        const bool borrowed = r->boff != nullptr;
+        int rc = nido_graph_create(r->nv, r->ne, borrowed ? r->boff : r->off.data(),
+                                   borrowed ? r->bcol : r->col.data(), borrowed ? r->bw : r->w.data(),
+                                   cfg.nbatches, streams[(size_t)slot], &g);
+        if (rc == NIDO_OK)
+            rc = nido_louvain_run(g, cfg.max_loops, cfg.tol, cfg.tol_phase, 0, 0, &r->st);
+        if (rc == NIDO_OK) {
+            r->clustering.resize((size_t)r->nv);
+            rc = r->nv ? nido_get_clustering(g, r->clustering.data()) : NIDO_OK;
+        }
+        if (rc != NIDO_OK) r->err = nido_last_error();
+        if (g) nido_graph_destroy(g);
+        r->rc = rc;
+        // the input is no longer needed
+        std::vector<int64_t>().swap(r->off);
+        std::vector<int64_t>().swap(r->col);
+        std::vector<double>().swap(r->w);
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            r->t_end = now_s();
+            r->state = 2;
+            budget_free += r->footprint;
+            --running;
+            ++completed;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+}
+
+extern "C" {
+
+int64_t nido_serve_footprint(int64_t nv, int64_t ne)
+{
+    // resident CSR + 32-bit mirrors + per-vertex state + long-row and
+    // coarsening work space (bounded by a few bytes per arc)
+    return 160 * (nv + 1) + 96 * ne + (64ll << 20);
+}
+
+int nido_serve_create(const nido_serve_config* cfg, nido_server** out)
+{
+    if (!cfg || !out) return set_error(NIDO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->slots < 1 || cfg->slots > 64) return set_error(NIDO_ERR_INVALID, "slots must be in [1, 64]");
+    if (cfg->nbatches < 1) return set_error(NIDO_ERR_INVALID, "nbatches must be >= 1");
+    if (cfg->hbm_budget <= 0) return set_error(NIDO_ERR_INVALID, "hbm_budget must be > 0");
+    nido_server* s = new (std::nothrow) nido_server;
+    if (!s) return set_error(NIDO_ERR_NOMEM, "out of host memory");
+    s->cfg = *cfg;
+    s->budget_free = cfg->hbm_budget;
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return set_error(NIDO_ERR_HIP, "no HIP device");
+    }
+    for (int i = 0; i < cfg->slots; ++i) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            for (auto x : s->streams) (void)hipStreamDestroy(x);
+            delete s;
+            return set_error(NIDO_ERR_HIP, "stream creation failed");
+        }
+        s->streams.push_back(st);
+    }
+    for (int i = 0; i < cfg->slots; ++i) s->workers.emplace_back([s, i] { s->worker(i); });
+    *out = s;
+    return NIDO_OK;
+}
+
+int nido_serve_submit(nido_server* s, int64_t nv, const int64_t* off, const int64_t* col, const double* w,
+                      int flags, int64_t* id)
+{
+    if (!s || !off || !id || nv < 0) return set_error(NIDO_ERR_INVALID, "invalid argument");
+    const int64_t ne = off[nv] - off[0];
+    if (ne < 0 || (ne > 0 && (!col || !w))) return set_error(NIDO_ERR_INVALID, "invalid CSR");
+    Request* r = new (std::nothrow) Request;
+    if (!r) return set_error(NIDO_ERR_NOMEM, "out of host memory");
+    r->nv = nv;
+    r->footprint = nido_serve_footprint(nv, ne);
+    if (r->footprint > s->cfg.hbm_budget) {
+        delete r;
+        return set_error(NIDO_ERR_CAPACITY, "request needs ~%lld bytes, more than the whole budget",
+                         (long long)nido_serve_footprint(nv, ne));
+    }
+    r->ne = ne;
+    if (flags & NIDO_SERVE_BORROW) {
+        r->boff = off;
+        r->bcol = col;
+        r->bw = w;
+    } else {
+        r->off.resize((size_t)nv + 1);
+        for (int64_t i = 0; i <= nv; ++i) r->off[(size_t)i] = off[i] - off[0];
+        r->col.assign(col + off[0], col + off[0] + ne);
+        r->w.assign(w + off[0], w + off[0] + ne);
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        r->id = s->next_id++;
+        r->t_submit = now_s();
+        s->reqs[r->id] = r;
+        s->queue.push_back(r);
+    }
+    s->cv_work.notify_all();
+    *id = r->id;
+    return NIDO_OK;
+}
+
+int nido_serve_wait(nido_server* s, int64_t id, nido_run_stats* st, double* latency_s)
+{
+    if (!s) return set_error(NIDO_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lock(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return set_error(NIDO_ERR_INVALID, "unknown request %lld", (long long)id);
+    Request* r = it->second;
+    s->cv_done.wait(lock, [&] { return r->state == 2; });
+    if (r->rc != NIDO_OK) return set_error(r->rc, "request %lld failed: %s", (long long)id, r->err.c_str());
+    if (st) *st = r->st;
+    if (latency_s) *latency_s = r->t_end - r->t_submit;
+    return NIDO_OK;
+}
+
+int nido_serve_clustering(nido_server* s, int64_t id, int64_t* comm)
+{
+    if (!s || !comm) return set_error(NIDO_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return set_error(NIDO_ERR_INVALID, "unknown request %lld", (long long)id);
+    if (it->second->state != 2) return set_error(NIDO_ERR_STATE, "request %lld has not finished", (long long)id);
+    std::memcpy(comm, it->second->clustering.data(), sizeof(int64_t) * it->second->clustering.size());
+    return NIDO_OK;
+}
+
+int nido_serve_release(nido_server* s, int64_t id)
+{
+    if (!s) return set_error(NIDO_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return set_error(NIDO_ERR_INVALID, "unknown request %lld", (long long)id);
+    if (it->second->state != 2) return set_error(NIDO_ERR_STATE, "request %lld has not finished", (long long)id);
+    delete it->second;
+    s->reqs.erase(it);
+    return NIDO_OK;
+}
+
+int nido_serve_status(nido_server* s, int64_t* queued, int64_t* running, int64_t* completed, int64_t* peak_running)
+{
+    if (!s) return set_error(NIDO_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (queued) *queued = (int64_t)s->queue.size();
+    if (running) *running = s->running;
+    if (completed) *completed = s->completed;
+    if (peak_running) *peak_running = s->peak_running;
+    return NIDO_OK;
+}
+
+int nido_serve_destroy(nido_server* s)
+{
+    if (!s) return NIDO_OK;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;   // queued requests still run; workers exit once the queue is empty
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    for (auto x : s->streams) (void)hipStreamDestroy(x);
+    for (auto& kv : s->reqs) delete kv.second;
+    delete s;
+    return NIDO_OK;
+}
+
+}  // extern "C"
