@@ -281,6 +281,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kernel-events", action="store_true",
+                    help="time the steps without per-kernel HIP events (roofline fields are then null)")
     ap.add_argument("--tuning", default="", help="comma list of nido_tuning knobs, e.g. big_slots=2048")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device copies of BYTES each (scales the FETCH_SIZE / "
@@ -338,7 +340,7 @@ def main():
     for _ in range(args.warmup):
         step()
     g.sync()
-    g.set_profiling(True)
+    g.set_profiling(not args.no_kernel_events)
     g.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -389,6 +391,9 @@ def main():
         "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in ks.items()},
     }
     g.close()
+    if args.no_kernel_events:
+        out["roofline"] = None
+        out["kernel_ms_per_step"] = None
 
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")

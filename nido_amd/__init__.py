@@ -122,6 +122,7 @@ KERNEL_CLASSES = ("lm_short", "lm_long", "commit", "modularity")   # NIDO_K_* or
 _sig("nido_run_phase", C.c_int, C.c_void_p, C.c_int, C.POINTER(PhaseStats))
 _sig("nido_run_iteration_q", C.c_int, C.c_void_p, C.c_int, _pdbl)
 _sig("nido_run_phase_communities", C.c_int, C.c_void_p, C.c_int, _pi64)
+_sig("nido_autotune", C.c_int, C.c_void_p, C.c_int, C.POINTER(Tuning))
 _sig("nido_batch_create", C.c_int, C.c_int, _pi64, _pi64, _pi64, _pdbl, C.c_int, C.c_void_p,
      C.POINTER(C.c_void_p))
 _sig("nido_batch_run", C.c_int, C.c_void_p, _i64, C.c_double, C.c_double, C.POINTER(RunStats))
@@ -317,6 +318,13 @@ class GraphGPU:
         out = np.zeros(self.nbatches + 1, dtype=np.int64)
         _check(lib.nido_graph_batch_bounds(self._h, _p64(out)), "nido_graph_batch_bounds")
         return out
+
+    def autotune(self, iterations: int = 2) -> dict:
+        """Times candidate launch knobs on the first iterations of the phase
+        (nido_autotune), applies the fastest and returns it."""
+        t = Tuning()
+        _check(lib.nido_autotune(self._h, iterations, C.byref(t)), "nido_autotune")
+        return t.as_dict()
 
     def get_tuning(self) -> dict:
         t = Tuning()
@@ -519,6 +527,20 @@ class BatchGPU:
         st = RunStats()
         _check(lib.nido_batch_graph_stats(self._h, i, C.byref(st)), "nido_batch_graph_stats")
         return st
+
+    def autotune(self, iterations: int = 2) -> dict:
+        t = Tuning()
+        _check(lib.nido_autotune(self._h, iterations, C.byref(t)), "nido_autotune")
+        return t.as_dict()
+
+    def set_tuning(self, **kw) -> None:
+        t = Tuning()
+        _check(lib.nido_graph_get_tuning(self._h, C.byref(t)), "nido_graph_get_tuning")
+        for k, v in kw.items():
+            if k not in t.as_dict():
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        _check(lib.nido_graph_set_tuning(self._h, C.byref(t)), "nido_graph_set_tuning")
 
     def graph_phase(self, i: int, phase: int) -> PhaseStats:
         ps = PhaseStats()

@@ -318,6 +318,7 @@ struct nido_graph {
     SegItem* d_seg = nullptr;
     int64_t nseg = 0;
     std::vector<int64_t> blist0, blistn; // per batch: its slice of d_binv (non-isolated vertices)
+    hipGraphExec_t iter_exec = nullptr;  // one iteration's launches, captured once per phase
     std::vector<BatchResult> bres;       // per graph, from the last batched run
 };
 
@@ -395,6 +396,8 @@ int upload(T** d, const std::vector<T>& h, hipStream_t s)
     return NIDO_OK;
 }
 
+void drop_iteration_graph(nido_graph* g);
+
 // Per-phase set-up: maximum order, batches, degree bins, the row slices of
 // the degree/modularity sweeps, k_v and 2m (GraphGPU ctor steps
 // graph_gpu.cpp:94-139 and aggregation steps graph_gpu.cpp:1487-1516).
@@ -402,6 +405,7 @@ int setup_phase(nido_graph* g)
 {
     const int64_t nv = g->nv;
     const auto& off = g->h_off;
+    drop_iteration_graph(g);
     g->maxdeg = 0;
     for (int64_t v = 0; v < nv; ++v) g->maxdeg = std::max(g->maxdeg, off[v + 1] - off[v]);
     g->committed.assign((size_t)g->nb, 0);
@@ -769,14 +773,68 @@ int do_restore(nido_graph* g)
 
 // The loop body of LouvainGPU::run, louvain_gpu.cpp:55-90; *qt receives
 // the modularity evaluated after the batches (before any rollback).
-int do_iteration(nido_graph* g, double* q, double* dq, double* qt_out = nullptr)
+// The launches of one iteration: every batch's local moving and commit,
+// then the modularity kernels (the value lands in d_q, or d_gq per graph).
+int iteration_launches(nido_graph* g)
 {
     for (int b = 0; b < g->nb; ++b) {
         NIDO_TRY(do_lm(g, b));
         NIDO_TRY(do_commit(g, b));
     }
+    ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
+    if (g->G > 1)
+        launch_modularity_seg(g->d_seg, g->nseg, g->d_src, g->d_col32, g->d_w, g->d_C32, g->d_a, g->d_twomg,
+                              g->G, g->d_gacc, g->d_gq, g->stream);
+    else if (g->twom != 0)
+        launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d, g->d_part,
+                          g->d_q, g->stream);
+    HIP_TRY(hipGetLastError());
+    return NIDO_OK;
+}
+
+void drop_iteration_graph(nido_graph* g)
+{
+    if (g->iter_exec) (void)hipGraphExecDestroy(g->iter_exec);
+    g->iter_exec = nullptr;
+}
+
+// With use_graph set (and no per-kernel events being recorded) the
+// iteration's launches are captured into a hipGraph once per phase and
+// replayed; the kernels' arguments do not change within a phase.
+int run_iteration_launches(nido_graph* g)
+{
+    const bool use = g->tuning.use_graph && !g->prof.on && (g->G > 1 || g->twom != 0);
+    if (!use) return iteration_launches(g);
+    if (!g->iter_exec) {
+        HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = iteration_launches(g);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+        if (rc != NIDO_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIP_TRY(e);
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ei);
+        g->iter_exec = ex;
+    }
+    HIP_TRY(hipGraphLaunch(g->iter_exec, g->stream));
+    return NIDO_OK;
+}
+
+int do_iteration(nido_graph* g, double* q, double* dq, double* qt_out = nullptr)
+{
+    NIDO_TRY(run_iteration_launches(g));
     double qt = 0;
-    NIDO_TRY(do_modularity(g, &qt));
+    if (g->twom != 0) {
+        HIP_TRY(hipMemcpyAsync(g->h_q, g->d_q, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        g->prof.harvest();
+        qt = *g->h_q;
+    }
     if (qt_out) *qt_out = qt;
     *dq = qt - *q;
     if (*dq < 0) NIDO_TRY(do_restore(g));
@@ -1030,11 +1088,11 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
         int64_t iters = 0;
         while (any(act)) {
             NIDO_TRY(upload_flags(g, g->d_act, act));
-            for (int b = 0; b < g->nb; ++b) {
-                NIDO_TRY(do_lm(g, b));
-                NIDO_TRY(do_commit(g, b));
-            }
-            NIDO_TRY(batch_modularity(g, qt));
+            NIDO_TRY(run_iteration_launches(g));
+            qt.resize((size_t)G);
+            HIP_TRY(hipMemcpyAsync(qt.data(), g->d_gq, sizeof(double) * (size_t)G, hipMemcpyDeviceToHost, g->stream));
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            g->prof.harvest();
             bool rollback = false;
             for (int i = 0; i < G; ++i) {
                 flag[(size_t)i] = 0;
@@ -1153,6 +1211,70 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
     }
     g->last = S;
     if (st) *st = S;
+    return NIDO_OK;
+}
+
+// ---- autotuner ----------------------------------------------------------
+// Picks the launch knobs (degree-bin bounds, long-row partition size and
+// table, stream overlap, graph replay) by timing the first iterations of the
+// current phase from the singleton partition: one knob at a time, keeping a
+// value only when it is at least 2% faster.  Results never depend on the
+// knobs, so any choice is safe; the handle is left in the singleton
+// partition of its current phase.
+double tune_trial(nido_graph* g, const nido_tuning& t, int iters)
+{
+    g->tuning = t;
+    if (setup_phase(g) != NIDO_OK || do_singleton(g) != NIDO_OK) return -1.0;
+    if (g->G > 1) {
+        std::vector<unsigned char> act((size_t)g->G);
+        for (int i = 0; i < g->G; ++i) act[(size_t)i] = g->twom_g[(size_t)i] != 0;
+        if (upload_flags(g, g->d_act, act) != NIDO_OK) return -1.0;
+    }
+    // one untimed pass (first launches, graph capture), then the timed ones
+    if (run_iteration_launches(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
+    if (do_singleton(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
+    const double t0 = now_s();
+    for (int i = 0; i < iters; ++i)
+        if (run_iteration_launches(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
+    return now_s() - t0;
+}
+
+int do_autotune(nido_graph* g, int iters, nido_tuning* out)
+{
+    nido_tuning best = g->tuning;
+    const bool prof = g->prof.on;
+    g->prof.on = false;
+    double tb = tune_trial(g, best, iters);
+    if (tb < 0) {
+        g->prof.on = prof;
+        return set_error(NIDO_ERR_HIP, "autotune trial failed: %s", nido_last_error());
+    }
+    auto try_value = [&](auto setter) {
+        nido_tuning t = best;
+        setter(t);
+        if (std::memcmp(&t, &best, sizeof t) == 0) return;
+        const double x = tune_trial(g, t, iters);
+        if (x >= 0 && x < 0.98 * tb) {
+            best = t;
+            tb = x;
+        }
+    };
+    for (int v : {4, 6, 8}) try_value([&](nido_tuning& t) { t.thread_max_deg = v; });
+    for (int v : {128, 256, 512}) try_value([&](nido_tuning& t) { t.big_min_deg = v; });
+    for (int v : {768, 1536, 3072}) try_value([&](nido_tuning& t) { t.big_part_arcs = v; });
+    for (int v : {2048, 4096})
+        try_value([&](nido_tuning& t) {
+            t.big_slots = v;
+            t.big_key_limit = v / 2;
+        });
+    for (int v : {0, 1}) try_value([&](nido_tuning& t) { t.overlap_long = v; });
+    for (int v : {0, 1}) try_value([&](nido_tuning& t) { t.use_graph = v; });
+    g->prof.on = prof;
+    g->tuning = best;
+    NIDO_TRY(setup_phase(g));
+    NIDO_TRY(do_singleton(g));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (out) *out = best;
     return NIDO_OK;
 }
 
@@ -1317,6 +1439,7 @@ int nido_graph_destroy(nido_graph* g)
     if (!g) return NIDO_OK;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     AllocScope scope(g->stream);
+    drop_iteration_graph(g);
     free_graph_arrays(g);
     dev_free(g->d_C); dev_free(g->d_Cn); dev_free(g->d_k); dev_free(g->d_a); dev_free(g->d_kd);
     dev_free(g->d_clus); dev_free(g->d_work); dev_free(g->d_acc); dev_free(g->d_q); dev_free(g->d_tot);
@@ -1567,6 +1690,14 @@ int nido_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phas
                   return NIDO_OK;)
     }
     API_GUARD(return do_batch_run(g, max_loops, tol, tol_phase, stats);)
+}
+
+int nido_autotune(nido_graph* g, int iterations, nido_tuning* chosen)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (iterations < 1) return set_error(NIDO_ERR_INVALID, "iterations must be >= 1");
+    AllocScope scope(g->stream);
+    API_GUARD(return do_autotune(g, iterations, chosen);)
 }
 
 int nido_batch_size(const nido_graph* g, int* ngraphs)

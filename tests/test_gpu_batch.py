@@ -34,6 +34,8 @@ def _mixed_graphs(na):
 def _check(na, graphs, nb, **tuning):
     ref = [run_oracle(off, col, w, nb) for off, col, w in graphs]
     bg = na.BatchGPU(graphs, nb)
+    if tuning:
+        bg.set_tuning(**tuning)
     st = bg.run(500, 1e-6)
     assert st.phases == max(len(r["phases"]) for r in ref)
     for i, r in enumerate(ref):
@@ -56,6 +58,12 @@ def test_batch_equals_each_graph_alone(nb):
     _check(na, _mixed_graphs(na), nb)
 
 
+def test_batch_replayed_from_hip_graphs():
+    na = _na()
+    print("case: mixed batch nb=8 use_graph=1", flush=True)
+    _check(na, _mixed_graphs(na), 8, use_graph=1)
+
+
 def test_batch_of_one_graph_is_the_plain_run():
     na = _na()
     off, col, w = na.gen_sbm(2000, 64, 0.2, 0.003, True, 5).arrays()
@@ -73,3 +81,17 @@ def test_batch_many_small_graphs():
     print("case: 64 small sbms nb=32", flush=True)
     _check(na, graphs, 32)
 
+
+
+def test_batch_autotune_keeps_results():
+    na = _na()
+    graphs = _mixed_graphs(na)
+    print("case: mixed batch nb=4 autotuned", flush=True)
+    ref = [run_oracle(off, col, w, 4) for off, col, w in graphs]
+    bg = na.BatchGPU(graphs, 4)
+    bg.autotune(1)
+    bg.run(500, 1e-6)
+    for i, r in enumerate(ref):
+        assert abs(bg.graph_stats(i).q - r["q"]) <= Q_TOL, f"graph {i}"
+        assert np.array_equal(bg.clustering(i), r["clustering"]), f"graph {i}"
+    bg.close()

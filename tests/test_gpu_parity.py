@@ -127,6 +127,8 @@ KNOBS = [
     {"split_small": 1},                                             # one launch per short-row bin
     {"overlap_long": 0},                                            # long rows on the main stream
     {"overlap_long": 0, "split_small": 1},
+    {"use_graph": 1},                                               # iterations replayed from a hipGraph
+    {"use_graph": 1, "overlap_long": 1},
 ]
 
 
@@ -188,3 +190,17 @@ def test_tiny_graphs(case):
     ref = run_oracle(off, col, w, nb)
     assert_runs_equal(ref, got)
     assert abs(got["q"] - ref["q"]) <= Q_TOL
+
+
+def test_autotune_keeps_results():
+    na = _na()
+    off, col, w = na.gen_rmat(13, 16, 0.57, 0.19, 0.19, True, 9).arrays()
+    print("case: autotune rmat s13", flush=True)
+    ref = run_oracle(off, col, w, 8)
+    g = na.GraphGPU((off, col, w), 8)
+    chosen = g.autotune(1)
+    assert g.get_tuning() == chosen
+    st = na.LouvainGPU(500, 1e-6, 8).run(g, record=True)
+    assert st.phases == len(ref["phases"]) and st.q == ref["q"]
+    assert np.array_equal(g.clustering(), ref["clustering"])
+    g.close()
