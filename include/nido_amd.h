@@ -80,6 +80,7 @@ typedef struct {
     int32_t big_slots;        /* LDS table of a long-row workgroup: 2048 or 4096 slots */
     int32_t split_small;      /* launch every short-row bin as a kernel of its own (0/1; profiling) */
     int32_t overlap_long;     /* run the long rows of a batch on a second stream, beside the short rows (0/1) */
+    int32_t full_modularity;  /* 1: every iteration sweeps all arcs for Q; 0: exact incremental update (0/1) */
 } nido_tuning;
 
 const char* nido_last_error(void);
@@ -206,6 +207,8 @@ int nido_louvain_iteration(nido_graph* g, double* q, double* dq, double* q_eval)
 #define NIDO_K_MODULARITY  3   /* modularity sweep + finish, one pair per evaluation */
 #define NIDO_K_CLASSES     4
 int nido_graph_set_profiling(nido_graph* g, int enable);
+/* Events only around the classes whose bit (1 << class) is set in mask. */
+int nido_graph_set_profiling_classes(nido_graph* g, unsigned mask);
 /* Launch count, summed duration (ms) and algorithmic HBM bytes of one class
  * since profiling was enabled (24 B per arc + 40 B per vertex for local
  * moving, 24 B per arc + 24 B per vertex for modularity, 0 for commits). */

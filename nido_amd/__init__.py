@@ -62,7 +62,7 @@ class Tuning(C.Structure):
     _fields_ = [("thread_max_deg", C.c_int32), ("big_min_deg", C.c_int32),
                 ("big_part_arcs", C.c_int32), ("big_key_limit", C.c_int32),
                 ("use_graph", C.c_int32), ("big_slots", C.c_int32), ("split_small", C.c_int32),
-                ("overlap_long", C.c_int32)]
+                ("overlap_long", C.c_int32), ("full_modularity", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -114,6 +114,7 @@ _sig("nido_get_clustering", C.c_int, C.c_void_p, _pi64)
 _sig("nido_dump_partition", C.c_int, C.c_void_p, C.c_char_p)
 _sig("nido_louvain_iteration", C.c_int, C.c_void_p, _pdbl, _pdbl, _pdbl)
 _sig("nido_graph_set_profiling", C.c_int, C.c_void_p, C.c_int)
+_sig("nido_graph_set_profiling_classes", C.c_int, C.c_void_p, C.c_uint)
 _sig("nido_graph_kernel_stats", C.c_int, C.c_void_p, C.c_int, _pi64, _pdbl, _pdbl)
 _sig("nido_louvain_run", C.c_int, C.c_void_p, _i64, C.c_double, C.c_double, C.c_int, C.c_int,
      C.POINTER(RunStats))
@@ -379,8 +380,16 @@ class GraphGPU:
                "nido_louvain_iteration")
         return qq.value, dq.value, qe.value
 
-    def set_profiling(self, on: bool) -> None:
-        _check(lib.nido_graph_set_profiling(self._h, int(on)), "nido_graph_set_profiling")
+    def set_profiling(self, on: bool, classes=None) -> None:
+        """classes: names from KERNEL_CLASSES to bracket with events (all by default)."""
+        if classes is None:
+            _check(lib.nido_graph_set_profiling(self._h, int(on)), "nido_graph_set_profiling")
+        else:
+            mask = 0
+            for c in classes:
+                mask |= 1 << KERNEL_CLASSES.index(c)
+            _check(lib.nido_graph_set_profiling_classes(self._h, mask if on else 0),
+                   "nido_graph_set_profiling_classes")
 
     def kernel_stats(self) -> dict:
         out = {}

@@ -149,6 +149,7 @@ struct Prof {
         double bytes;
     };
     bool on = false;
+    unsigned mask = ~0u;               // kernel classes that get events
     std::vector<hipEvent_t> pool;
     std::vector<Entry> pending;
     int64_t launches[NIDO_K_CLASSES] = {};
@@ -209,7 +210,7 @@ struct ProfScope {
     hipEvent_t a = nullptr;
     ProfScope(Prof& p, hipStream_t st, int c, double b) : P(p), s(st), cls(c), bytes(b)
     {
-        if (P.on && (a = P.get())) (void)hipEventRecord(a, s);
+        if (P.on && ((P.mask >> c) & 1u) && (a = P.get())) (void)hipEventRecord(a, s);
     }
     ~ProfScope()
     {
@@ -318,7 +319,17 @@ struct nido_graph {
     SegItem* d_seg = nullptr;
     int64_t nseg = 0;
     std::vector<int64_t> blist0, blistn; // per batch: its slice of d_binv (non-isolated vertices)
-    hipGraphExec_t iter_exec = nullptr;  // one iteration's launches, captured once per phase
+    hipGraphExec_t iter_exec[2] = {};    // one iteration's launches (full / incremental Q), per phase
+    // incremental modularity (one graph)
+    int* d_dcounts = nullptr;            // 2 per batch
+    RowItem* d_dchunks = nullptr;
+    int64_t dchunk_cap = 0;
+    int* d_dcomms = nullptr;             // capacity n0
+    int* d_touched = nullptr;            // capacity n0, kept zero between batches
+    fx_t* d_term = nullptr;              // capacity n0
+    fx_t* d_iacc = nullptr;              // 2
+    bool inc_ok = false;                 // d_term / d_iacc describe the current partition
+    ModDelta delta() const { return ModDelta{d_dcounts, d_dchunks, d_dcomms, d_touched, d_term, d_iacc}; }
     std::vector<BatchResult> bres;       // per graph, from the last batched run
 };
 
@@ -344,6 +355,7 @@ void default_tuning(nido_tuning& t)
     t.big_slots = kBigSlots;
     t.split_small = 0;
     t.overlap_long = 0;
+    t.full_modularity = 0;
 }
 
 // partition_graph_edge_batch with part_on_batch=1 on one device,
@@ -406,6 +418,7 @@ int setup_phase(nido_graph* g)
     const int64_t nv = g->nv;
     const auto& off = g->h_off;
     drop_iteration_graph(g);
+    g->inc_ok = false;
     g->maxdeg = 0;
     for (int64_t v = 0; v < nv; ++v) g->maxdeg = std::max(g->maxdeg, off[v + 1] - off[v]);
     g->committed.assign((size_t)g->nb, 0);
@@ -626,6 +639,26 @@ int setup_phase(nido_graph* g)
     const double maxk = nido_bitsd(maxk_bits);
     const int narrow = g->nonneg && maxk < 2147483648.0 - 256.0;
     for (auto& B : g->big_args) B.narrow = narrow;
+    if (G == 1) {
+        // incremental modularity: room for every row chunk of the largest batch
+        int64_t need = 0;
+        for (int b = 0; b < g->nb; ++b) {
+            int64_t n = 0;
+            for (int64_t v = g->parts[(size_t)b]; v < g->parts[(size_t)b + 1]; ++v)
+                n += (off[v + 1] - off[v] + kDeltaArcs - 1) / kDeltaArcs;
+            need = std::max(need, n);
+        }
+        if (need > g->dchunk_cap) {
+            dev_free(g->d_dchunks);
+            NIDO_TRY(dev_alloc(&g->d_dchunks, need));
+            g->dchunk_cap = need;
+        }
+        dev_free(g->d_dcounts);
+        NIDO_TRY(dev_alloc(&g->d_dcounts, 2 * (int64_t)g->nb));
+        launch_fill_i32(g->d_dcounts, 0, 2 * (int64_t)g->nb, g->stream);
+        launch_fill_i32(g->d_touched, 0, g->n0, g->stream);
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
     if (G > 1) {
         // per-graph modularity items: arc and vertex chunks of each graph
         std::vector<SegItem> seg;
@@ -669,6 +702,7 @@ void free_graph_arrays(nido_graph* g)
 // ---- the GraphGPU method set ---------------------------------------------
 int do_singleton(nido_graph* g)
 {
+    g->inc_ok = false;
     launch_singleton(g->d_C, g->d_Cn, g->d_C32, g->d_a, g->d_k, g->nv, g->stream);
     HIP_TRY(hipGetLastError());
     std::fill(g->committed.begin(), g->committed.end(), 0);
@@ -684,8 +718,10 @@ int do_modularity(nido_graph* g, double* q)
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
         launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d,
-                          g->d_part, g->d_q, g->stream);
+                          g->d_part, g->d_q, g->stream, g->G == 1 ? g->d_term : nullptr,
+                          g->G == 1 ? g->d_iacc : nullptr);
     }
+    g->inc_ok = g->G == 1;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(g->h_q, g->d_q, sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
@@ -750,6 +786,7 @@ int do_lm(nido_graph* g, int b)
 
 int do_commit(nido_graph* g, int b)
 {
+    g->inc_ok = false;
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
         if (g->G == 1)
@@ -766,6 +803,7 @@ int do_commit(nido_graph* g, int b)
 
 int do_restore(nido_graph* g)
 {
+    g->inc_ok = false;
     launch_commit(0, g->nv, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->stream);
     HIP_TRY(hipGetLastError());
     return NIDO_OK;
@@ -775,39 +813,70 @@ int do_restore(nido_graph* g)
 // the modularity evaluated after the batches (before any rollback).
 // The launches of one iteration: every batch's local moving and commit,
 // then the modularity kernels (the value lands in d_q, or d_gq per graph).
-int iteration_launches(nido_graph* g)
+// inc: Q by the exact incremental update (tracked commits + per-batch
+// deltas) instead of a sweep over every arc.
+int iteration_launches(nido_graph* g, bool inc)
 {
+    const ModDelta D = g->delta();
     for (int b = 0; b < g->nb; ++b) {
         NIDO_TRY(do_lm(g, b));
-        NIDO_TRY(do_commit(g, b));
+        if (!inc) {
+            NIDO_TRY(do_commit(g, b));
+            continue;
+        }
+        const int64_t v0 = g->parts[(size_t)b], v1 = g->parts[(size_t)b + 1];
+        {
+            ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
+            launch_commit_tracked(v0, v1, b, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->d_off, D, g->stream);
+        }
+        {
+            ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 0.0);
+            launch_mod_delta(b, v0, v1, g->d_Cn, g->d_C32, g->d_col32, g->d_w, g->d_a, g->twom_d, D, g->stream);
+        }
+        g->committed[(size_t)b] = 1;
     }
     ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
     if (g->G > 1)
         launch_modularity_seg(g->d_seg, g->nseg, g->d_src, g->d_col32, g->d_w, g->d_C32, g->d_a, g->d_twomg,
                               g->G, g->d_gacc, g->d_gq, g->stream);
+    else if (g->twom != 0 && inc)
+        launch_mod_finish(D, g->nb, g->twom_d, g->d_q, g->stream);
     else if (g->twom != 0)
         launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d, g->d_part,
-                          g->d_q, g->stream);
+                          g->d_q, g->stream, g->d_term, g->d_iacc);
     HIP_TRY(hipGetLastError());
     return NIDO_OK;
 }
 
 void drop_iteration_graph(nido_graph* g)
 {
-    if (g->iter_exec) (void)hipGraphExecDestroy(g->iter_exec);
-    g->iter_exec = nullptr;
+    for (auto& ex : g->iter_exec) {
+        if (ex) (void)hipGraphExecDestroy(ex);
+        ex = nullptr;
+    }
 }
+
+int launch_or_replay(nido_graph* g, bool inc);
 
 // With use_graph set (and no per-kernel events being recorded) the
 // iteration's launches are captured into a hipGraph once per phase and
 // replayed; the kernels' arguments do not change within a phase.
 int run_iteration_launches(nido_graph* g)
 {
+    const bool inc = g->G == 1 && g->twom != 0 && g->inc_ok && !g->tuning.full_modularity;
+    NIDO_TRY(launch_or_replay(g, inc));
+    g->inc_ok = g->G == 1 && g->twom != 0;   // both forms leave term/acc describing the new partition
+    return NIDO_OK;
+}
+
+int launch_or_replay(nido_graph* g, bool inc)
+{
     const bool use = g->tuning.use_graph && !g->prof.on && (g->G > 1 || g->twom != 0);
-    if (!use) return iteration_launches(g);
-    if (!g->iter_exec) {
+    if (!use) return iteration_launches(g, inc);
+    hipGraphExec_t& exec = g->iter_exec[inc ? 1 : 0];
+    if (!exec) {
         HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = iteration_launches(g);
+        const int rc = iteration_launches(g, inc);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(g->stream, &graph);
         if (rc != NIDO_OK) {
@@ -819,9 +888,9 @@ int run_iteration_launches(nido_graph* g)
         const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIP_TRY(ei);
-        g->iter_exec = ex;
+        exec = ex;
     }
-    HIP_TRY(hipGraphLaunch(g->iter_exec, g->stream));
+    HIP_TRY(hipGraphLaunch(exec, g->stream));
     return NIDO_OK;
 }
 
@@ -1340,6 +1409,8 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         (rc = dev_alloc(&g->d_acc, 4)) || (rc = dev_alloc(&g->d_q, 1)) ||
         (rc = dev_alloc(&g->d_tot, 2)) || (rc = dev_alloc(&g->d_src, ne)) ||
         (rc = dev_alloc(&g->d_col32, ne)) || (rc = dev_alloc(&g->d_C32, nv)) ||
+        (rc = dev_alloc(&g->d_dcomms, nv)) || (rc = dev_alloc(&g->d_touched, nv)) ||
+        (rc = dev_alloc(&g->d_term, nv)) || (rc = dev_alloc(&g->d_iacc, 2)) ||
         (rc = dev_alloc(&g->d_part, 2 * (int64_t)kPartBlocks)))
         return fail(rc);
     if ((rc = raw_alloc(&g->d_scan_tmp, scan_temp_bytes(nv + 1)))) return fail(rc);
@@ -1447,6 +1518,8 @@ int nido_graph_destroy(nido_graph* g)
     dev_free(g->d_gid); dev_free(g->d_gid0); dev_free(g->d_vb); dev_free(g->d_vbn); dev_free(g->d_mg);
     dev_free(g->d_twomg); dev_free(g->d_act); dev_free(g->d_keep); dev_free(g->d_gacc); dev_free(g->d_gq);
     dev_free(g->d_seg);
+    dev_free(g->d_dcounts); dev_free(g->d_dchunks); dev_free(g->d_dcomms); dev_free(g->d_touched);
+    dev_free(g->d_term); dev_free(g->d_iacc);
     dev_free(g->d_binv); dev_free(g->d_big_items); dev_free(g->d_big_poff); dev_free(g->d_big_toff);
     dev_free(g->d_key_chunks); dev_free(g->d_bkey); dev_free(g->d_bw); dev_free(g->d_cpo);
     dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt);
@@ -1627,6 +1700,18 @@ int nido_graph_set_profiling(nido_graph* g, int enable)
     API_GUARD(g->prof.harvest();)
     g->prof.reset();
     g->prof.on = enable != 0;
+    g->prof.mask = ~0u;
+    return NIDO_OK;
+}
+
+int nido_graph_set_profiling_classes(nido_graph* g, unsigned mask)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    API_GUARD(g->prof.harvest();)
+    g->prof.reset();
+    g->prof.on = mask != 0;
+    g->prof.mask = mask;
     return NIDO_OK;
 }
 

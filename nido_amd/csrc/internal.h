@@ -146,10 +146,33 @@ void launch_commit_filtered(const int* list, int64_t v0, int64_t n, int64_t* C, 
                             const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s);
 
 // Modularity over the arc-parallel CSR (src = row ids): writes Q to *q;
-// part holds 2 * kPartBlocks partials.
+// part holds 2 * kPartBlocks partials.  With term/acc set it also leaves
+// every community's square term in term[] and the exact internal weight and
+// sum of squares in acc[0], acc[1] (the state the incremental form updates).
 void launch_modularity(const int* src, const int* col, const double* w, const int* C,
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
-                       hipStream_t s);
+                       hipStream_t s, fx_t* term = nullptr, fx_t* acc = nullptr);
+
+// Incremental modularity (one graph).  The commit of batch b lists the rows
+// of its moved vertices (in chunks of kDeltaArcs arcs) and the communities
+// whose total changed; the delta kernel then adds the exact change of the
+// internal weight and of the sum of square terms to acc, so Q after the last
+// batch equals a full sweep bit for bit.
+constexpr int64_t kDeltaArcs = 1024;
+struct ModDelta {
+    int* counts;             // per batch: [2b] row chunks, [2b+1] communities
+    RowItem* chunks;         // row chunks of moved vertices (current batch)
+    int* comms;              // communities touched (current batch)
+    int* touched;            // per community: listed in the current batch
+    fx_t* term;              // per community: fx((a_c / 2m)^2)
+    fx_t* acc;               // [0] internal weight, [1] sum of square terms
+};
+void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* Cn, int* C32, const fx_t* k,
+                           fx_t* a, const int64_t* off, const ModDelta& D, hipStream_t s);
+void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const int* C32, const int* col,
+                      const double* w, const fx_t* a, double twom_d, const ModDelta& D, hipStream_t s);
+// q = I / 2m - T from acc; zeroes the per-batch counts for the next iteration.
+void launch_mod_finish(const ModDelta& D, int nb, double twom_d, double* q, hipStream_t s);
 
 // ---- several graphs in one CSR ----------------------------------------------
 // Arcs [e0, e1) and vertices [v0, v1) of graph g.

@@ -152,7 +152,8 @@ constexpr int kModUnroll = 8;
 __global__ void __launch_bounds__(kBlock)
 mod_kernel(const int* __restrict__ src, const int* __restrict__ col,
            const double* __restrict__ w, const int* __restrict__ C, int64_t ne,
-           const fx_t* __restrict__ a, int64_t nv, double twom_d, fx_t* __restrict__ part)
+           const fx_t* __restrict__ a, int64_t nv, double twom_d, fx_t* __restrict__ part,
+           fx_t* __restrict__ term)
 {
     __shared__ fx_t red[kBlock / 64];
     fx_t I = 0, T = 0;
@@ -179,7 +180,9 @@ mod_kernel(const int* __restrict__ src, const int* __restrict__ col,
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nv;
          i += (int64_t)gridDim.x * kBlock) {
         const double x = fx_to_double(a[i]) / twom_d;
-        T += fx_from_double(x * x);
+        const fx_t t = fx_from_double(x * x);
+        T += t;
+        if (term) term[i] = t;
     }
     I = block_sum_fx(I, red);
     T = block_sum_fx(T, red);
@@ -190,7 +193,7 @@ mod_kernel(const int* __restrict__ src, const int* __restrict__ col,
 }
 
 __global__ void __launch_bounds__(kBlock)
-mod_finalize_kernel(const fx_t* __restrict__ part, int nblocks, double twom_d, double* q)
+mod_finalize_kernel(const fx_t* __restrict__ part, int nblocks, double twom_d, double* q, fx_t* acc)
 {
     __shared__ fx_t red[kBlock / 64];
     fx_t I = 0, T = 0;
@@ -200,7 +203,71 @@ mod_finalize_kernel(const fx_t* __restrict__ part, int nblocks, double twom_d, d
     }
     I = block_sum_fx(I, red);
     T = block_sum_fx(T, red);
-    if (threadIdx.x == 0) *q = fx_to_double(I) / twom_d - fx_to_double(T);
+    if (threadIdx.x == 0) {
+        *q = fx_to_double(I) / twom_d - fx_to_double(T);
+        if (acc) {
+            acc[0] = I;
+            acc[1] = T;
+        }
+    }
+}
+
+// The exact change of the internal weight and of the square terms caused by
+// batch [v0, v1)'s moves (after its commit).  For a moved vertex v and arc
+// (v, u): old = [C'(v) == old(u)], new = [C(v) == C(u)] where old(u) is C'(u)
+// inside the batch (the commit left every batch vertex's previous community
+// there) and C(u) outside it; the arc (u, v) changes the same way, so the
+// difference counts twice unless u moved too (then u's row counts it).
+__global__ void __launch_bounds__(kBlock)
+mod_delta_kernel(int b, int64_t v0, int64_t v1, const int64_t* __restrict__ Cn, const int* __restrict__ C32,
+                 const int* __restrict__ col, const double* __restrict__ w, const fx_t* __restrict__ a,
+                 double twom_d, ModDelta D)
+{
+    __shared__ fx_t red[kBlock / 64];
+    const int nch = D.counts[2 * b], ncm = D.counts[2 * b + 1];
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
+    const int64_t nwv = ((int64_t)gridDim.x * kBlock) >> 6;
+    fx_t dI = 0, dT = 0;
+    for (int64_t i = wid; i < nch; i += nwv) {
+        const RowItem ch = D.chunks[i];
+        const int v = (int)ch.v;
+        const int cvn = C32[v], cvo = (int)Cn[v];
+        for (int64_t e = ch.e0 + lane; e < ch.e1; e += 64) {
+            const int u = col[e];
+            const int cun = C32[u];
+            const bool inb = u >= v0 && u < v1;
+            const int cuo = inb ? (int)Cn[u] : cun;
+            const int dlt = (int)(cvn == cun) - (int)(cvo == cuo);
+            if (dlt) {
+                fx_t f = fx_from_double(w[e]);
+                if (!(inb && cuo != cun)) f += f;
+                dI += dlt > 0 ? f : -f;
+            }
+        }
+    }
+    for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < ncm; j += (int64_t)gridDim.x * kBlock) {
+        const int c = D.comms[j];
+        const double x = fx_to_double(a[c]) / twom_d;
+        const fx_t nt = fx_from_double(x * x);
+        dT += nt - D.term[c];
+        D.term[c] = nt;
+        D.touched[c] = 0;
+    }
+    dI = block_sum_fx(dI, red);
+    dT = block_sum_fx(dT, red);
+    if (threadIdx.x == 0 && (dI != 0 || dT != 0)) {
+        acc_add(D.acc, dI);
+        acc_add(D.acc + 1, dT);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+mod_finish_kernel(ModDelta D, int nb, double twom_d, double* q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *q = fx_to_double(D.acc[0]) / twom_d - fx_to_double(D.acc[1]);
+    for (int i = threadIdx.x; i < 2 * nb; i += kBlock) D.counts[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -534,13 +601,24 @@ void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s)
 
 void launch_modularity(const int* src, const int* col, const double* w, const int* C,
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
-                       hipStream_t s)
+                       hipStream_t s, fx_t* term, fx_t* acc)
 {
     const int64_t tiles = (ne + (int64_t)kBlock * kModUnroll - 1) / ((int64_t)kBlock * kModUnroll);
     const int64_t need = tiles > (nv + kBlock - 1) / kBlock ? tiles : (nv + kBlock - 1) / kBlock;
     const unsigned g = grid_for(need, 1, kPartBlocks);
-    mod_kernel<<<g, kBlock, 0, s>>>(src, col, w, C, ne, a, nv, twom_d, part);
-    mod_finalize_kernel<<<1, kBlock, 0, s>>>(part, (int)g, twom_d, q);
+    mod_kernel<<<g, kBlock, 0, s>>>(src, col, w, C, ne, a, nv, twom_d, part, term);
+    mod_finalize_kernel<<<1, kBlock, 0, s>>>(part, (int)g, twom_d, q, acc);
+}
+
+void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const int* C32, const int* col,
+                      const double* w, const fx_t* a, double twom_d, const ModDelta& D, hipStream_t s)
+{
+    mod_delta_kernel<<<512, kBlock, 0, s>>>(b, v0, v1, Cn, C32, col, w, a, twom_d, D);
+}
+
+void launch_mod_finish(const ModDelta& D, int nb, double twom_d, double* q, hipStream_t s)
+{
+    mod_finish_kernel<<<1, kBlock, 0, s>>>(D, nb, twom_d, q);
 }
 
 size_t scan_temp_bytes(int64_t n)

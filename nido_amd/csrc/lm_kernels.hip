@@ -717,6 +717,39 @@ commit_kernel(const int* __restrict__ list, int64_t v0, int64_t n, int64_t* __re
     }
 }
 
+// The commit plus the bookkeeping of the incremental modularity: a moved
+// vertex lists its row (in chunks) and its old and new community (each
+// community once per batch, through its touched flag).
+__global__ void __launch_bounds__(kBlock)
+commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, int64_t* __restrict__ Cn,
+                      int* __restrict__ C32, const fx_t* __restrict__ k, fx_t* __restrict__ a,
+                      const int64_t* __restrict__ off, ModDelta D)
+{
+    for (int64_t v = v0 + blockIdx.x * (int64_t)kBlock + threadIdx.x; v < v1; v += (int64_t)gridDim.x * kBlock) {
+        const int64_t src = C[v];
+        const int64_t dst = Cn[v];
+        if (src != dst) {
+            const fx_t kv = k[v];
+            acc_add(a + src, -kv);
+            acc_add(a + dst, kv);
+            C32[v] = (int)dst;
+            const int64_t e0 = off[v], e1 = off[v + 1];
+            if (e1 > e0) {
+                const int n = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
+                const int base = atomicAdd(&D.counts[2 * b], n);
+                for (int i = 0; i < n; ++i) {
+                    const int64_t s0 = e0 + (int64_t)i * kDeltaArcs;
+                    D.chunks[base + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
+                }
+            }
+            if (atomicExch(&D.touched[src], 1) == 0) D.comms[atomicAdd(&D.counts[2 * b + 1], 1)] = (int)src;
+            if (atomicExch(&D.touched[dst], 1) == 0) D.comms[atomicAdd(&D.counts[2 * b + 1], 1)] = (int)dst;
+        }
+        C[v] = dst;
+        Cn[v] = src;
+    }
+}
+
 }  // namespace
 
 // Rows per workgroup of each short-row bin.
@@ -784,6 +817,13 @@ void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, co
     if (v1 <= v0) return;
     commit_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(nullptr, v0, v1 - v0, C, Cn, C32, k, a,
                                                                nullptr, nullptr);
+}
+
+void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* Cn, int* C32, const fx_t* k,
+                           fx_t* a, const int64_t* off, const ModDelta& D, hipStream_t s)
+{
+    if (v1 <= v0) return;
+    commit_tracked_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(v0, v1, b, C, Cn, C32, k, a, off, D);
 }
 
 void launch_commit_filtered(const int* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,

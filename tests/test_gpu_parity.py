@@ -129,6 +129,8 @@ KNOBS = [
     {"overlap_long": 0, "split_small": 1},
     {"use_graph": 1},                                               # iterations replayed from a hipGraph
     {"use_graph": 1, "overlap_long": 1},
+    {"full_modularity": 1},                                         # every iteration sweeps all arcs for Q
+    {"full_modularity": 1, "use_graph": 1},
 ]
 
 
