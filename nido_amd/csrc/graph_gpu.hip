@@ -14,6 +14,7 @@
 
 #include "../../include/nido_amd.h"
 #include "errors.h"
+#include "host_check.h"
 #include "internal.h"
 
 using namespace nido;
@@ -329,6 +330,7 @@ struct nido_graph {
     fx_t* d_term = nullptr;              // capacity n0
     fx_t* d_iacc = nullptr;              // 2
     bool inc_ok = false;                 // d_term / d_iacc describe the current partition
+    int64_t iters_since_singleton = 0;   // the first iteration moves most vertices: full sweep there
     ModDelta delta() const { return ModDelta{d_dcounts, d_dchunks, d_dcomms, d_touched, d_term, d_iacc}; }
     std::vector<BatchResult> bres;       // per graph, from the last batched run
 };
@@ -568,6 +570,26 @@ int setup_phase(nido_graph* g)
         launch_fill_i32(g->d_cnt, 0, slot_need, g->stream);   // the kernels keep them zero afterwards
         g->slot_cap = slot_need;
     }
+    if (G == 1) {
+        // incremental modularity: room for every row chunk of the largest batch
+        int64_t need = 0;
+        for (int b = 0; b < g->nb; ++b) {
+            int64_t n = 0;
+            for (int64_t v = g->parts[(size_t)b]; v < g->parts[(size_t)b + 1]; ++v)
+                n += (off[v + 1] - off[v] + kDeltaArcs - 1) / kDeltaArcs;
+            need = std::max(need, n);
+        }
+        if (need > g->dchunk_cap) {
+            dev_free(g->d_dchunks);
+            NIDO_TRY(dev_alloc(&g->d_dchunks, need));
+            g->dchunk_cap = need;
+        }
+        dev_free(g->d_dcounts);
+        NIDO_TRY(dev_alloc(&g->d_dcounts, 2 * (int64_t)g->nb));
+        launch_fill_i32(g->d_dcounts, 0, 2 * (int64_t)g->nb, g->stream);
+        launch_fill_i32(g->d_touched, 0, g->n0, g->stream);
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
     const size_t nbig_items = bitems.size();
     g->small_bins.assign((size_t)g->nb, SmallBins{});
     g->big_args.assign((size_t)g->nb, BigArgs{});
@@ -582,6 +604,14 @@ int setup_phase(nido_graph* g)
         L.bkey = g->d_bkey;
         L.bw = g->d_bw;
         L.cpo = g->d_cpo;
+        if (G == 1 && b > 0) {   // the previous batch's modularity delta rides in this launch
+            L.dblocks = kDeltaBlocks;
+            L.db = b - 1;
+            L.dv0 = g->parts[(size_t)b - 1];
+            L.dv1 = g->parts[(size_t)b];
+            L.D = g->delta();
+            L.Cn64 = g->d_Cn;
+        }
         small_bins_blocks(L, 4096);
         BigArgs& B = g->big_args[(size_t)b];
         const int64_t i0 = item_base[(size_t)b];
@@ -639,26 +669,7 @@ int setup_phase(nido_graph* g)
     const double maxk = nido_bitsd(maxk_bits);
     const int narrow = g->nonneg && maxk < 2147483648.0 - 256.0;
     for (auto& B : g->big_args) B.narrow = narrow;
-    if (G == 1) {
-        // incremental modularity: room for every row chunk of the largest batch
-        int64_t need = 0;
-        for (int b = 0; b < g->nb; ++b) {
-            int64_t n = 0;
-            for (int64_t v = g->parts[(size_t)b]; v < g->parts[(size_t)b + 1]; ++v)
-                n += (off[v + 1] - off[v] + kDeltaArcs - 1) / kDeltaArcs;
-            need = std::max(need, n);
-        }
-        if (need > g->dchunk_cap) {
-            dev_free(g->d_dchunks);
-            NIDO_TRY(dev_alloc(&g->d_dchunks, need));
-            g->dchunk_cap = need;
-        }
-        dev_free(g->d_dcounts);
-        NIDO_TRY(dev_alloc(&g->d_dcounts, 2 * (int64_t)g->nb));
-        launch_fill_i32(g->d_dcounts, 0, 2 * (int64_t)g->nb, g->stream);
-        launch_fill_i32(g->d_touched, 0, g->n0, g->stream);
-        HIP_TRY(hipStreamSynchronize(g->stream));
-    }
+    for (auto& L : g->small_bins) L.twom_d = g->twom_d;
     if (G > 1) {
         // per-graph modularity items: arc and vertex chunks of each graph
         std::vector<SegItem> seg;
@@ -703,6 +714,7 @@ void free_graph_arrays(nido_graph* g)
 int do_singleton(nido_graph* g)
 {
     g->inc_ok = false;
+    g->iters_since_singleton = 0;
     launch_singleton(g->d_C, g->d_Cn, g->d_C32, g->d_a, g->d_k, g->nv, g->stream);
     HIP_TRY(hipGetLastError());
     std::fill(g->committed.begin(), g->committed.end(), 0);
@@ -736,10 +748,12 @@ int do_lm(nido_graph* g, int b)
     LmArgs A{g->d_off, g->d_col32, g->d_w, g->d_C32, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m};
     if (g->G == 1) {
         const int64_t v0 = g->parts[(size_t)b], v1 = g->parts[(size_t)b + 1];
-        if (v1 <= v0) return NIDO_OK;
+        // an empty batch still carries the previous batch's modularity delta
+        if (v1 <= v0 && g->small_bins[(size_t)b].dblocks == 0) return NIDO_OK;
         if (g->twom == 0) {
-            HIP_TRY(hipMemcpyAsync(g->d_Cn + v0, g->d_C + v0, sizeof(int64_t) * (size_t)(v1 - v0),
-                                   hipMemcpyDeviceToDevice, g->stream));
+            if (v1 > v0)
+                HIP_TRY(hipMemcpyAsync(g->d_Cn + v0, g->d_C + v0, sizeof(int64_t) * (size_t)(v1 - v0),
+                                       hipMemcpyDeviceToDevice, g->stream));
             return NIDO_OK;
         }
     } else {
@@ -829,13 +843,12 @@ int iteration_launches(nido_graph* g, bool inc)
             ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
             launch_commit_tracked(v0, v1, b, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->d_off, D, g->stream);
         }
-        {
-            ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 0.0);
-            launch_mod_delta(b, v0, v1, g->d_Cn, g->d_C32, g->d_col32, g->d_w, g->d_a, g->twom_d, D, g->stream);
-        }
         g->committed[(size_t)b] = 1;
     }
     ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
+    if (inc && g->nb > 0)   // the earlier batches' deltas rode in the next batch's short-row launch
+        launch_mod_delta(g->nb - 1, g->parts[(size_t)g->nb - 1], g->parts[(size_t)g->nb], g->d_Cn, g->d_C32,
+                         g->d_col32, g->d_w, g->d_a, g->twom_d, D, g->stream);
     if (g->G > 1)
         launch_modularity_seg(g->d_seg, g->nseg, g->d_src, g->d_col32, g->d_w, g->d_C32, g->d_a, g->d_twomg,
                               g->G, g->d_gacc, g->d_gq, g->stream);
@@ -863,7 +876,13 @@ int launch_or_replay(nido_graph* g, bool inc);
 // replayed; the kernels' arguments do not change within a phase.
 int run_iteration_launches(nido_graph* g)
 {
-    const bool inc = g->G == 1 && g->twom != 0 && g->inc_ok && !g->tuning.full_modularity;
+    // The first iteration from the singleton partition moves most vertices
+    // (their rows hold ~70% of the arcs on R-MAT), where one sweep of every
+    // arc is cheaper than listing the moved rows; later iterations move a
+    // few percent, and the exact incremental update takes over.
+    const bool inc = g->G == 1 && g->twom != 0 && g->inc_ok && !g->tuning.full_modularity &&
+                     g->iters_since_singleton > 0;
+    ++g->iters_since_singleton;
     NIDO_TRY(launch_or_replay(g, inc));
     g->inc_ok = g->G == 1 && g->twom != 0;   // both forms leave term/acc describing the new partition
     return NIDO_OK;
@@ -1356,18 +1375,11 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     if (nv >= (1ll << 31) - 1) return set_error(NIDO_ERR_CAPACITY, "nv must be below 2^31-1");
     if (nb < 1) return set_error(NIDO_ERR_INVALID, "nbatches must be >= 1");
     const int64_t base = off[0];
-    for (int64_t i = 0; i < nv; ++i)
-        if (off[i + 1] < off[i]) return set_error(NIDO_ERR_INVALID, "offsets decrease at %lld", (long long)i);
     const int64_t ne = off[nv] - base;
-    if (ne > 0 && (!col || !w)) return set_error(NIDO_ERR_INVALID, "null cols/weights");
     bool nonneg = true;
-    for (int64_t e = 0; e < ne; ++e) {
-        const int64_t c = col[base + e];
-        if (c < 0 || c >= nv) return set_error(NIDO_ERR_INVALID, "col[%lld]=%lld out of range", (long long)e, (long long)c);
-        const double x = w[base + e];
-        if (!(x == x) || x >= 0x1p62 || x <= -0x1p62)
-            return set_error(NIDO_ERR_INVALID, "weight[%lld] is not finite or |w| >= 2^62", (long long)e);
-        nonneg = nonneg && !(x < 0);
+    {
+        const int64_t whole[2] = {0, nv};
+        NIDO_TRY(check_csr(nv, off, col, w, G, G > 1 ? vptr : whole, &nonneg));
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -1435,12 +1447,9 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         g->twom_g.assign((size_t)G, 0);
         g->twom_dg.assign((size_t)G, 0.0);
         std::vector<double> mg((size_t)G), tw((size_t)G);
+        graph_twom(off, w, G, vptr, g->twom_g.data());   // 2m per graph, exactly (graph_cuda.cu:1855)
         for (int i = 0; i < G; ++i) {
-            fx_t sum = 0;   // 2m of graph i, exactly (graph_cuda.cu:1855 sums the same k_v)
-            for (int64_t e = g->h_off[(size_t)vptr[i]]; e < g->h_off[(size_t)vptr[i + 1]]; ++e)
-                sum += fx_from_double(w[base + e]);
-            g->twom_g[(size_t)i] = sum;
-            tw[(size_t)i] = g->twom_dg[(size_t)i] = fx_to_double(sum);
+            tw[(size_t)i] = g->twom_dg[(size_t)i] = fx_to_double(g->twom_g[(size_t)i]);
             mg[(size_t)i] = tw[(size_t)i] * 0.5;
         }
         if ((rc = dev_alloc(&g->d_gid, nv)) || (rc = dev_alloc(&g->d_gid0, nv)) ||
@@ -1746,15 +1755,8 @@ int nido_batch_create(int ngraphs, const int64_t* vptr, const int64_t* off, cons
         if (vptr[i + 1] < vptr[i]) return set_error(NIDO_ERR_INVALID, "vptr decreases at graph %d", i);
     const int64_t nv = vptr[ngraphs];
     if (nv >= (1ll << 31) - 1) return set_error(NIDO_ERR_CAPACITY, "total vertices must be below 2^31-1");
-    // graphs are disjoint: graph i's arcs stay inside [vptr[i], vptr[i+1])
-    for (int i = 0; i < ngraphs; ++i)
-        for (int64_t v = vptr[i]; v < vptr[i + 1]; ++v) {
-            if (off[v + 1] < off[v]) return set_error(NIDO_ERR_INVALID, "offsets decrease at %lld", (long long)v);
-            if (!col) continue;
-            for (int64_t e = off[v]; e < off[v + 1]; ++e)
-                if (col[e] < vptr[i] || col[e] >= vptr[i + 1])
-                    return set_error(NIDO_ERR_INVALID, "arc %lld of graph %d leaves the graph", (long long)e, i);
-        }
+    // graphs are disjoint (graph i's arcs stay inside [vptr[i], vptr[i+1])):
+    // checked with the rest of the CSR when the handle is created
     API_GUARD(return create_impl(nv, off, col, w, nbatches, stream, out, ngraphs == 1 ? 1 : ngraphs, vptr);)
 }
 

@@ -1,0 +1,15 @@
+// host_check.h -- host-side CSR validation and per-graph sums (OpenMP),
+// shared by the device handle's constructors.
+#pragma once
+#include <stdint.h>
+#include "fixed.h"
+
+namespace nido {
+// Offsets non-decreasing; every arc of graph g (vertices [vptr[g],
+// vptr[g+1])) points inside graph g; weights finite with |w| < 2^62.
+// off/col/w are indexed from off[0].  *nonneg: every weight >= 0.
+int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int G,
+              const int64_t* vptr, bool* nonneg);
+// Exact 2m of every graph: the Q64.64 sum of its arc weights.
+void graph_twom(const int64_t* off, const double* w, int G, const int64_t* vptr, fx_t* out);
+}  // namespace nido

@@ -12,6 +12,7 @@
 
 #include "../../include/nido_amd.h"
 #include "errors.h"
+#include "host_check.h"
 
 struct nido_hgraph {
     int64_t nv = 0, ne = 0;
@@ -411,3 +412,71 @@ int nido_gen_chung_lu(int64_t nv, double alpha, double dmin, double dmax, double
 }
 
 }  // extern "C"
+
+// ---- checks and sums used by the device handle's constructors ------------
+namespace nido {
+
+int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int G,
+              const int64_t* vptr, bool* nonneg)
+{
+    const int64_t base = off[0];
+    int64_t bad_off = INT64_MAX;
+#pragma omp parallel for reduction(min : bad_off) schedule(static)
+    for (int64_t i = 0; i < nv; ++i)
+        if (off[i + 1] < off[i] && i < bad_off) bad_off = i;
+    if (bad_off != INT64_MAX) return set_error(NIDO_ERR_INVALID, "offsets decrease at %lld", (long long)bad_off);
+    const int64_t ne = off[nv] - base;
+    if (ne > 0 && (!col || !w)) return set_error(NIDO_ERR_INVALID, "null cols/weights");
+    int64_t bad_col = INT64_MAX, bad_w = INT64_MAX;
+    int neg = 0;
+    for (int g = 0; g < G; ++g) {
+        const int64_t lo = vptr[g], hi = vptr[g + 1];
+        const int64_t e0 = off[lo] - base, e1 = off[hi] - base;
+#pragma omp parallel for reduction(min : bad_col, bad_w) reduction(| : neg) schedule(static)
+        for (int64_t e = e0; e < e1; ++e) {
+            const int64_t c = col[base + e];
+            if ((c < lo || c >= hi) && e < bad_col) bad_col = e;
+            const double x = w[base + e];
+            if (!(x == x) || x >= 0x1p62 || x <= -0x1p62) {
+                if (e < bad_w) bad_w = e;
+            }
+            neg |= x < 0;
+        }
+        if (bad_col != INT64_MAX && G > 1)
+            return set_error(NIDO_ERR_INVALID, "arc %lld (col %lld) leaves graph %d", (long long)bad_col,
+                             (long long)col[base + bad_col], g);
+        if (bad_col != INT64_MAX)
+            return set_error(NIDO_ERR_INVALID, "col[%lld]=%lld out of range", (long long)bad_col,
+                             (long long)col[base + bad_col]);
+        if (bad_w != INT64_MAX)
+            return set_error(NIDO_ERR_INVALID, "weight[%lld] is not finite or |w| >= 2^62", (long long)bad_w);
+    }
+    if (nonneg) *nonneg = !neg;
+    return NIDO_OK;
+}
+
+void graph_twom(const int64_t* off, const double* w, int G, const int64_t* vptr, fx_t* out)
+{
+    const int64_t base = off[0];
+    for (int g = 0; g < G; ++g) {
+        const int64_t e0 = off[vptr[g]] - base, e1 = off[vptr[g + 1]] - base;
+        uint64_t lo = 0;
+        int64_t hi = 0;
+        // exact: the 128-bit sum is accumulated as two 64-bit halves with carry
+#pragma omp parallel
+        {
+            fx_t s = 0;
+#pragma omp for schedule(static) nowait
+            for (int64_t e = e0; e < e1; ++e) s += fx_from_double(w[base + e]);
+#pragma omp critical
+            {
+                const fx_t t = fx_make(lo, hi) + s;
+                lo = fx_lo(t);
+                hi = fx_hi(t);
+            }
+        }
+        out[g] = fx_make(lo, hi);
+    }
+}
+
+}  // namespace nido

@@ -38,6 +38,28 @@ struct KeyChunk {
 constexpr int64_t kKeyChunkArcs = 2048;
 constexpr int kMaxParts = 1024;           // partitions per row, at most
 
+// A slice of one row for the degree / modularity sweeps over long rows.
+struct RowItem {
+    int64_t v;
+    int64_t e0, e1;
+};
+
+// Incremental modularity (one graph).  The commit of batch b lists the rows
+// of its moved vertices (in chunks of kDeltaArcs arcs) and the communities
+// whose total changed; the delta kernel then adds the exact change of the
+// internal weight and of the sum of square terms to acc, so Q after the last
+// batch equals a full sweep bit for bit.
+constexpr int64_t kDeltaArcs = 1024;
+struct ModDelta {
+    int* counts;             // per batch: [2b] row chunks, [2b+1] communities
+    RowItem* chunks;         // row chunks of moved vertices (current batch)
+    int* comms;              // communities touched (current batch)
+    int* touched;            // per community: listed in the current batch
+    fx_t* term;              // per community: fx((a_c / 2m)^2)
+    fx_t* acc;               // [0] internal weight, [1] sum of square terms
+};
+constexpr int kDeltaBlocks = 256;          // workgroups of a delta riding in a short-row launch
+
 // The first launch of a batch: the small bins plus the key fill of the
 // batch's partitioned long rows.
 struct SmallBins {
@@ -48,7 +70,14 @@ struct SmallBins {
     int* bkey;
     double* bw;
     int* cpo;
-    int blk0[NUM_SMALL_BINS + 2];          // first block of each bin, then of the key fill
+    // The incremental-modularity delta of the previous batch (db, vertices
+    // [dv0, dv1)) rides in this launch: dblocks workgroups, 0 if none.
+    int dblocks, db;
+    int64_t dv0, dv1;
+    ModDelta D;
+    const int64_t* Cn64;                   // C' as int64 (previous communities after a commit)
+    double twom_d;
+    int blk0[NUM_SMALL_BINS + 3];          // first block of each bin, of the key fill, of the delta; end
 };
 
 // One workgroup's share of a long row: partition p of P (P a power of 2);
@@ -101,12 +130,6 @@ struct LmArgs {
     const unsigned char* act;
 };
 
-// A slice of one row for the degree / modularity sweeps over long rows.
-struct RowItem {
-    int64_t v;
-    int64_t e0, e1;
-};
-
 // ---- launch helpers (kernels.hip) ----------------------------------------
 void launch_fill_i64(int64_t* p, int64_t val, int64_t n, hipStream_t s);
 void launch_fill_i32(int32_t* p, int32_t val, int64_t n, hipStream_t s);
@@ -153,20 +176,6 @@ void launch_modularity(const int* src, const int* col, const double* w, const in
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
                        hipStream_t s, fx_t* term = nullptr, fx_t* acc = nullptr);
 
-// Incremental modularity (one graph).  The commit of batch b lists the rows
-// of its moved vertices (in chunks of kDeltaArcs arcs) and the communities
-// whose total changed; the delta kernel then adds the exact change of the
-// internal weight and of the sum of square terms to acc, so Q after the last
-// batch equals a full sweep bit for bit.
-constexpr int64_t kDeltaArcs = 1024;
-struct ModDelta {
-    int* counts;             // per batch: [2b] row chunks, [2b+1] communities
-    RowItem* chunks;         // row chunks of moved vertices (current batch)
-    int* comms;              // communities touched (current batch)
-    int* touched;            // per community: listed in the current batch
-    fx_t* term;              // per community: fx((a_c / 2m)^2)
-    fx_t* acc;               // [0] internal weight, [1] sum of square terms
-};
 void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* Cn, int* C32, const fx_t* k,
                            fx_t* a, const int64_t* off, const ModDelta& D, hipStream_t s);
 void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const int* C32, const int* col,

@@ -17,20 +17,6 @@ __global__ void fill_kernel(T* __restrict__ p, T val, int64_t n)
 // ---------------------------------------------------------------------------
 // Vertex weights (graph_cuda.cu:302-346), singleton (graph_cuda.cu:410-429)
 // ---------------------------------------------------------------------------
-// Per-block partial sums of Q64.64 values: one global store per block and a
-// single-block finish, instead of same-address atomics from every wavefront.
-__device__ __forceinline__ fx_t block_sum_fx(fx_t x, fx_t* red)
-{
-    x = wave_sum_fx(x);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
-    __syncthreads();
-    fx_t t = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kBlock / 64; ++i) t += red[i];
-    __syncthreads();
-    return t;   // valid in thread 0
-}
-
 __global__ void __launch_bounds__(kBlock)
 vw_small_kernel(const int64_t* __restrict__ off, const double* __restrict__ w,
                 const int* __restrict__ small, int64_t n, fx_t* __restrict__ k)
@@ -212,54 +198,13 @@ mod_finalize_kernel(const fx_t* __restrict__ part, int nblocks, double twom_d, d
     }
 }
 
-// The exact change of the internal weight and of the square terms caused by
-// batch [v0, v1)'s moves (after its commit).  For a moved vertex v and arc
-// (v, u): old = [C'(v) == old(u)], new = [C(v) == C(u)] where old(u) is C'(u)
-// inside the batch (the commit left every batch vertex's previous community
-// there) and C(u) outside it; the arc (u, v) changes the same way, so the
-// difference counts twice unless u moved too (then u's row counts it).
 __global__ void __launch_bounds__(kBlock)
 mod_delta_kernel(int b, int64_t v0, int64_t v1, const int64_t* __restrict__ Cn, const int* __restrict__ C32,
                  const int* __restrict__ col, const double* __restrict__ w, const fx_t* __restrict__ a,
                  double twom_d, ModDelta D)
 {
     __shared__ fx_t red[kBlock / 64];
-    const int nch = D.counts[2 * b], ncm = D.counts[2 * b + 1];
-    const int lane = threadIdx.x & 63;
-    const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
-    const int64_t nwv = ((int64_t)gridDim.x * kBlock) >> 6;
-    fx_t dI = 0, dT = 0;
-    for (int64_t i = wid; i < nch; i += nwv) {
-        const RowItem ch = D.chunks[i];
-        const int v = (int)ch.v;
-        const int cvn = C32[v], cvo = (int)Cn[v];
-        for (int64_t e = ch.e0 + lane; e < ch.e1; e += 64) {
-            const int u = col[e];
-            const int cun = C32[u];
-            const bool inb = u >= v0 && u < v1;
-            const int cuo = inb ? (int)Cn[u] : cun;
-            const int dlt = (int)(cvn == cun) - (int)(cvo == cuo);
-            if (dlt) {
-                fx_t f = fx_from_double(w[e]);
-                if (!(inb && cuo != cun)) f += f;
-                dI += dlt > 0 ? f : -f;
-            }
-        }
-    }
-    for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < ncm; j += (int64_t)gridDim.x * kBlock) {
-        const int c = D.comms[j];
-        const double x = fx_to_double(a[c]) / twom_d;
-        const fx_t nt = fx_from_double(x * x);
-        dT += nt - D.term[c];
-        D.term[c] = nt;
-        D.touched[c] = 0;
-    }
-    dI = block_sum_fx(dI, red);
-    dT = block_sum_fx(dT, red);
-    if (threadIdx.x == 0 && (dI != 0 || dT != 0)) {
-        acc_add(D.acc, dI);
-        acc_add(D.acc + 1, dT);
-    }
+    delta_body(b, v0, v1, Cn, C32, col, w, a, twom_d, D, (int)blockIdx.x, (int)gridDim.x, red);
 }
 
 __global__ void __launch_bounds__(kBlock)

@@ -365,9 +365,9 @@ lm_small_kernel(LmArgs A, SmallBins L)
     __shared__ BlockRed R;
     // Every index into L is a constant (a dynamically indexed kernel
     // argument would be copied to scratch memory).
-    int bin = 0;
+    int bin = 0;   // 0..5 short-row bins, 6 key fill, 7 the previous batch's modularity delta
 #pragma unroll
-    for (int k = 1; k <= NUM_SMALL_BINS; ++k) bin += (int)blockIdx.x >= L.blk0[k];
+    for (int k = 1; k <= NUM_SMALL_BINS + 1; ++k) bin += (int)blockIdx.x >= L.blk0[k];
     const int b = (int)blockIdx.x;
 #define NIDO_BIN_ARGS(K) L.verts[K], L.n[K], b - L.blk0[K], L.blk0[K + 1] - L.blk0[K]
     switch (bin) {
@@ -377,9 +377,13 @@ lm_small_kernel(LmArgs A, SmallBins L)
     case BIN_G32: group_body<32, 128, Multi>(A, NIDO_BIN_ARGS(BIN_G32), lds); break;
     case BIN_G64: group_body<64, 256, Multi>(A, NIDO_BIN_ARGS(BIN_G64), lds); break;
     case BIN_B256: block_body<1024, Multi>(A, NIDO_BIN_ARGS(BIN_B256), lds, R); break;
-    default:
+    case NUM_SMALL_BINS:
         keyfill_body<Multi>(A, L, b - L.blk0[NUM_SMALL_BINS], L.blk0[NUM_SMALL_BINS + 1] - L.blk0[NUM_SMALL_BINS],
                             lds);
+        break;
+    default:
+        delta_body(L.db, L.dv0, L.dv1, L.Cn64, A.C, A.col, A.w, A.a, L.twom_d, L.D, b - L.blk0[NUM_SMALL_BINS + 1],
+                   L.blk0[NUM_SMALL_BINS + 2] - L.blk0[NUM_SMALL_BINS + 1], reinterpret_cast<fx_t*>(lds));
         break;
     }
 #undef NIDO_BIN_ARGS
@@ -720,33 +724,68 @@ commit_kernel(const int* __restrict__ list, int64_t v0, int64_t n, int64_t* __re
 // The commit plus the bookkeeping of the incremental modularity: a moved
 // vertex lists its row (in chunks) and its old and new community (each
 // community once per batch, through its touched flag).
+// Exclusive prefix of x over the wavefront and its total (every lane).
+__device__ __forceinline__ int wave_excl_scan(int x, int& total)
+{
+    const int lane = threadIdx.x & 63;
+    int inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+    }
+    total = __shfl(inc, 63, 64);
+    return inc - x;
+}
+
 __global__ void __launch_bounds__(kBlock)
 commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, int64_t* __restrict__ Cn,
                       int* __restrict__ C32, const fx_t* __restrict__ k, fx_t* __restrict__ a,
                       const int64_t* __restrict__ off, ModDelta D)
 {
-    for (int64_t v = v0 + blockIdx.x * (int64_t)kBlock + threadIdx.x; v < v1; v += (int64_t)gridDim.x * kBlock) {
-        const int64_t src = C[v];
-        const int64_t dst = Cn[v];
-        if (src != dst) {
-            const fx_t kv = k[v];
-            acc_add(a + src, -kv);
-            acc_add(a + dst, kv);
-            C32[v] = (int)dst;
-            const int64_t e0 = off[v], e1 = off[v + 1];
-            if (e1 > e0) {
-                const int n = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
-                const int base = atomicAdd(&D.counts[2 * b], n);
-                for (int i = 0; i < n; ++i) {
-                    const int64_t s0 = e0 + (int64_t)i * kDeltaArcs;
-                    D.chunks[base + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
-                }
+    const int lane = threadIdx.x & 63;
+    // wavefront-uniform loop so that list space is reserved once per wavefront
+    for (int64_t w0 = v0 + blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); w0 < v1;
+         w0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t v = w0 + lane;
+        int nchunk = 0, ncomm = 0;
+        int64_t src = 0, dst = 0, e0 = 0, e1 = 0;
+        bool fs = false, fd = false;
+        if (v < v1) {
+            src = C[v];
+            dst = Cn[v];
+            if (src != dst) {
+                const fx_t kv = k[v];
+                acc_add(a + src, -kv);
+                acc_add(a + dst, kv);
+                C32[v] = (int)dst;
+                e0 = off[v];
+                e1 = off[v + 1];
+                nchunk = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
+                fs = atomicExch(&D.touched[src], 1) == 0;
+                fd = atomicExch(&D.touched[dst], 1) == 0;
+                ncomm = (int)fs + (int)fd;
             }
-            if (atomicExch(&D.touched[src], 1) == 0) D.comms[atomicAdd(&D.counts[2 * b + 1], 1)] = (int)src;
-            if (atomicExch(&D.touched[dst], 1) == 0) D.comms[atomicAdd(&D.counts[2 * b + 1], 1)] = (int)dst;
+            C[v] = dst;
+            Cn[v] = src;
         }
-        C[v] = dst;
-        Cn[v] = src;
+        int tc, tm;
+        const int pc = wave_excl_scan(nchunk, tc);
+        const int pm = wave_excl_scan(ncomm, tm);
+        int bc = 0, bm = 0;
+        if (lane == 0) {
+            if (tc) bc = atomicAdd(&D.counts[2 * b], tc);
+            if (tm) bm = atomicAdd(&D.counts[2 * b + 1], tm);
+        }
+        bc = __shfl(bc, 0, 64);
+        bm = __shfl(bm, 0, 64);
+        for (int i = 0; i < nchunk; ++i) {
+            const int64_t s0 = e0 + (int64_t)i * kDeltaArcs;
+            D.chunks[bc + pc + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
+        }
+        int at = bm + pm;
+        if (fs) D.comms[at++] = (int)src;
+        if (fd) D.comms[at] = (int)dst;
     }
 }
 
@@ -766,6 +805,8 @@ int small_bins_blocks(SmallBins& L, unsigned cap_per_bin)
     L.blk0[NUM_SMALL_BINS] = total;
     if (L.nchunks > 0) total += (int)grid_for(L.nchunks, 1, cap_per_bin);
     L.blk0[NUM_SMALL_BINS + 1] = total;
+    total += L.dblocks;
+    L.blk0[NUM_SMALL_BINS + 2] = total;
     return total;
 }
 
@@ -777,20 +818,26 @@ static void small_launch(const LmArgs& A, const SmallBins& L, int blocks, hipStr
 
 void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s)
 {
-    if (bins && fill && !split) {
-        const int blocks = L.blk0[NUM_SMALL_BINS + 1];
-        if (blocks > 0) small_launch(A, L, blocks, s);
+    // parts: 0..5 short-row bins, 6 key fill, 7 delta (goes with the bins)
+    bool want[NUM_SMALL_BINS + 2];
+    for (int k = 0; k < NUM_SMALL_BINS + 2; ++k) want[k] = k == NUM_SMALL_BINS ? fill : bins;
+    auto launch_parts = [&](const bool* on) {
+        SmallBins one = L;
+        for (int k = 0; k < NUM_SMALL_BINS; ++k) one.n[k] = on[k] ? L.n[k] : 0;
+        if (!on[NUM_SMALL_BINS]) one.nchunks = 0;
+        if (!on[NUM_SMALL_BINS + 1]) one.dblocks = 0;
+        const int blocks = small_bins_blocks(one, 4096);
+        if (blocks > 0) small_launch(A, one, blocks, s);
+    };
+    if (!split) {
+        launch_parts(want);
         return;
     }
-    // b < NUM_SMALL_BINS: bin b; b == NUM_SMALL_BINS: the key fill
-    const int first = bins ? 0 : NUM_SMALL_BINS, last = fill ? NUM_SMALL_BINS : NUM_SMALL_BINS - 1;
-    for (int b = first; b <= last; ++b) {
-        SmallBins one = L;
-        for (int k = 0; k < NUM_SMALL_BINS; ++k) one.n[k] = (split ? k == b : bins) ? L.n[k] : 0;
-        if (b != NUM_SMALL_BINS) one.nchunks = 0;
-        if (small_bins_blocks(one, 4096) > 0)
-            small_launch(A, one, one.blk0[NUM_SMALL_BINS + 1], s);
-        if (!split && b < NUM_SMALL_BINS) b = NUM_SMALL_BINS - 1;   // all bins went in one launch
+    for (int k = 0; k < NUM_SMALL_BINS + 2; ++k) {
+        if (!want[k]) continue;
+        bool on[NUM_SMALL_BINS + 2] = {};
+        on[k] = true;
+        launch_parts(on);
     }
 }
 
