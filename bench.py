@@ -62,6 +62,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel_class):
+    """HBM bytes per launch of a kernel class on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated);
+    None when the summary has no line for it."""
+    path = os.path.join(HERE, "profiles", "traffic_rmat_s22_b32.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel_class:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(off, col, w, nb, tol, max_iters, budget_s, gpu_first_q):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -283,6 +299,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch knobs with nido_autotune before timing (reported in config.tuning)")
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="time the steps without per-kernel HIP events (roofline fields are then null)")
     ap.add_argument("--tuning", default="", help="comma list of nido_tuning knobs, e.g. big_slots=2048")
@@ -333,6 +351,10 @@ def main():
     g = na.GraphGPU(hg, args.batches)
     if args.tuning:
         g.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
+    if args.autotune:
+        t0 = time.perf_counter()
+        chosen = g.autotune(2)
+        log(f"autotune: {chosen} ({time.perf_counter() - t0:.1f}s)")
     g.singleton_partition()
     q = g.compute_modularity()
     state = {"q": q, "first_q": None, "resets": 0}
@@ -369,6 +391,8 @@ def main():
     ks = g.kernel_stats()
     g.set_profiling(False)
 
+    default_workload = (args.workload == "rmat" and args.scale == 22 and args.edge_factor == 16
+                        and args.batches == 32 and args.seed == 7)
     gteps = ne * args.steps / dt / 1e9
     ms_step = dt * 1e3 / args.steps
     d = ks[dom]
@@ -397,7 +421,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom) if default_workload else None,
             "launches": d["launches"], "avg_launch_us": round(d["ms"] * 1e3 / max(d["launches"], 1), 3),
             "algorithmic_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1), 1),
         },
