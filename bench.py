@@ -151,13 +151,19 @@ def bench_batched(args):
         o = off[v0:v1 + 1] - off[v0]
         return o, col[off[v0]:off[v1]] - v0, w[off[v0]:off[v1]]
 
-    # warm-up: a small batch through both paths (first-launch costs)
-    wb = na.BatchGPU([sub(i) for i in range(min(G, 8))], nb)
+    # warm-up: a small batch through both paths (first-launch costs); the
+    # autotuner picks the launch knobs on the first 256 graphs
+    wb = na.BatchGPU([sub(i) for i in range(min(G, 256))], nb)
+    defaults = wb.get_tuning()
+    chosen = wb.autotune(2) if not args.no_autotune else defaults
+    log(f"autotune on {min(G, 256)} graphs: {chosen}")
     wb.run(500, tol)
     wb.close()
 
     t0 = time.perf_counter()
     bg = na.BatchGPU(csr=(vptr, off, col, w), nbatches=nb)
+    if chosen != defaults:
+        bg.set_tuning(**chosen)
     t1 = time.perf_counter()
     st = bg.run(500, tol)
     t_batch = time.perf_counter() - t0
@@ -207,7 +213,7 @@ def bench_batched(args):
                                f"intra-degree ~12, inter-degree ~2, U(0,1] weights, seeds 1000+i, -b {nb} "
                                f"-t {tol:g}, full runs (all phases)",
                    "graphs": G, "vertices": int(vptr[-1]), "arcs": int(len(col)),
-                   "phases": st.phases, "shared_iterations": st.total_loops},
+                   "phases": st.phases, "shared_iterations": st.total_loops, "tuning": chosen},
         "plain": {"graphs_per_s": round(plain_gps, 3), "sample": len(sample),
                   "sample_rule": f"every {max(1, G // args.plain_sample)}th graph"},
         "speedup_vs_plain": round(batch_gps / plain_gps, 2),
@@ -299,6 +305,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batched: keep the default launch knobs")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch knobs with nido_autotune before timing (reported in config.tuning)")
     ap.add_argument("--no-kernel-events", action="store_true",

@@ -542,6 +542,11 @@ class BatchGPU:
         _check(lib.nido_autotune(self._h, iterations, C.byref(t)), "nido_autotune")
         return t.as_dict()
 
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib.nido_graph_get_tuning(self._h, C.byref(t)), "nido_graph_get_tuning")
+        return t.as_dict()
+
     def set_tuning(self, **kw) -> None:
         t = Tuning()
         _check(lib.nido_graph_get_tuning(self._h, C.byref(t)), "nido_graph_get_tuning")
