@@ -262,7 +262,7 @@ struct nido_graph {
     double* h_q = nullptr;         // pinned
     // local-moving bins: per batch, the fused small-row launch and the
     // partitioned long-row launch
-    int* d_binv = nullptr;
+    BinRow* d_binv = nullptr;
     std::vector<SmallBins> small_bins;   // nb
     std::vector<BigArgs> big_args;       // nb
     BigItem* d_big_items = nullptr;
@@ -454,7 +454,7 @@ int setup_phase(nido_graph* g)
     // per batch, P = 2^ceil(log2(d / part_arcs)) partition items, their
     // partial slots, HBM fallback tables, and for P > 1 the key-fill chunks
     // and bucket space.
-    std::vector<int> binv((size_t)std::max<int64_t>(nv, 1));
+    std::vector<BinRow> binv((size_t)std::max<int64_t>(nv, 1));
     std::vector<BigItem> bitems;
     std::vector<KeyChunk> chunks;
     std::vector<int64_t> poff, toff;
@@ -499,7 +499,7 @@ int setup_phase(nido_graph* g)
             // C[col]) + 40 per vertex (offset, C, k, a_C, C')
             (k != BIN_BIG ? g->short_bytes : g->long_bytes)[(size_t)b] += 24.0 * (double)d + 40.0;
             if (d == 0) continue;
-            binv[(size_t)cur[k]++] = (int)v;
+            binv[(size_t)cur[k]++] = BinRow{(int)v, (int)std::min<int64_t>(d, INT32_MAX), off[v]};
             if (k != BIN_BIG) continue;
             int P = 1;
             while (P < kMaxParts && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
@@ -1356,6 +1356,14 @@ int do_autotune(nido_graph* g, int iters, nido_tuning* out)
             t.big_key_limit = v / 2;
         });
     for (int v : {0, 1}) try_value([&](nido_tuning& t) { t.overlap_long = v; });
+    // small long-row tables beside the short rows on a second stream
+    for (int v : {384, 512})
+        try_value([&](nido_tuning& t) {
+            t.big_slots = 1024;
+            t.big_key_limit = 512;
+            t.big_part_arcs = v;
+            t.overlap_long = 1;
+        });
     for (int v : {0, 1}) try_value([&](nido_tuning& t) { t.use_graph = v; });
     g->prof.on = prof;
     g->tuning = best;
@@ -1570,8 +1578,8 @@ int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
         return set_error(NIDO_ERR_INVALID, "big_min_deg must be in [0, %d]", (int)kBinMaxDeg[BIN_B256]);
     if (t->big_part_arcs < 1)
         return set_error(NIDO_ERR_INVALID, "big_part_arcs must be >= 1");
-    if (t->big_slots != 2048 && t->big_slots != 4096)
-        return set_error(NIDO_ERR_INVALID, "big_slots must be 2048 or 4096");
+    if (t->big_slots != 1024 && t->big_slots != 2048 && t->big_slots != 4096)
+        return set_error(NIDO_ERR_INVALID, "big_slots must be 1024, 2048 or 4096");
     if (t->big_key_limit < 1 || t->big_key_limit > t->big_slots / 2)
         return set_error(NIDO_ERR_INVALID, "big_key_limit must be in [1, big_slots/2]");
     AllocScope scope(g->stream);

@@ -60,10 +60,17 @@ struct ModDelta {
 };
 constexpr int kDeltaBlocks = 256;          // workgroups of a delta riding in a short-row launch
 
+// A vertex of a degree bin with its row: arcs [e0, e0 + d) (saves the
+// offset load on the local-moving kernels' dependent-load chain).
+struct BinRow {
+    int v, d;
+    int64_t e0;
+};
+
 // The first launch of a batch: the small bins plus the key fill of the
 // batch's partitioned long rows.
 struct SmallBins {
-    const int* verts[NUM_SMALL_BINS];
+    const BinRow* verts[NUM_SMALL_BINS];
     int64_t n[NUM_SMALL_BINS];
     const KeyChunk* chunks;
     int64_t nchunks;
@@ -165,7 +172,7 @@ void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, co
                    hipStream_t s);
 // The commit over list[0, n) (or [v0, v0 + n) with a null list), restricted
 // to the vertices v whose graph has flag[gid[v]] set.
-void launch_commit_filtered(const int* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
+void launch_commit_filtered(const BinRow* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
                             const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s);
 
 // Modularity over the arc-parallel CSR (src = row ids): writes Q to *q;

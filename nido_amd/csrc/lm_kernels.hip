@@ -126,11 +126,10 @@ __device__ __forceinline__ void lds_insert(int* key, unsigned long long* lo, uns
 // Scans row v with `lanes` lanes (lane index `lane`), two arcs per lane per
 // step, inserting every non-self arc into the S-slot table.
 template <int S>
-__device__ __forceinline__ void scan_row(const LmArgs& A, int v, int lane, int lanes, int* key,
+__device__ __forceinline__ void scan_row(const LmArgs& A, int v, int64_t e0, int64_t e1, int lane, int lanes, int* key,
                                          unsigned long long* lo, unsigned long long* hi)
 {
-    const int64_t e1 = A.off[v + 1];
-    for (int64_t e = A.off[v] + lane; e < e1; e += 2 * (int64_t)lanes) {
+    for (int64_t e = e0 + lane; e < e1; e += 2 * (int64_t)lanes) {
         const int64_t f = e + lanes;
         const bool two = f < e1;
         const int u1 = (int)A.col[e];
@@ -147,17 +146,18 @@ __device__ __forceinline__ void scan_row(const LmArgs& A, int v, int lane, int l
 // ---- launch 1 bodies (blocks [bi, bi+nbk) of the fused launch) -------------
 
 template <bool Multi>
-__device__ __forceinline__ void thread_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk)
+__device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk)
 {
     for (int64_t i = (int64_t)bi * kBlock + threadIdx.x; i < n; i += (int64_t)nbk * kBlock) {
-        const int v = verts[i];
+        const BinRow r = verts[i];
+        const int v = r.v;
         double m;
         if (!graph_mass<Multi>(A, v, m)) continue;
         const int own = A.C[v];
         const double ki = A.kd[v];
         const fx_t kv = A.k[v];
-        const int64_t e0 = A.off[v];
-        const int d = (int)(A.off[v + 1] - e0);
+        const int64_t e0 = r.e0;
+        const int d = r.d;
         const fx_t rest = A.a[own] - kv;
         int uu[kThreadMaxDeg];
         double wd[kThreadMaxDeg];
@@ -191,7 +191,7 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const int* __restri
 // G lanes per vertex (G <= 64), S-slot LDS table per vertex; 256/G vertices
 // per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
 template <int G, int S, bool Multi>
-__device__ __forceinline__ void group_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
+__device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
                            char* lds)
 {
     constexpr int NG = kBlock / G;
@@ -212,14 +212,15 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const int* __restric
         }
         const int64_t idx = base + grp;
         bool act = idx < n;
-        const int v = act ? verts[idx] : 0;
+        const BinRow r = act ? verts[idx] : BinRow{0, 0, 0};
+        const int v = r.v;
         double m = 0.0;
         act = act && graph_mass<Multi>(A, v, m);
         const int own = act ? A.C[v] : 0;
         const double ki = act ? A.kd[v] : 0.0;
         const fx_t rest = act ? A.a[own] - A.k[v] : fx_t(0);
         wave_lds_sync();
-        if (act) scan_row<S>(A, v, lane, G, key, lo, hi);
+        if (act) scan_row<S>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi);
         wave_lds_sync();
         if (act) {   // uniform within the group
             fx_t self = 0;
@@ -247,7 +248,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const int* __restric
 
 // One workgroup per vertex, S-slot LDS table.
 template <int S, bool Multi>
-__device__ __forceinline__ void block_body(const LmArgs& A, const int* __restrict__ verts, int64_t n, int bi, int nbk,
+__device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
                            char* lds, BlockRed& R)
 {
     int* key = reinterpret_cast<int*>(lds);
@@ -259,14 +260,15 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const int* __restric
             lo[s] = 0;
             hi[s] = 0;
         }
-        const int v = verts[idx];
+        const BinRow r = verts[idx];
+        const int v = r.v;
         double m;
         if (!graph_mass<Multi>(A, v, m)) continue;   // uniform: the workgroup holds one vertex
         const int own = A.C[v];
         const double ki = A.kd[v];
         const fx_t rest = A.a[own] - A.k[v];
         __syncthreads();
-        scan_row<S>(A, v, threadIdx.x, kBlock, key, lo, hi);
+        scan_row<S>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi);
         __syncthreads();
         fx_t self = 0;
         double bg = -kMaxFloat;
@@ -701,12 +703,12 @@ lm_big_kernel(LmArgs A, BigArgs B)
 // The vertices are list[0, n) or, with a null list, [v0, v0 + n); with gid
 // set, only vertices of graphs whose flag is set take part.
 __global__ void __launch_bounds__(kBlock)
-commit_kernel(const int* __restrict__ list, int64_t v0, int64_t n, int64_t* __restrict__ C,
+commit_kernel(const BinRow* __restrict__ list, int64_t v0, int64_t n, int64_t* __restrict__ C,
               int64_t* __restrict__ Cn, int* __restrict__ C32, const fx_t* __restrict__ k,
               fx_t* __restrict__ a, const int* __restrict__ gid, const unsigned char* __restrict__ flag)
 {
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const int64_t v = list ? (int64_t)list[i] : v0 + i;
+        const int64_t v = list ? (int64_t)list[i].v : v0 + i;
         if (gid && !flag[gid[v]]) continue;
         const int64_t src = C[v];
         const int64_t dst = Cn[v];
@@ -845,15 +847,22 @@ void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s)
 {
     if (B.nitems <= 0) return;
     const unsigned g = grid_for(B.nitems, 1, 2048);
-    const int which = (B.slots == 2048 ? 0 : 4) + (B.narrow ? 0 : 2) + (A.gid ? 1 : 0);
+    // a 1024-slot table keeps the workgroup at ~20 KB of LDS, so it can run
+    // beside the short-row workgroups (overlap_long)
+    const int size = B.slots == 1024 ? 0 : B.slots == 2048 ? 1 : 2;
+    const int which = size * 4 + (B.narrow ? 0 : 2) + (A.gid ? 1 : 0);
     switch (which) {
-    case 0: lm_big_kernel<true, 2048, false><<<g, kBlock, 0, s>>>(A, B); break;
-    case 1: lm_big_kernel<true, 2048, true><<<g, kBlock, 0, s>>>(A, B); break;
-    case 2: lm_big_kernel<false, 2048, false><<<g, kBlock, 0, s>>>(A, B); break;
-    case 3: lm_big_kernel<false, 2048, true><<<g, kBlock, 0, s>>>(A, B); break;
-    case 4: lm_big_kernel<true, 4096, false><<<g, kBlock, 0, s>>>(A, B); break;
-    case 5: lm_big_kernel<true, 4096, true><<<g, kBlock, 0, s>>>(A, B); break;
-    case 6: lm_big_kernel<false, 4096, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 0: lm_big_kernel<true, 1024, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 1: lm_big_kernel<true, 1024, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 2: lm_big_kernel<false, 1024, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 3: lm_big_kernel<false, 1024, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 4: lm_big_kernel<true, 2048, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 5: lm_big_kernel<true, 2048, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 6: lm_big_kernel<false, 2048, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 7: lm_big_kernel<false, 2048, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 8: lm_big_kernel<true, 4096, false><<<g, kBlock, 0, s>>>(A, B); break;
+    case 9: lm_big_kernel<true, 4096, true><<<g, kBlock, 0, s>>>(A, B); break;
+    case 10: lm_big_kernel<false, 4096, false><<<g, kBlock, 0, s>>>(A, B); break;
     default: lm_big_kernel<false, 4096, true><<<g, kBlock, 0, s>>>(A, B); break;
     }
 }
@@ -873,7 +882,7 @@ void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* C
     commit_tracked_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(v0, v1, b, C, Cn, C32, k, a, off, D);
 }
 
-void launch_commit_filtered(const int* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
+void launch_commit_filtered(const BinRow* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
                             const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s)
 {
     if (n <= 0) return;

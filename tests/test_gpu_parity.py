@@ -129,6 +129,8 @@ KNOBS = [
     {"overlap_long": 0, "split_small": 1},
     {"use_graph": 1},                                               # iterations replayed from a hipGraph
     {"use_graph": 1, "overlap_long": 1},
+    {"big_slots": 1024, "big_key_limit": 512, "big_part_arcs": 384},
+    {"big_slots": 1024, "big_key_limit": 512, "big_part_arcs": 384, "overlap_long": 1},
     {"full_modularity": 1},                                         # every iteration sweeps all arcs for Q
     {"full_modularity": 1, "use_graph": 1},
 ]
@@ -206,3 +208,33 @@ def test_autotune_keeps_results():
     assert st.phases == len(ref["phases"]) and st.q == ref["q"]
     assert np.array_equal(g.clustering(), ref["clustering"])
     g.close()
+
+
+def test_chung_lu_full_run():
+    """The large-graph family (Chung-Lu, Pareto degrees) at a size the oracle
+    finishes in seconds."""
+    na = _na()
+    off, col, w = na.gen_chung_lu(30000, 2.1, 2.0, 3000.0, 16.0, False, 11).arrays()
+    print(f"case: chung-lu 30000 maxdeg={int(np.max(np.diff(off)))}", flush=True)
+    got = run_gpu(off, col, w, 32)
+    ref = run_oracle(off, col, w, 32)
+    assert_runs_equal(ref, got)
+
+
+def test_cli_matches_oracle(golden_dir, tmp_path):
+    """nido_louvain -f <file> -b 2 -t 1e-6 -o <out>: the reference's command
+    line (main.cpp:15-177); the written partition is the oracle's."""
+    import subprocess
+    na = _na()
+    exe = os.path.join(os.path.dirname(na.LIB_PATH), "nido_louvain")
+    src = os.path.join(golden_dir, "sbm1024_seed1.bin")
+    out = tmp_path / "comm.txt"
+    print("case: CLI sbm1024", flush=True)
+    r = subprocess.run([exe, "-f", src, "-b", "2", "-t", "1e-6", "-o", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "#Batches: 2" in r.stdout
+    got = np.loadtxt(out, dtype=np.int64)
+    off, col, w = na.Graph.read(src).arrays()
+    ref = run_oracle(off, col, w, 2)
+    assert np.array_equal(got, ref["clustering"])
