@@ -84,6 +84,12 @@ typedef struct {
 } nido_tuning;
 
 const char* nido_last_error(void);
+/* The exact Q64.64 <-> FP64 conversions every weight sum goes through
+ * (round-to-nearest-even both ways), as compiled for the host: value =
+ * hi + lo * 2^-64, two's complement.  No counterpart in the reference, whose
+ * sums are FP64; exposed so that the conversions can be checked. */
+void nido_fx_from_double(double w, uint64_t* lo, int64_t* hi);
+double nido_fx_to_double(uint64_t lo, int64_t hi);
 int nido_abi_version(void);
 /* Number of HIP devices visible (0 on a machine without a GPU). */
 int nido_device_count(int* count);

@@ -49,9 +49,16 @@ NIDO_HD fx_t fx_from_double(double w)
         mt |= 0x0010000000000000ull;
     }
     int sh = (int)ex - 1011;                     // w * 2^64 = mt * 2^sh
-    ufx_t u;
-    if (sh >= 0) {
-        u = (ufx_t)mt << sh;
+    uint64_t lo, hi;                             // |w| * 2^64 as two 64-bit words
+    if (sh >= 64) {                              // |w| < 2^62 keeps sh < 74
+        lo = 0;
+        hi = mt << (sh - 64);
+    } else if (sh > 0) {
+        lo = mt << sh;
+        hi = mt >> (64 - sh);
+    } else if (sh == 0) {
+        lo = mt;
+        hi = 0;
     } else {
         int r = -sh;
         if (r >= 54) return 0;
@@ -59,32 +66,39 @@ NIDO_HD fx_t fx_from_double(double w)
         uint64_t rest = mt & ((1ull << r) - 1);
         uint64_t half = 1ull << (r - 1);
         q += (rest > half) || (rest == half && (q & 1ull));
-        u = q;
+        lo = q;
+        hi = 0;
     }
-    fx_t x = (fx_t)u;
-    return (b >> 63) ? -x : x;
+    if (b >> 63) {                               // two's-complement negation
+        hi = ~hi + (lo == 0);
+        lo = 0 - lo;
+    }
+    return fx_make(lo, (int64_t)hi);
 }
 
-// Q64.64 -> FP64, round-to-nearest-even.
+// Q64.64 -> FP64, round-to-nearest-even.  The magnitude is normalised
+// into one 64-bit word t (its leading 1 at bit 63 when the high word is
+// nonzero) with every bit shifted out ORed into t's lowest bit: those bits
+// lie below the rounding position (bit 10), so the sticky bit keeps the
+// above/at/below-half decision intact and the correctly rounded u64 -> f64
+// conversion does the RNE.  Scaling by a power of two is exact.
 NIDO_HD double fx_to_double(fx_t x)
 {
-    if (x == 0) return 0.0;
-    bool neg = x < 0;
-    ufx_t u = neg ? (ufx_t)0 - (ufx_t)x : (ufx_t)x;
-    uint64_t hi = (uint64_t)(u >> 64);
-    int top = hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll((uint64_t)u);
+    uint64_t lo = fx_lo(x);
+    uint64_t hi = (uint64_t)fx_hi(x);
+    const bool neg = (int64_t)hi < 0;
+    if (neg) {
+        hi = ~hi + (lo == 0);
+        lo = 0 - lo;
+    }
     double r;
-    if (top <= 52) {
-        // < 2^53 units: the integer converts exactly; 2^-64 scaling is exact
-        r = (double)(uint64_t)u * nido_bitsd((uint64_t)(1023 - 64) << 52);
+    if (hi == 0) {
+        r = (double)lo * nido_bitsd((uint64_t)(1023 - 64) << 52);     // lo * 2^-64
     } else {
-        int s = top - 52;
-        uint64_t mt = (uint64_t)(u >> s);
-        ufx_t rest = u & (((ufx_t)1 << s) - 1);
-        ufx_t half = (ufx_t)1 << (s - 1);
-        mt += (rest > half) || (rest == half && (mt & 1ull));
-        if (mt == (1ull << 53)) { mt >>= 1; ++s; }
-        r = (double)mt * nido_bitsd((uint64_t)(1023 + s - 64) << 52);
+        const int n = __builtin_clzll(hi);                           // 0..63 (hi < 2^63 here)
+        const uint64_t t = n ? (hi << n) | (lo >> (64 - n)) : hi;
+        const uint64_t sticky = (n ? lo << n : lo) != 0;
+        r = (double)(t | sticky) * nido_bitsd((uint64_t)(1023 - n) << 52);   // * 2^-n
     }
     return neg ? -r : r;
 }

@@ -480,3 +480,15 @@ void graph_twom(const int64_t* off, const double* w, int G, const int64_t* vptr,
 }
 
 }  // namespace nido
+
+// ---- the Q64.64 conversions of fixed.h (host build) -----------------------
+extern "C" {
+void nido_fx_from_double(double w, uint64_t* lo, int64_t* hi)
+{
+    const fx_t x = fx_from_double(w);
+    *lo = fx_lo(x);
+    *hi = fx_hi(x);
+}
+
+double nido_fx_to_double(uint64_t lo, int64_t hi) { return fx_to_double(fx_make(lo, hi)); }
+}

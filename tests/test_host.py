@@ -187,3 +187,39 @@ def test_concat_graphs_layout():
     assert list(off) == [0, 1, 2, 4, 5, 6]
     assert list(col) == [1, 0, 3, 4, 2, 2]
     assert list(w) == [0.5, 0.5, 1.0, 2.0, 1.0, 2.0]
+
+
+def _lib_fx(w):
+    import ctypes as C
+    lo, hi = C.c_uint64(), C.c_int64()
+    na.lib.nido_fx_from_double(C.c_double(w), C.byref(lo), C.byref(hi))
+    return (hi.value << 64) | lo.value
+
+
+def test_device_conversions_match_the_oracle():
+    """fixed.h (the conversions the kernels run, built for the host) against
+    the oracle's independent restatement, itself pinned by exact rational
+    arithmetic in test_oracle.py: weights of every magnitude and sign,
+    subnormals, exact ties at 2^-65, and 128-bit values spanning all widths."""
+    import ctypes as C
+    import oracle
+    na.lib.nido_fx_from_double.argtypes = [C.c_double, C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]
+    na.lib.nido_fx_to_double.argtypes = [C.c_uint64, C.c_int64]
+    na.lib.nido_fx_to_double.restype = C.c_double
+    rng = np.random.default_rng(3)
+    ws = [0.0, -0.0, 1.0, -1.0, 2.0 ** -64, 2.0 ** -65, 3 * 2.0 ** -65, 5 * 2.0 ** -65, -(3 * 2.0 ** -65),
+          2.0 ** -1074, 2.0 ** -1022, 2.0 ** 61, -(2.0 ** 61), np.nextafter(2.0 ** 62, 0), 1.0 - 2.0 ** -53]
+    ws += list(rng.uniform(-1, 1, 20000) * 2.0 ** rng.integers(-80, 62, 20000))
+    ws += list(np.ldexp(rng.integers(1, 1 << 20, 5000), rng.integers(-90, -60, 5000)) + 0.0)
+    for w in ws:
+        got = _lib_fx(float(w))
+        got = got - (1 << 128) if got >= (1 << 127) else got
+        assert got == oracle.fx_from_double(float(w)), w
+    xs = [0, 1, -1, (1 << 64), -(1 << 64), (1 << 127) - 1, -(1 << 127), (1 << 53) + 1, 3 << 70]
+    for bits in rng.integers(1, 127, 20000):
+        x = int(rng.integers(0, 1 << 62)) << max(0, int(bits) - 62) | int(rng.integers(0, 1 << 62))
+        x &= (1 << int(bits)) - 1
+        xs.append(x if rng.random() < 0.5 else -x)
+    for x in xs:
+        lo, hi = x & ((1 << 64) - 1), x >> 64
+        assert na.lib.nido_fx_to_double(lo, hi) == oracle.fx_to_double(x), x
