@@ -95,3 +95,18 @@ def test_batch_autotune_keeps_results():
         assert abs(bg.graph_stats(i).q - r["q"]) <= Q_TOL, f"graph {i}"
         assert np.array_equal(bg.clustering(i), r["clustering"]), f"graph {i}"
     bg.close()
+
+
+def test_batch_with_partitioned_long_rows():
+    """Graphs whose hubs exceed the long-row threshold and the partition
+    size (rows split over several workgroups) next to small graphs."""
+    na = _na()
+    graphs = [na.gen_rmat(14, 16, 0.57, 0.19, 0.19, True, 21).arrays(),
+              na.gen_sbm(500, 50, 0.3, 0.02, True, 22).arrays(),
+              na.gen_rmat(13, 32, 0.6, 0.15, 0.15, False, 23).arrays()]
+    for off, col, w in graphs[::2]:
+        assert int(np.max(np.diff(off))) > 1536
+    print("case: batch with long rows nb=8", flush=True)
+    _check(na, graphs, 8)
+    print("case: batch with long rows nb=8, small partitions", flush=True)
+    _check(na, graphs, 8, big_part_arcs=64, big_min_deg=128)
