@@ -74,6 +74,23 @@ __device__ __forceinline__ int64_t decide(const LmArgs& A, double m, int v, int 
     return decide(m, A.kd[v], A.a[own] - A.k[v], own, self, best_g, best_c);
 }
 
+// Candidate update for one table entry whose a_c the caller has loaded.
+__device__ __forceinline__ void consider_a(double m, double ki, int own, int c, fx_t e, fx_t ac,
+                                           fx_t& self, double& bg, int& bc)
+{
+    if (c == own) {
+        self = e;
+        return;
+    }
+    double g = fx_to_double(e);
+    g -= ki * fx_to_double(ac) / (2. * m);
+    g /= m;
+    if (better(g, c, bg, bc)) {
+        bg = g;
+        bc = c;
+    }
+}
+
 // Candidate update for one table entry.
 __device__ __forceinline__ void consider(const LmArgs& A, double m, double ki, int own, int c, fx_t e,
                                          fx_t& self, double& bg, int& bc)
@@ -360,7 +377,7 @@ constexpr int kSmallLds = 20480;   // (256/G)*S*20 for every short-row bin
 static_assert((kMaxParts + 1 + kBlock / 64) * 4 <= kSmallLds, "key-fill counters exceed the LDS buffer");
 
 template <bool Multi>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 7)
 lm_small_kernel(LmArgs A, SmallBins L)
 {
     __shared__ __align__(16) char lds[kSmallLds];
@@ -454,7 +471,7 @@ __device__ __forceinline__ void lds_add(unsigned long long* lo, int* hi, fx_t w)
     fx_atomic_add_narrow(lo, hi, w);
 }
 
-constexpr int kQueue = 128;   // per-wavefront compaction queue of the sweep (entries)
+constexpr int kFill = 6;      // entries per thread loaded ahead of their inserts
 constexpr int kOverflow = -2; // partial community of a partition that outgrew its table
 
 // S-slot LDS table per workgroup.  Narrow: 32-bit high words (S*16 bytes),
@@ -472,15 +489,15 @@ __global__ void __launch_bounds__(kBlock)
 lm_big_kernel(LmArgs A, BigArgs B)
 {
     using Hi = typename HiWord<Narrow>::type;
+    constexpr int kList = S / 2 + kBlock;   // key_limit (<= S/2) + one step of every wavefront
     __shared__ int key[S];
     __shared__ unsigned long long lo[S];
     __shared__ Hi hi[S];
-    __shared__ int qkey[kBlock / 64][kQueue];
+    __shared__ int klist[kList];            // occupied slots, in order of first insertion
     __shared__ int s_pre[kBlock], s_lo[kBlock], wtot[kBlock / 64];
     __shared__ BlockRed R;
     __shared__ int s_nkeys, s_ovf, s_last;
     const int lane = threadIdx.x & 63;
-    int* const qk = qkey[threadIdx.x >> 6];
 
     for (int64_t it = blockIdx.x; it < B.nitems; it += gridDim.x) {
         const BigItem item = B.items[it];
@@ -532,14 +549,17 @@ lm_big_kernel(LmArgs A, BigArgs B)
         // Inserts (c, w) when c >= 0.  Probing gives up after kMaxProbe
         // slots and marks the partition overflowed (the row then takes the
         // exact HBM path), so no lane can spin on a full table.  New keys are
-        // counted per wavefront step; past key_limit (<= S/2) the partition is
-        // marked overflowed, and as at most 256 uncounted keys exist at any
-        // time the table never holds more than key_limit + 256 < S keys.
-        // Called by all lanes of the workgroup with a uniform trip count.
+        // counted per wavefront step and their slots appended to klist; past
+        // key_limit (<= S/2) the partition is marked overflowed.  A wavefront
+        // step adds at most 64 keys, so a partition that stays within the
+        // limit has every slot in the list; entries past kList are dropped
+        // (their partition has overflowed).  Called by all lanes of the
+        // workgroup with a uniform trip count.
         auto insert = [&](int c, double w) {
             bool fresh = false;
+            unsigned h = 0;
             if (c >= 0) {
-                unsigned h = ((unsigned)c * kHashMul) >> (32 - lg);
+                h = ((unsigned)c * kHashMul) >> (32 - lg);
                 bool ok = false;
                 for (int probes = 0; probes < kMaxProbe; ++probes) {
                     const int kk = key[h];
@@ -558,23 +578,36 @@ lm_big_kernel(LmArgs A, BigArgs B)
                 else s_ovf = 1;
             }
             const unsigned long long nf = __ballot(fresh);
-            if (lane == 0 && nf) {
-                if (atomicAdd(&s_nkeys, __popcll(nf)) + __popcll(nf) > B.key_limit) s_ovf = 1;
+            if (nf) {                                   // uniform
+                int base = 0;
+                if (lane == 0) {
+                    base = atomicAdd(&s_nkeys, __popcll(nf));
+                    if (base + __popcll(nf) > B.key_limit) s_ovf = 1;
+                }
+                base = __shfl(base, 0, 64);
+                const int at = base + lane_rank(nf);
+                if (fresh && at < kList) klist[at] = (int)h;
             }
         };
 
+        // Each pass loads kFill entries per thread before inserting any, so
+        // their gathers are in flight together (LDS caps this kernel at two
+        // workgroups per CU; registers are not the limit).
         if (P == 1) {
-            for (int64_t i0 = 0; i0 < d; i0 += 2 * kBlock) {
+            for (int64_t i0 = 0; i0 < d; i0 += kFill * kBlock) {
                 if (s_ovf) break;
-                const int64_t i1 = i0 + threadIdx.x, i2 = i1 + kBlock;
-                const int u1 = i1 < d ? A.col[e0 + i1] : v;
-                const int u2 = i2 < d ? A.col[e0 + i2] : v;
-                const double w1 = i1 < d ? A.w[e0 + i1] : 0.0;
-                const double w2 = i2 < d ? A.w[e0 + i2] : 0.0;
-                const int c1 = u1 != v ? A.C[u1] : -1;     // self arcs weigh 0 (graph_cuda.cu:2312-2315)
-                const int c2 = u2 != v ? A.C[u2] : -1;
-                insert(c1, w1);
-                insert(c2, w2);
+                int us[kFill], cs[kFill];
+                double ws[kFill];
+#pragma unroll
+                for (int r = 0; r < kFill; ++r) {
+                    const int64_t i = i0 + r * kBlock + threadIdx.x;
+                    us[r] = i < d ? A.col[e0 + i] : v;
+                    ws[r] = i < d ? A.w[e0 + i] : 0.0;
+                }
+#pragma unroll
+                for (int r = 0; r < kFill; ++r) cs[r] = us[r] != v ? A.C[us[r]] : -1;   // self arcs weigh 0 (graph_cuda.cu:2312-2315)
+#pragma unroll
+                for (int r = 0; r < kFill; ++r) insert(cs[r], ws[r]);
             }
         } else {
             for (int g = 0; g * kBlock < nch; ++g) {
@@ -585,23 +618,36 @@ lm_big_kernel(LmArgs A, BigArgs B)
                 }
                 const int ng = min(kBlock, nch - g * kBlock);
                 const int64_t cbase = item.k0 + (int64_t)g * kBlock * kKeyChunkArcs;
-                for (int i0 = 0; i0 < total; i0 += kBlock) {
+                int top = 1;                                 // largest power of two < ng (or 1)
+                while (2 * top < ng) top *= 2;
+                for (int i0 = 0; i0 < total; i0 += kFill * kBlock) {
                     if (s_ovf) break;
-                    const int i = i0 + threadIdx.x;
-                    int c = -1;
-                    double w = 0.0;
-                    if (i < total) {
-                        int l = 0, h = ng;                   // last chunk whose start is <= i
-                        while (h - l > 1) {
-                            const int mid = (l + h) >> 1;
-                            if (s_pre[mid] <= i) l = mid;
-                            else h = mid;
+                    int l[kFill], cs[kFill];
+                    double ws[kFill];
+#pragma unroll
+                    for (int r = 0; r < kFill; ++r) l[r] = 0;
+                    // last chunk whose start is <= i, all kFill searches in step
+                    for (int st = top; st > 0; st >>= 1) {
+#pragma unroll
+                        for (int r = 0; r < kFill; ++r) {
+                            const int i = i0 + r * kBlock + threadIdx.x;
+                            const int j = l[r] + st;
+                            if (j < ng && s_pre[j] <= i) l[r] = j;
                         }
-                        const int64_t at = cbase + (int64_t)l * kKeyChunkArcs + s_lo[l] + (i - s_pre[l]);
-                        c = B.bkey[at];
-                        w = B.bw[at];
                     }
-                    insert(c, w);
+#pragma unroll
+                    for (int r = 0; r < kFill; ++r) {
+                        const int i = i0 + r * kBlock + threadIdx.x;
+                        cs[r] = -1;
+                        ws[r] = 0.0;
+                        if (i < total) {
+                            const int64_t at = cbase + (int64_t)l[r] * kKeyChunkArcs + s_lo[l[r]] + (i - s_pre[l[r]]);
+                            cs[r] = B.bkey[at];
+                            ws[r] = B.bw[at];
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < kFill; ++r) insert(cs[r], ws[r]);
                 }
             }
         }
@@ -611,32 +657,19 @@ lm_big_kernel(LmArgs A, BigArgs B)
         double bg = -kMaxFloat;
         int bc = kNone;
         if (!ovf) {
-            // each wavefront sweeps a quarter of the table, queueing occupied
-            // slots and scoring them 64 at a time
-            auto score = [&](int cnt, int qn) {
-                if (lane < cnt) {
-                    const int s = qk[lane];
-                    consider(A, m, ki, own, key[s], fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
-                }
-                if (lane < qn - cnt) qk[lane] = qk[cnt + lane];
-                wave_lds_sync();
-            };
-            const int quarter = ts / (kBlock / 64);
-            const int s_begin = (threadIdx.x >> 6) * quarter;
-            int qn = 0;
-            for (int s0 = s_begin; s0 < s_begin + quarter; s0 += 64) {
-                const bool occ = key[s0 + lane] >= 0;
-                const unsigned long long mask = __ballot(occ);
-                if (occ) qk[qn + lane_rank(mask)] = s0 + lane;
-                qn += __popcll(mask);
-                if (qn >= 64) {
-                    wave_lds_sync();
-                    score(64, qn);
-                    qn -= 64;
-                }
+            // score the occupied slots from the list, two per thread at a
+            // time with both a_c gathers in flight
+            const int nk = s_nkeys;
+            for (int i = threadIdx.x; i < nk; i += 2 * kBlock) {
+                const int s1 = klist[i];
+                const int s2 = i + kBlock < nk ? klist[i + kBlock] : -1;
+                const int c1 = key[s1];
+                const int c2 = s2 >= 0 ? key[s2] : -1;
+                const fx_t a1 = A.a[c1];
+                const fx_t a2 = c2 >= 0 ? A.a[c2] : fx_t(0);
+                consider_a(m, ki, own, c1, fx_make(lo[s1], (int64_t)hi[s1]), a1, self, bg, bc);
+                if (c2 >= 0) consider_a(m, ki, own, c2, fx_make(lo[s2], (int64_t)hi[s2]), a2, self, bg, bc);
             }
-            wave_lds_sync();
-            if (qn > 0) score(qn, qn);
         }
         block_reduce(R, self, bg, bc);
         const int64_t tb = B.toff[slot];
