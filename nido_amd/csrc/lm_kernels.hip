@@ -106,6 +106,26 @@ __device__ __forceinline__ void consider(const LmArgs& A, double m, double ki, i
     }
 }
 
+// Scores the K table slots key[j * stride] (j < K) of one lane: the a_c
+// gathers of all K are issued before the first is used (community 0 stands
+// in for an empty slot's, whose result is ignored).
+template <int K>
+__device__ __forceinline__ void sweep_slots(const LmArgs& A, double m, double ki, int own, const int* key,
+                                            const unsigned long long* lo, const unsigned long long* hi, int stride,
+                                            fx_t& self, double& bg, int& bc)
+{
+    int cs[K];
+    fx_t as[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) cs[j] = key[j * stride];
+#pragma unroll
+    for (int j = 0; j < K; ++j) as[j] = A.a[cs[j] >= 0 ? cs[j] : 0];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (cs[j] >= 0)
+            consider_a(m, ki, own, cs[j], fx_make(lo[j * stride], (int64_t)hi[j * stride]), as[j], self, bg, bc);
+}
+
 // Makes one lane's LDS writes visible to the other lanes of its wavefront.
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -243,10 +263,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
             fx_t self = 0;
             double bg = -kMaxFloat;
             int bc = kNone;
-            for (int s = lane; s < S; s += G) {
-                const int c = key[s];
-                if (c >= 0) consider(A, m, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
-            }
+            sweep_slots<S / G>(A, m, ki, own, key + lane, lo + lane, hi + lane, G, self, bg, bc);
 #pragma unroll
             for (int msk = G / 2; msk > 0; msk >>= 1) {
                 const double og = __shfl_xor(bg, msk, G);
@@ -290,10 +307,8 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
         fx_t self = 0;
         double bg = -kMaxFloat;
         int bc = kNone;
-        for (int s = threadIdx.x; s < S; s += kBlock) {
-            const int c = key[s];
-            if (c >= 0) consider(A, m, ki, own, c, fx_make(lo[s], (int64_t)hi[s]), self, bg, bc);
-        }
+        sweep_slots<S / kBlock>(A, m, ki, own, key + threadIdx.x, lo + threadIdx.x, hi + threadIdx.x, kBlock, self,
+                                bg, bc);
         block_reduce(R, self, bg, bc);
         if (threadIdx.x == 0) A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
     }
