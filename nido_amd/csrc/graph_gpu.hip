@@ -503,7 +503,7 @@ int setup_phase(nido_graph* g)
             if (k != BIN_BIG) continue;
             int P = 1;
             while (P < kMaxParts && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
-            for (int p = 0; p < P; ++p) bitems.push_back(BigItem{(int)v, p, P, slot, ksum, csum});
+            for (int p = 0; p < P; ++p) bitems.push_back(BigItem{(int)v, p, P, slot, ksum, csum, off[v], d});
             if (P > 1) {
                 for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs) {
                     chunks.push_back(KeyChunk{(int)v, P, e, std::min(off[v + 1], e + kKeyChunkArcs),
@@ -1349,12 +1349,15 @@ int do_autotune(nido_graph* g, int iters, nido_tuning* out)
     };
     for (int v : {4, 6, 8}) try_value([&](nido_tuning& t) { t.thread_max_deg = v; });
     for (int v : {128, 256, 512}) try_value([&](nido_tuning& t) { t.big_min_deg = v; });
-    for (int v : {768, 1536, 3072}) try_value([&](nido_tuning& t) { t.big_part_arcs = v; });
-    for (int v : {2048, 4096})
-        try_value([&](nido_tuning& t) {
-            t.big_slots = v;
-            t.big_key_limit = v / 2;
-        });
+    // table size and partition size move together: a partition of up to
+    // big_part_arcs arcs fits a table of twice as many slots
+    for (int v : {1024, 2048, 4096})
+        for (int part : {v / 2, 3 * v / 8})
+            try_value([&](nido_tuning& t) {
+                t.big_slots = v;
+                t.big_key_limit = v / 2;
+                t.big_part_arcs = part;
+            });
     for (int v : {0, 1}) try_value([&](nido_tuning& t) { t.overlap_long = v; });
     // small long-row tables beside the short rows on a second stream
     for (int v : {384, 512})

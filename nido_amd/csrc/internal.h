@@ -23,8 +23,8 @@ constexpr int NUM_SMALL_BINS = BIN_BIG;
 
 constexpr int kThreadMaxDeg = 8;          // registers hold up to this many arcs
 constexpr int64_t kBinMaxDeg[NUM_BINS] = {8, 16, 32, 64, 128, 512, INT64_MAX};
-constexpr int64_t kBigPartArcs = 1536;    // expected distinct keys per partition, at most
-constexpr int kBigSlots = 4096;           // LDS table of a partition workgroup (64/80 KB)
+constexpr int64_t kBigPartArcs = 1024;    // expected distinct keys per partition, at most
+constexpr int kBigSlots = 2048;           // LDS table of a partition workgroup (40/48 KB: 4 or 3 per CU)
 constexpr int kBigLimit = kBigSlots / 2;  // distinct keys before the exact HBM fallback
 
 // Arcs [e0, e1) of a long row v split into P > 1 partitions.  The key fill
@@ -95,6 +95,7 @@ struct SmallBins {
 struct BigItem {
     int v, p, P, slot;
     int64_t k0, c0;
+    int64_t e0, d;           // the row's first arc and length
 };
 
 struct BigArgs {

@@ -499,6 +499,37 @@ constexpr int kOverflow = -2; // partial community of a partition that outgrew i
 // chunks are scanned across the workgroup and the concatenated segments are
 // processed 256 entries at a time, each thread locating its chunk by binary
 // search in the scanned starts.  Every key a workgroup reads belongs to it.
+__device__ __forceinline__ int rfl(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t rfl64(int64_t x)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// An item read with vector loads: a scalar load would share its wait
+// counter with the LDS traffic of the item in progress, and every LDS wait
+// would then wait for the prefetch too.
+__device__ __forceinline__ BigItem load_item_v(const BigItem* items, int64_t i)
+{
+    asm volatile("" : "+v"(i));
+    return items[i];
+}
+
+// Segment start and length of partition x.p in chunk j of item x's row
+// (zero past the row's chunks or for an unpartitioned row).
+__device__ __forceinline__ void item_segment(const BigArgs& B, const BigItem& x, int j, int& l0, int& len)
+{
+    l0 = 0;
+    len = 0;
+    const int nch = x.P > 1 ? (int)((x.d + kKeyChunkArcs - 1) / kKeyChunkArcs) : 0;
+    if (j < nch) {
+        const int* cp = B.cpo + x.c0 + (int64_t)j * (x.P + 1) + x.p;
+        l0 = cp[0];
+        len = cp[1] - l0;
+    }
+}
+
 template <bool Narrow, int S, bool Multi>
 __global__ void __launch_bounds__(kBlock)
 lm_big_kernel(LmArgs A, BigArgs B)
@@ -514,26 +545,47 @@ lm_big_kernel(LmArgs A, BigArgs B)
     __shared__ int s_nkeys, s_ovf, s_last;
     const int lane = threadIdx.x & 63;
 
-    for (int64_t it = blockIdx.x; it < B.nitems; it += gridDim.x) {
-        const BigItem item = B.items[it];
-        const int v = item.v, p = item.p, P = item.P, slot = item.slot;
+    // Software pipeline over the workgroup's items: the next item's fields
+    // and its first group of segment starts are loaded while the current
+    // item runs, so an item starts with its key reads.
+    int64_t it = blockIdx.x;
+    BigItem nx{};
+    int n_l0 = 0, n_len = 0;
+    if (it < B.nitems) {
+        nx = load_item_v(B.items, it);
+        item_segment(B, nx, threadIdx.x, n_l0, n_len);
+    }
+    for (; it < B.nitems; it += gridDim.x) {
+        const int v = rfl(nx.v), p = rfl(nx.p), P = rfl(nx.P), slot = rfl(nx.slot);
+        const int64_t k0 = rfl64(nx.k0), c0 = rfl64(nx.c0), e0 = rfl64(nx.e0), d = rfl64(nx.d);
+        const int c_l0 = n_l0, c_len = n_len;
+        const int64_t itn = it + gridDim.x;
+        if (itn < B.nitems) nx = load_item_v(B.items, itn);
+        auto prefetch_next = [&]() {
+            if (itn < B.nitems) item_segment(B, nx, threadIdx.x, n_l0, n_len);
+        };
         double m;
-        if (!graph_mass<Multi>(A, v, m)) continue;   // uniform; every partition of the row skips it
+        if (!graph_mass<Multi>(A, v, m)) {   // uniform; every partition of the row skips it
+            prefetch_next();
+            continue;
+        }
         const int own = A.C[v];
         const double ki = A.kd[v];
-        const int64_t e0 = A.off[v];
-        const int64_t d = A.off[v + 1] - e0;
         const int nch = P > 1 ? (int)((d + kKeyChunkArcs - 1) / kKeyChunkArcs) : 0;
 
         // Segments of chunks [g * kBlock, g * kBlock + kBlock) of partition p:
         // s_pre = exclusive scan of their lengths, s_lo = their starts.
         auto scan_group = [&](int g) -> int {
             const int j = g * kBlock + threadIdx.x;
-            int l0 = 0, len = 0;
-            if (j < nch) {
-                const int* cp = B.cpo + item.c0 + (int64_t)j * (P + 1) + p;
-                l0 = cp[0];
-                len = cp[1] - l0;
+            int l0 = c_l0, len = c_len;
+            if (g > 0) {
+                l0 = 0;
+                len = 0;
+                if (j < nch) {
+                    const int* cp = B.cpo + c0 + (int64_t)j * (P + 1) + p;
+                    l0 = cp[0];
+                    len = cp[1] - l0;
+                }
             }
             int tot;
             const int pre = block_excl_scan_i32(len, wtot, &tot);
@@ -632,7 +684,7 @@ lm_big_kernel(LmArgs A, BigArgs B)
                     total = scan_group(g);
                 }
                 const int ng = min(kBlock, nch - g * kBlock);
-                const int64_t cbase = item.k0 + (int64_t)g * kBlock * kKeyChunkArcs;
+                const int64_t cbase = k0 + (int64_t)g * kBlock * kKeyChunkArcs;
                 int top = 1;                                 // largest power of two < ng (or 1)
                 while (2 * top < ng) top *= 2;
                 for (int i0 = 0; i0 < total; i0 += kFill * kBlock) {
@@ -667,6 +719,7 @@ lm_big_kernel(LmArgs A, BigArgs B)
             }
         }
         __syncthreads();
+        prefetch_next();
         const bool ovf = s_ovf != 0;
         fx_t self = 0;
         double bg = -kMaxFloat;
