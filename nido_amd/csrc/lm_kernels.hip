@@ -798,6 +798,45 @@ lm_big_kernel(LmArgs A, BigArgs B)
     }
 }
 
+// Community-total updates of a commit, aggregated per workgroup: the moved
+// vertices of a kBlock-vertex chunk add -k_v to their old and +k_v to their
+// new community in an LDS table, and each distinct community of the chunk
+// then takes one global atomic.  Rows moving into one popular community
+// would otherwise serialise on its a_c.  The sums are exact, so the totals
+// do not depend on the grouping.
+constexpr int kCommitSlots = 4 * kBlock;   // at most 2 * kBlock keys: load <= 1/2
+struct CommitTable {
+    int key[kCommitSlots];
+    unsigned long long lo[kCommitSlots];
+    unsigned long long hi[kCommitSlots];
+};
+
+__device__ __forceinline__ void ctab_clear(CommitTable& T)
+{
+#pragma unroll
+    for (int j = 0; j < kCommitSlots / kBlock; ++j) {
+        const int s = threadIdx.x + j * kBlock;
+        T.key[s] = -1;
+        T.lo[s] = 0;
+        T.hi[s] = 0;
+    }
+}
+
+__device__ __forceinline__ void ctab_add(CommitTable& T, int c, fx_t x)
+{
+    unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(kCommitSlots));
+    for (;;) {   // ends: the table never holds more than half its slots
+        const int kk = T.key[h];
+        if (kk == c) break;
+        if (kk < 0) {
+            const int prev = atomicCAS(&T.key[h], -1, c);
+            if (prev < 0 || prev == c) break;
+        }
+        h = (h + 1) & (kCommitSlots - 1);
+    }
+    fx_atomic_add(&T.lo[h], &T.hi[h], x);
+}
+
 // update_community_weights + update_community_ids of one batch, fused
 // (graph_cuda.cu:1234-1249 then 1191-1196): each thread touches only its own
 // vertex's C/C' entries, so the swap needs no second launch.  C32 mirrors C.
@@ -808,87 +847,106 @@ commit_kernel(const BinRow* __restrict__ list, int64_t v0, int64_t n, int64_t* _
               int64_t* __restrict__ Cn, int* __restrict__ C32, const fx_t* __restrict__ k,
               fx_t* __restrict__ a, const int* __restrict__ gid, const unsigned char* __restrict__ flag)
 {
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const int64_t v = list ? (int64_t)list[i].v : v0 + i;
-        if (gid && !flag[gid[v]]) continue;
-        const int64_t src = C[v];
-        const int64_t dst = Cn[v];
-        if (src != dst) {
-            const fx_t kv = k[v];
-            acc_add(a + src, -kv);
-            acc_add(a + dst, kv);
-            C32[v] = (int)dst;
+    __shared__ CommitTable T;
+    for (int64_t base = blockIdx.x * (int64_t)kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+        ctab_clear(T);
+        __syncthreads();
+        const int64_t i = base + threadIdx.x;
+        if (i < n) {
+            const int64_t v = list ? (int64_t)list[i].v : v0 + i;
+            if (!gid || flag[gid[v]]) {
+                const int64_t src = C[v];
+                const int64_t dst = Cn[v];
+                if (src != dst) {
+                    const fx_t kv = k[v];
+                    ctab_add(T, (int)src, -kv);
+                    ctab_add(T, (int)dst, kv);
+                    C32[v] = (int)dst;
+                }
+                C[v] = dst;
+                Cn[v] = src;
+            }
         }
-        C[v] = dst;
-        Cn[v] = src;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kCommitSlots / kBlock; ++j) {
+            const int s = threadIdx.x + j * kBlock;
+            const int c = T.key[s];
+            const fx_t x = fx_make(T.lo[s], (int64_t)T.hi[s]);
+            if (c >= 0 && x != 0) acc_add(a + c, x);
+        }
     }
 }
 
 // The commit plus the bookkeeping of the incremental modularity: a moved
 // vertex lists its row (in chunks) and its old and new community (each
 // community once per batch, through its touched flag).
-// Exclusive prefix of x over the wavefront and its total (every lane).
-__device__ __forceinline__ int wave_excl_scan(int x, int& total)
-{
-    const int lane = threadIdx.x & 63;
-    int inc = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-    }
-    total = __shfl(inc, 63, 64);
-    return inc - x;
-}
-
 __global__ void __launch_bounds__(kBlock)
 commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, int64_t* __restrict__ Cn,
                       int* __restrict__ C32, const fx_t* __restrict__ k, fx_t* __restrict__ a,
                       const int64_t* __restrict__ off, ModDelta D)
 {
-    const int lane = threadIdx.x & 63;
-    // wavefront-uniform loop so that list space is reserved once per wavefront
-    for (int64_t w0 = v0 + blockIdx.x * (int64_t)kBlock + (threadIdx.x & ~63); w0 < v1;
-         w0 += (int64_t)gridDim.x * kBlock) {
-        const int64_t v = w0 + lane;
-        int nchunk = 0, ncomm = 0;
-        int64_t src = 0, dst = 0, e0 = 0, e1 = 0;
-        bool fs = false, fd = false;
+    __shared__ CommitTable T;
+    __shared__ int wtot[kBlock / 64];
+    __shared__ int s_chunk0, s_comm0;
+    // workgroup-uniform loop: list space is reserved once per workgroup
+    // (one atomic on each shared counter, not one per wavefront)
+    for (int64_t base = v0 + blockIdx.x * (int64_t)kBlock; base < v1; base += (int64_t)gridDim.x * kBlock) {
+        ctab_clear(T);
+        __syncthreads();
+        const int64_t v = base + threadIdx.x;
+        int nchunk = 0;
+        int64_t e0 = 0, e1 = 0;
         if (v < v1) {
-            src = C[v];
-            dst = Cn[v];
+            const int64_t src = C[v];
+            const int64_t dst = Cn[v];
             if (src != dst) {
                 const fx_t kv = k[v];
-                acc_add(a + src, -kv);
-                acc_add(a + dst, kv);
+                ctab_add(T, (int)src, -kv);
+                ctab_add(T, (int)dst, kv);
                 C32[v] = (int)dst;
                 e0 = off[v];
                 e1 = off[v + 1];
                 nchunk = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
-                fs = atomicExch(&D.touched[src], 1) == 0;
-                fd = atomicExch(&D.touched[dst], 1) == 0;
-                ncomm = (int)fs + (int)fd;
             }
             C[v] = dst;
             Cn[v] = src;
         }
-        int tc, tm;
-        const int pc = wave_excl_scan(nchunk, tc);
-        const int pm = wave_excl_scan(ncomm, tm);
-        int bc = 0, bm = 0;
-        if (lane == 0) {
-            if (tc) bc = atomicAdd(&D.counts[2 * b], tc);
-            if (tm) bm = atomicAdd(&D.counts[2 * b + 1], tm);
-        }
-        bc = __shfl(bc, 0, 64);
-        bm = __shfl(bm, 0, 64);
+        int tc;
+        const int pc = block_excl_scan_i32(nchunk, wtot, &tc);
+        if (threadIdx.x == 0 && tc) s_chunk0 = atomicAdd(&D.counts[2 * b], tc);
+        __syncthreads();                  // also: the table is complete
         for (int i = 0; i < nchunk; ++i) {
             const int64_t s0 = e0 + (int64_t)i * kDeltaArcs;
-            D.chunks[bc + pc + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
+            D.chunks[s_chunk0 + pc + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
         }
-        int at = bm + pm;
-        if (fs) D.comms[at++] = (int)src;
-        if (fd) D.comms[at] = (int)dst;
+        // every community a moved vertex left or joined: its total, and a
+        // place in the touched list the first time in this batch
+        constexpr int kPer = kCommitSlots / kBlock;
+        int cs[kPer];
+        int nf = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int s = threadIdx.x + j * kBlock;
+            const int c = T.key[s];
+            cs[j] = -1;
+            if (c >= 0) {
+                const fx_t x = fx_make(T.lo[s], (int64_t)T.hi[s]);
+                if (x != 0) acc_add(a + c, x);
+                if (atomicExch(&D.touched[c], 1) == 0) {
+                    cs[j] = c;
+                    ++nf;
+                }
+            }
+        }
+        int tm;
+        int at = block_excl_scan_i32(nf, wtot, &tm);
+        if (threadIdx.x == 0 && tm) s_comm0 = atomicAdd(&D.counts[2 * b + 1], tm);
+        __syncthreads();
+        at += s_comm0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (cs[j] >= 0) D.comms[at++] = cs[j];
     }
 }
 
