@@ -106,24 +106,22 @@ __device__ __forceinline__ void consider(const LmArgs& A, double m, double ki, i
     }
 }
 
-// Scores the K table slots key[j * stride] (j < K) of one lane: the a_c
-// gathers of all K are issued before the first is used (community 0 stands
-// in for an empty slot's, whose result is ignored).
-template <int K>
-__device__ __forceinline__ void sweep_slots(const LmArgs& A, double m, double ki, int own, const int* key,
-                                            const unsigned long long* lo, const unsigned long long* hi, int stride,
-                                            fx_t& self, double& bg, int& bc)
+// Scores the listed slots of one row's table with L lanes: a row of a
+// short-row bin has at most 2L arcs, hence at most 2L keys, so each lane
+// takes at most two, with both a_c gathers in flight.
+template <int L>
+__device__ __forceinline__ void sweep_list(const LmArgs& A, double m, double ki, int own, const int* key,
+                                           const unsigned long long* lo, const unsigned long long* hi,
+                                           const int* klist, int nk, int lane, fx_t& self, double& bg, int& bc)
 {
-    int cs[K];
-    fx_t as[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) cs[j] = key[j * stride];
-#pragma unroll
-    for (int j = 0; j < K; ++j) as[j] = A.a[cs[j] >= 0 ? cs[j] : 0];
-#pragma unroll
-    for (int j = 0; j < K; ++j)
-        if (cs[j] >= 0)
-            consider_a(m, ki, own, cs[j], fx_make(lo[j * stride], (int64_t)hi[j * stride]), as[j], self, bg, bc);
+    const int s1 = lane < nk ? klist[lane] : -1;
+    const int s2 = lane + L < nk ? klist[lane + L] : -1;
+    const int c1 = s1 >= 0 ? key[s1] : -1;
+    const int c2 = s2 >= 0 ? key[s2] : -1;
+    const fx_t a1 = A.a[c1 >= 0 ? c1 : 0];
+    const fx_t a2 = A.a[c2 >= 0 ? c2 : 0];
+    if (c1 >= 0) consider_a(m, ki, own, c1, fx_make(lo[s1], (int64_t)hi[s1]), a1, self, bg, bc);
+    if (c2 >= 0) consider_a(m, ki, own, c2, fx_make(lo[s2], (int64_t)hi[s2]), a2, self, bg, bc);
 }
 
 // Makes one lane's LDS writes visible to the other lanes of its wavefront.
@@ -145,7 +143,7 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask)
 // 2, never more than half full by construction of the bins).
 template <int S>
 __device__ __forceinline__ void lds_insert(int* key, unsigned long long* lo, unsigned long long* hi,
-                                           int c, fx_t wf)
+                                           int* nkeys, int* klist, int c, fx_t wf)
 {
     unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(S));
     for (;;) {
@@ -153,7 +151,11 @@ __device__ __forceinline__ void lds_insert(int* key, unsigned long long* lo, uns
         if (kk == c) break;
         if (kk < 0) {
             const int prev = atomicCAS(&key[h], -1, c);
-            if (prev < 0 || prev == c) break;
+            if (prev < 0) {
+                klist[atomicAdd(nkeys, 1)] = (int)h;   // a new key: list its slot
+                break;
+            }
+            if (prev == c) break;
         }
         h = (h + 1) & (S - 1);
     }
@@ -161,10 +163,11 @@ __device__ __forceinline__ void lds_insert(int* key, unsigned long long* lo, uns
 }
 
 // Scans row v with `lanes` lanes (lane index `lane`), two arcs per lane per
-// step, inserting every non-self arc into the S-slot table.
+// step, inserting every non-self arc into the S-slot table; the slots of new
+// keys are appended to klist (nkeys counts them).
 template <int S>
 __device__ __forceinline__ void scan_row(const LmArgs& A, int v, int64_t e0, int64_t e1, int lane, int lanes, int* key,
-                                         unsigned long long* lo, unsigned long long* hi)
+                                         unsigned long long* lo, unsigned long long* hi, int* nkeys, int* klist)
 {
     for (int64_t e = e0 + lane; e < e1; e += 2 * (int64_t)lanes) {
         const int64_t f = e + lanes;
@@ -175,8 +178,8 @@ __device__ __forceinline__ void scan_row(const LmArgs& A, int v, int64_t e0, int
         const double w2 = two ? A.w[f] : 0.0;
         const int c1 = u1 != v ? (int)A.C[u1] : -1;     // self arcs weigh 0 (graph_cuda.cu:2312-2315)
         const int c2 = u2 != v ? (int)A.C[u2] : -1;
-        if (c1 >= 0) lds_insert<S>(key, lo, hi, c1, fx_from_double(w1));
-        if (c2 >= 0) lds_insert<S>(key, lo, hi, c2, fx_from_double(w2));
+        if (c1 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c1, fx_from_double(w1));
+        if (c2 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c2, fx_from_double(w2));
     }
 }
 
@@ -229,8 +232,9 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
 // per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
 template <int G, int S, bool Multi>
 __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
-                           char* lds)
+                           char* lds, int* kcnt, int* klist_all)
 {
+    static_assert(S == 4 * G, "a group's key list holds 2G slots");
     constexpr int NG = kBlock / G;
     int* skey = reinterpret_cast<int*>(lds);
     unsigned long long* slo = reinterpret_cast<unsigned long long*>(lds + NG * S * 4);
@@ -240,6 +244,8 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
     int* key = skey + grp * S;
     unsigned long long* lo = slo + grp * S;
     unsigned long long* hi = shi + grp * S;
+    int* const klist = klist_all + grp * 2 * G;
+    int* const nkeys = kcnt + grp;
 
     for (int64_t base = (int64_t)bi * NG; base < n; base += (int64_t)nbk * NG) {
         for (int s = lane; s < S; s += G) {
@@ -247,6 +253,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
             lo[s] = 0;
             hi[s] = 0;
         }
+        if (lane == 0) *nkeys = 0;
         const int64_t idx = base + grp;
         bool act = idx < n;
         const BinRow r = act ? verts[idx] : BinRow{0, 0, 0};
@@ -257,13 +264,13 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
         const double ki = act ? A.kd[v] : 0.0;
         const fx_t rest = act ? A.a[own] - A.k[v] : fx_t(0);
         wave_lds_sync();
-        if (act) scan_row<S>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi);
+        if (act) scan_row<S>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi, nkeys, klist);
         wave_lds_sync();
         if (act) {   // uniform within the group
             fx_t self = 0;
             double bg = -kMaxFloat;
             int bc = kNone;
-            sweep_slots<S / G>(A, m, ki, own, key + lane, lo + lane, hi + lane, G, self, bg, bc);
+            sweep_list<G>(A, m, ki, own, key, lo, hi, klist, *nkeys, lane, self, bg, bc);
 #pragma unroll
             for (int msk = G / 2; msk > 0; msk >>= 1) {
                 const double og = __shfl_xor(bg, msk, G);
@@ -283,8 +290,9 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
 // One workgroup per vertex, S-slot LDS table.
 template <int S, bool Multi>
 __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
-                           char* lds, BlockRed& R)
+                           char* lds, BlockRed& R, int* kcnt, int* klist)
 {
+    static_assert(S == 4 * kBlock, "the key list holds 2 * kBlock slots");
     int* key = reinterpret_cast<int*>(lds);
     unsigned long long* lo = reinterpret_cast<unsigned long long*>(lds + S * 4);
     unsigned long long* hi = lo + S;
@@ -294,6 +302,7 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
             lo[s] = 0;
             hi[s] = 0;
         }
+        if (threadIdx.x == 0) *kcnt = 0;
         const BinRow r = verts[idx];
         const int v = r.v;
         double m;
@@ -302,13 +311,12 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
         const double ki = A.kd[v];
         const fx_t rest = A.a[own] - A.k[v];
         __syncthreads();
-        scan_row<S>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi);
+        scan_row<S>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi, kcnt, klist);
         __syncthreads();
         fx_t self = 0;
         double bg = -kMaxFloat;
         int bc = kNone;
-        sweep_slots<S / kBlock>(A, m, ki, own, key + threadIdx.x, lo + threadIdx.x, hi + threadIdx.x, kBlock, self,
-                                bg, bc);
+        sweep_list<kBlock>(A, m, ki, own, key, lo, hi, klist, *kcnt, threadIdx.x, self, bg, bc);
         block_reduce(R, self, bg, bc);
         if (threadIdx.x == 0) A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
     }
@@ -397,6 +405,8 @@ lm_small_kernel(LmArgs A, SmallBins L)
 {
     __shared__ __align__(16) char lds[kSmallLds];
     __shared__ BlockRed R;
+    __shared__ int klist[2 * kBlock];          // slots of the keys of the rows in flight
+    __shared__ int kcnt[kBlock / 8];           // keys per row (one counter per group)
     // Every index into L is a constant (a dynamically indexed kernel
     // argument would be copied to scratch memory).
     int bin = 0;   // 0..5 short-row bins, 6 key fill, 7 the previous batch's modularity delta
@@ -406,11 +416,11 @@ lm_small_kernel(LmArgs A, SmallBins L)
 #define NIDO_BIN_ARGS(K) L.verts[K], L.n[K], b - L.blk0[K], L.blk0[K + 1] - L.blk0[K]
     switch (bin) {
     case BIN_THREAD: thread_body<Multi>(A, NIDO_BIN_ARGS(BIN_THREAD)); break;
-    case BIN_G8: group_body<8, 32, Multi>(A, NIDO_BIN_ARGS(BIN_G8), lds); break;
-    case BIN_G16: group_body<16, 64, Multi>(A, NIDO_BIN_ARGS(BIN_G16), lds); break;
-    case BIN_G32: group_body<32, 128, Multi>(A, NIDO_BIN_ARGS(BIN_G32), lds); break;
-    case BIN_G64: group_body<64, 256, Multi>(A, NIDO_BIN_ARGS(BIN_G64), lds); break;
-    case BIN_B256: block_body<1024, Multi>(A, NIDO_BIN_ARGS(BIN_B256), lds, R); break;
+    case BIN_G8: group_body<8, 32, Multi>(A, NIDO_BIN_ARGS(BIN_G8), lds, kcnt, klist); break;
+    case BIN_G16: group_body<16, 64, Multi>(A, NIDO_BIN_ARGS(BIN_G16), lds, kcnt, klist); break;
+    case BIN_G32: group_body<32, 128, Multi>(A, NIDO_BIN_ARGS(BIN_G32), lds, kcnt, klist); break;
+    case BIN_G64: group_body<64, 256, Multi>(A, NIDO_BIN_ARGS(BIN_G64), lds, kcnt, klist); break;
+    case BIN_B256: block_body<1024, Multi>(A, NIDO_BIN_ARGS(BIN_B256), lds, R, kcnt, klist); break;
     case NUM_SMALL_BINS:
         keyfill_body<Multi>(A, L, b - L.blk0[NUM_SMALL_BINS], L.blk0[NUM_SMALL_BINS + 1] - L.blk0[NUM_SMALL_BINS],
                             lds);
