@@ -155,6 +155,35 @@ struct BlockRed {
     unsigned long long shi[kBlock / 64];
 };
 
+// Workgroup-wide argmax alone; result valid in thread 0.
+__device__ __forceinline__ void block_argmax(BlockRed& R, double& bg, int& bc)
+{
+#pragma unroll
+    for (int msk = 32; msk > 0; msk >>= 1) {
+        const double og = __shfl_xor(bg, msk, 64);
+        const int oc = __shfl_xor(bc, msk, 64);
+        if (better(og, oc, bg, bc)) {
+            bg = og;
+            bc = oc;
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        R.g[wv] = bg;
+        R.c[wv] = bc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kBlock / 64; ++i) {
+            if (better(R.g[i], R.c[i], bg, bc)) {
+                bg = R.g[i];
+                bc = R.c[i];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ void block_reduce(BlockRed& R, fx_t& self, double& bg, int& bc)
 {
 #pragma unroll

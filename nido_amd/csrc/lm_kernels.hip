@@ -55,17 +55,24 @@ __device__ __forceinline__ bool graph_mass(const LmArgs& A, int v, double& m)
 // Move iff g_best - s' > 0 with s' = (e_{v,own} - k_v (a_own - k_v)/(2m))/m,
 // graph_cuda.cu:1527-1536.
 // `rest` = a_own - k_v, loaded by the caller ahead of the row scan.
-__device__ __forceinline__ int64_t decide(double m, double ki, fx_t rest, int own, fx_t self,
-                                          double best_g, int best_c)
+// `self_d` = e_{v,own} rounded to FP64.
+__device__ __forceinline__ int64_t decide_d(double m, double ki, fx_t rest, int own, double self_d,
+                                            double best_g, int best_c)
 {
     if (best_c == kNone) return own;    // gain stays -MAX_FLOAT, graph_cuda.cu:1489
-    double s = fx_to_double(self);
+    double s = self_d;
     const double ar = fx_to_double(rest);
     s -= ki * ar / (2. * m);
     s /= m;
     double gain = best_g;
     gain -= s;
     return gain > 0 ? (int64_t)best_c : (int64_t)own;
+}
+
+__device__ __forceinline__ int64_t decide(double m, double ki, fx_t rest, int own, fx_t self, double best_g,
+                                          int best_c)
+{
+    return decide_d(m, ki, rest, own, fx_to_double(self), best_g, best_c);
 }
 
 __device__ __forceinline__ int64_t decide(const LmArgs& A, double m, int v, int own, fx_t self,
@@ -108,11 +115,13 @@ __device__ __forceinline__ void consider(const LmArgs& A, double m, double ki, i
 
 // Scores the listed slots of one row's table with L lanes: a row of a
 // short-row bin has at most 2L arcs, hence at most 2L keys, so each lane
-// takes at most two, with both a_c gathers in flight.
+// takes at most two, with both a_c gathers in flight.  The lane holding the
+// row's own community stores e_{v,own} (rounded) in *self_d, so only the
+// argmax needs a cross-lane reduction.
 template <int L>
 __device__ __forceinline__ void sweep_list(const LmArgs& A, double m, double ki, int own, const int* key,
                                            const unsigned long long* lo, const unsigned long long* hi,
-                                           const int* klist, int nk, int lane, fx_t& self, double& bg, int& bc)
+                                           const int* klist, int nk, int lane, double* self_d, double& bg, int& bc)
 {
     const int s1 = lane < nk ? klist[lane] : -1;
     const int s2 = lane + L < nk ? klist[lane + L] : -1;
@@ -120,8 +129,10 @@ __device__ __forceinline__ void sweep_list(const LmArgs& A, double m, double ki,
     const int c2 = s2 >= 0 ? key[s2] : -1;
     const fx_t a1 = A.a[c1 >= 0 ? c1 : 0];
     const fx_t a2 = A.a[c2 >= 0 ? c2 : 0];
+    fx_t self = 0;
     if (c1 >= 0) consider_a(m, ki, own, c1, fx_make(lo[s1], (int64_t)hi[s1]), a1, self, bg, bc);
     if (c2 >= 0) consider_a(m, ki, own, c2, fx_make(lo[s2], (int64_t)hi[s2]), a2, self, bg, bc);
+    if (c1 == own || c2 == own) *self_d = fx_to_double(self);
 }
 
 // Makes one lane's LDS writes visible to the other lanes of its wavefront.
@@ -232,7 +243,7 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
 // per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
 template <int G, int S, bool Multi>
 __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
-                           char* lds, int* kcnt, int* klist_all)
+                           char* lds, int* kcnt, int* klist_all, double* sself)
 {
     static_assert(S == 4 * G, "a group's key list holds 2G slots");
     constexpr int NG = kBlock / G;
@@ -246,6 +257,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
     unsigned long long* hi = shi + grp * S;
     int* const klist = klist_all + grp * 2 * G;
     int* const nkeys = kcnt + grp;
+    double* const self_d = sself + grp;
 
     for (int64_t base = (int64_t)bi * NG; base < n; base += (int64_t)nbk * NG) {
         for (int s = lane; s < S; s += G) {
@@ -253,7 +265,10 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
             lo[s] = 0;
             hi[s] = 0;
         }
-        if (lane == 0) *nkeys = 0;
+        if (lane == 0) {
+            *nkeys = 0;
+            *self_d = 0.0;
+        }
         const int64_t idx = base + grp;
         bool act = idx < n;
         const BinRow r = act ? verts[idx] : BinRow{0, 0, 0};
@@ -267,21 +282,20 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
         if (act) scan_row<S>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi, nkeys, klist);
         wave_lds_sync();
         if (act) {   // uniform within the group
-            fx_t self = 0;
             double bg = -kMaxFloat;
             int bc = kNone;
-            sweep_list<G>(A, m, ki, own, key, lo, hi, klist, *nkeys, lane, self, bg, bc);
+            sweep_list<G>(A, m, ki, own, key, lo, hi, klist, *nkeys, lane, self_d, bg, bc);
 #pragma unroll
             for (int msk = G / 2; msk > 0; msk >>= 1) {
                 const double og = __shfl_xor(bg, msk, G);
                 const int oc = __shfl_xor(bc, msk, G);
-                self += fx_shfl_xor(self, msk, G);
                 if (better(og, oc, bg, bc)) {
                     bg = og;
                     bc = oc;
                 }
             }
-            if (lane == 0) A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
+            wave_lds_sync();
+            if (lane == 0) A.Cn[v] = decide_d(m, ki, rest, own, *self_d, bg, bc);
         }
         wave_lds_sync();
     }
@@ -290,7 +304,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
 // One workgroup per vertex, S-slot LDS table.
 template <int S, bool Multi>
 __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
-                           char* lds, BlockRed& R, int* kcnt, int* klist)
+                           char* lds, BlockRed& R, int* kcnt, int* klist, double* self_d)
 {
     static_assert(S == 4 * kBlock, "the key list holds 2 * kBlock slots");
     int* key = reinterpret_cast<int*>(lds);
@@ -302,7 +316,10 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
             lo[s] = 0;
             hi[s] = 0;
         }
-        if (threadIdx.x == 0) *kcnt = 0;
+        if (threadIdx.x == 0) {
+            *kcnt = 0;
+            *self_d = 0.0;
+        }
         const BinRow r = verts[idx];
         const int v = r.v;
         double m;
@@ -313,12 +330,11 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
         __syncthreads();
         scan_row<S>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi, kcnt, klist);
         __syncthreads();
-        fx_t self = 0;
         double bg = -kMaxFloat;
         int bc = kNone;
-        sweep_list<kBlock>(A, m, ki, own, key, lo, hi, klist, *kcnt, threadIdx.x, self, bg, bc);
-        block_reduce(R, self, bg, bc);
-        if (threadIdx.x == 0) A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
+        sweep_list<kBlock>(A, m, ki, own, key, lo, hi, klist, *kcnt, threadIdx.x, self_d, bg, bc);
+        block_argmax(R, bg, bc);         // its barriers also publish *self_d
+        if (threadIdx.x == 0) A.Cn[v] = decide_d(m, ki, rest, own, *self_d, bg, bc);
     }
 }
 
@@ -407,6 +423,7 @@ lm_small_kernel(LmArgs A, SmallBins L)
     __shared__ BlockRed R;
     __shared__ int klist[2 * kBlock];          // slots of the keys of the rows in flight
     __shared__ int kcnt[kBlock / 8];           // keys per row (one counter per group)
+    __shared__ double sself[kBlock / 8];       // e_{v,own} per row, rounded
     // Every index into L is a constant (a dynamically indexed kernel
     // argument would be copied to scratch memory).
     int bin = 0;   // 0..5 short-row bins, 6 key fill, 7 the previous batch's modularity delta
@@ -416,11 +433,11 @@ lm_small_kernel(LmArgs A, SmallBins L)
 #define NIDO_BIN_ARGS(K) L.verts[K], L.n[K], b - L.blk0[K], L.blk0[K + 1] - L.blk0[K]
     switch (bin) {
     case BIN_THREAD: thread_body<Multi>(A, NIDO_BIN_ARGS(BIN_THREAD)); break;
-    case BIN_G8: group_body<8, 32, Multi>(A, NIDO_BIN_ARGS(BIN_G8), lds, kcnt, klist); break;
-    case BIN_G16: group_body<16, 64, Multi>(A, NIDO_BIN_ARGS(BIN_G16), lds, kcnt, klist); break;
-    case BIN_G32: group_body<32, 128, Multi>(A, NIDO_BIN_ARGS(BIN_G32), lds, kcnt, klist); break;
-    case BIN_G64: group_body<64, 256, Multi>(A, NIDO_BIN_ARGS(BIN_G64), lds, kcnt, klist); break;
-    case BIN_B256: block_body<1024, Multi>(A, NIDO_BIN_ARGS(BIN_B256), lds, R, kcnt, klist); break;
+    case BIN_G8: group_body<8, 32, Multi>(A, NIDO_BIN_ARGS(BIN_G8), lds, kcnt, klist, sself); break;
+    case BIN_G16: group_body<16, 64, Multi>(A, NIDO_BIN_ARGS(BIN_G16), lds, kcnt, klist, sself); break;
+    case BIN_G32: group_body<32, 128, Multi>(A, NIDO_BIN_ARGS(BIN_G32), lds, kcnt, klist, sself); break;
+    case BIN_G64: group_body<64, 256, Multi>(A, NIDO_BIN_ARGS(BIN_G64), lds, kcnt, klist, sself); break;
+    case BIN_B256: block_body<1024, Multi>(A, NIDO_BIN_ARGS(BIN_B256), lds, R, kcnt, klist, sself); break;
     case NUM_SMALL_BINS:
         keyfill_body<Multi>(A, L, b - L.blk0[NUM_SMALL_BINS], L.blk0[NUM_SMALL_BINS + 1] - L.blk0[NUM_SMALL_BINS],
                             lds);
