@@ -311,6 +311,9 @@ def main():
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="time the steps without per-kernel HIP events (roofline fields are then null)")
     ap.add_argument("--tuning", default="", help="comma list of nido_tuning knobs, e.g. big_slots=2048")
+    ap.add_argument("--event-stride", type=int, default=3,
+                    help="HIP events around every N-th launch of the dominant kernel class in the timed "
+                         "steps (rotating over the batches; 1 = every launch)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device copies of BYTES each (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a rocprofv3 --pmc pass) and exit")
@@ -380,15 +383,17 @@ def main():
         step()
     # The warm-up steps carry events around every kernel class: they name the
     # dominant class.  The timed steps bracket only that class's launches
-    # with events (on the stream they run on), which keeps the event overhead
-    # out of the other ~2/3 of the launches.
+    # with events (on the stream they run on), and only every
+    # --event-stride-th of them: a stride coprime with the batch count
+    # rotates over the batches, so every batch is sampled equally often,
+    # while the GPU time an event pair costs stays out of most launches.
     g.sync()
     g.set_profiling(True)
     for _ in range(max(1, min(args.warmup, 2))):
         step()
     probe = g.kernel_stats()
     dom = max(probe, key=lambda k: probe[k]["ms"])
-    g.set_profiling(not args.no_kernel_events, classes=[dom])
+    g.set_profiling(not args.no_kernel_events, classes=[dom], stride=args.event_stride)
     g.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -430,7 +435,8 @@ def main():
             "bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom) if default_workload else None,
-            "launches": d["launches"], "avg_launch_us": round(d["ms"] * 1e3 / max(d["launches"], 1), 3),
+            "launches": d["launches"], "event_stride": args.event_stride,
+            "avg_launch_us": round(d["ms"] * 1e3 / max(d["launches"], 1), 3),
             "algorithmic_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1), 1),
         },
         "roofline_iter": {

@@ -215,6 +215,11 @@ int nido_louvain_iteration(nido_graph* g, double* q, double* dq, double* q_eval)
 int nido_graph_set_profiling(nido_graph* g, int enable);
 /* Events only around the classes whose bit (1 << class) is set in mask. */
 int nido_graph_set_profiling_classes(nido_graph* g, unsigned mask);
+/* Events only around every stride-th launch of each profiled class (1, the
+ * default, brackets every launch).  A stride coprime with the batch count
+ * samples every batch equally often; each event pair costs a few
+ * microseconds of GPU time. */
+int nido_graph_set_profiling_stride(nido_graph* g, int stride);
 /* Launch count, summed duration (ms) and algorithmic HBM bytes of one class
  * since profiling was enabled (24 B per arc + 40 B per vertex for local
  * moving, 24 B per arc + 24 B per vertex for modularity, 0 for commits). */

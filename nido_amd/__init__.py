@@ -115,6 +115,7 @@ _sig("nido_dump_partition", C.c_int, C.c_void_p, C.c_char_p)
 _sig("nido_louvain_iteration", C.c_int, C.c_void_p, _pdbl, _pdbl, _pdbl)
 _sig("nido_graph_set_profiling", C.c_int, C.c_void_p, C.c_int)
 _sig("nido_graph_set_profiling_classes", C.c_int, C.c_void_p, C.c_uint)
+_sig("nido_graph_set_profiling_stride", C.c_int, C.c_void_p, C.c_int)
 _sig("nido_graph_kernel_stats", C.c_int, C.c_void_p, C.c_int, _pi64, _pdbl, _pdbl)
 _sig("nido_louvain_run", C.c_int, C.c_void_p, _i64, C.c_double, C.c_double, C.c_int, C.c_int,
      C.POINTER(RunStats))
@@ -380,8 +381,10 @@ class GraphGPU:
                "nido_louvain_iteration")
         return qq.value, dq.value, qe.value
 
-    def set_profiling(self, on: bool, classes=None) -> None:
-        """classes: names from KERNEL_CLASSES to bracket with events (all by default)."""
+    def set_profiling(self, on: bool, classes=None, stride: int = 1) -> None:
+        """classes: names from KERNEL_CLASSES to bracket with events (all by
+        default); stride: events on every stride-th launch of each class."""
+        _check(lib.nido_graph_set_profiling_stride(self._h, int(stride)), "nido_graph_set_profiling_stride")
         if classes is None:
             _check(lib.nido_graph_set_profiling(self._h, int(on)), "nido_graph_set_profiling")
         else:

@@ -151,6 +151,8 @@ struct Prof {
     };
     bool on = false;
     unsigned mask = ~0u;               // kernel classes that get events
+    int stride = 1;                    // events on every stride-th launch of a class
+    int64_t seq[NIDO_K_CLASSES] = {};
     std::vector<hipEvent_t> pool;
     std::vector<Entry> pending;
     int64_t launches[NIDO_K_CLASSES] = {};
@@ -190,6 +192,7 @@ struct Prof {
             launches[i] = 0;
             ms[i] = 0;
             bytes[i] = 0;
+            seq[i] = 0;
         }
     }
     ~Prof()
@@ -211,7 +214,8 @@ struct ProfScope {
     hipEvent_t a = nullptr;
     ProfScope(Prof& p, hipStream_t st, int c, double b) : P(p), s(st), cls(c), bytes(b)
     {
-        if (P.on && ((P.mask >> c) & 1u) && (a = P.get())) (void)hipEventRecord(a, s);
+        if (P.on && ((P.mask >> c) & 1u) && P.seq[c]++ % P.stride == 0 && (a = P.get()))
+            (void)hipEventRecord(a, s);
     }
     ~ProfScope()
     {
@@ -1732,6 +1736,17 @@ int nido_graph_set_profiling_classes(nido_graph* g, unsigned mask)
     g->prof.reset();
     g->prof.on = mask != 0;
     g->prof.mask = mask;
+    return NIDO_OK;
+}
+
+int nido_graph_set_profiling_stride(nido_graph* g, int stride)
+{
+    if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (stride < 1) return set_error(NIDO_ERR_INVALID, "profiling stride must be >= 1");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    API_GUARD(g->prof.harvest();)
+    g->prof.reset();
+    g->prof.stride = stride;
     return NIDO_OK;
 }
 

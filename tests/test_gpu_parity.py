@@ -238,3 +238,31 @@ def test_cli_matches_oracle(golden_dir, tmp_path):
     off, col, w = na.Graph.read(src).arrays()
     ref = run_oracle(off, col, w, 2)
     assert np.array_equal(got, ref["clustering"])
+
+
+def test_profiling_events_sample_every_stride_th_launch():
+    """Per-class HIP-event timing: every launch with stride 1, every third
+    with stride 3 (the bench's default), and the results are unchanged."""
+    na = _na()
+    off, col, w = na.gen_rmat(14, 16, 0.57, 0.19, 0.19, True, 5).arrays()
+    nb = 8
+    print(f"case: profiling strides, rmat14 -b {nb}", flush=True)
+    runs = []
+    for stride in (1, 3):
+        g = na.GraphGPU((off, col, w), nb)
+        g.singleton_partition()
+        q = g.compute_modularity()
+        g.set_profiling(True, classes=["lm_short", "lm_long"], stride=stride)
+        qa, dq, qe = g.louvain_iteration(q)
+        ks = g.kernel_stats()
+        assert ks["lm_short"]["launches"] == -(-nb // stride)
+        assert ks["lm_short"]["ms"] > 0 and ks["lm_short"]["bytes"] > 0
+        assert ks["commit"]["launches"] == 0
+        runs.append((qa, dq, qe, g.communities()))
+        g.close()
+    assert runs[0][:3] == runs[1][:3]
+    assert np.array_equal(runs[0][3], runs[1][3])
+    g = na.GraphGPU((off, col, w), nb)
+    with pytest.raises(na.NidoError):
+        g.set_profiling(True, stride=0)
+    g.close()
