@@ -266,3 +266,58 @@ def test_profiling_events_sample_every_stride_th_launch():
     with pytest.raises(na.NidoError):
         g.set_profiling(True, stride=0)
     g.close()
+
+
+def _hubs_at_bin_edges(seed=17):
+    """Hubs whose degrees sit on both sides of every bin and partition edge,
+    each joined to its own leaves (so in the first iteration every hub sees
+    d distinct communities: the tables' fullest case), plus a sparse random
+    layer among the leaves; U(0,1] weights on the 2^-53 grid."""
+    rng = np.random.default_rng(seed)
+    degs = [8, 9, 16, 17, 32, 33, 64, 65, 128, 129, 256, 257, 511, 512, 513, 1023, 1024, 1025, 2048, 2049, 4097]
+    nh = len(degs)
+    n = nh + sum(degs)
+    edges = set()
+    nxt = nh
+    for h, d in enumerate(degs):
+        for j in range(d):
+            edges.add((h, nxt + j))
+        nxt += d
+    leaves = np.arange(nh, n)
+    for _ in range(3 * n):
+        a, b = rng.choice(leaves, 2, replace=False)
+        edges.add((min(a, b), max(a, b)))
+    e = np.array(sorted(edges), dtype=np.int64)
+    wt = (np.int64(1 << 53) - rng.integers(0, 1 << 53, len(e))) * 2.0 ** -53
+    src = np.concatenate([e[:, 0], e[:, 1]])
+    dst = np.concatenate([e[:, 1], e[:, 0]])
+    ww = np.concatenate([wt, wt])
+    o = np.lexsort((dst, src))
+    src, dst, ww = src[o], dst[o], ww[o]
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(off, src + 1, 1)
+    return np.cumsum(off), dst, ww
+
+
+@pytest.mark.parametrize("knobs", [{}, {"big_slots": 1024, "big_part_arcs": 512, "big_key_limit": 512},
+                                   {"big_slots": 4096, "big_part_arcs": 2048, "big_key_limit": 2048}])
+def test_rows_at_bin_and_partition_edges(knobs):
+    """Every short-row bin at its capacity (keys = 2 x lanes) and long rows
+    just below and above each partition boundary, under three table sizes."""
+    na = _na()
+    off, col, w = _hubs_at_bin_edges()
+    nb = 3
+    print(f"case: hubs at bin edges, -b {nb}, {knobs}", flush=True)
+    ref = run_oracle(off, col, w, nb)
+    g = na.GraphGPU((off, col, w), nb)
+    if knobs:
+        g.set_tuning(**knobs)
+    st = na.LouvainGPU(500, 1e-6, nb).run(g, record=True)
+    got = {"phases": [], "q": st.q, "clustering": g.clustering()}
+    for i in range(st.phases):
+        p = g.phase(i).as_dict()
+        p["iter_q"] = g.iteration_q(i)
+        p["comm"] = g.phase_communities(i)
+        got["phases"].append(p)
+    g.close()
+    assert_runs_equal(ref, got)
