@@ -507,8 +507,10 @@ int setup_phase(nido_graph* g)
             if (k != BIN_BIG) continue;
             int P = 1;
             while (P < kMaxParts && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
-            for (int p = 0; p < P; ++p) bitems.push_back(BigItem{(int)v, p, P, slot, ksum, csum, off[v], d});
-            if (P > 1) {
+            const bool direct = P <= kDirectParts;
+            for (int p = 0; p < P; ++p)
+                bitems.push_back(BigItem{(int)v, p, P, slot, ksum, direct ? -1 : csum, off[v], d});
+            if (!direct) {
                 for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs) {
                     chunks.push_back(KeyChunk{(int)v, P, e, std::min(off[v + 1], e + kKeyChunkArcs),
                                               ksum + (e - off[v]), csum});

@@ -16,12 +16,13 @@
 //               P = 2^ceil(log2(d / part_arcs)) partitions, so that every
 //               e_{v,c} is complete and exact in one workgroup's LDS table;
 //               the last of the P workgroups to finish combines the P
-//               partial argmaxes.  For P > 1, launch 1 buckets the row's
+//               partial argmaxes.  For P > 2, launch 1 buckets the row's
 //               arcs by partition (a counting sort per 2048-arc chunk) and
-//               workgroup (v, p) of launch 2 reads only its own buckets.
-// Launch 1 = short-row bins + long-row key fill (20 KB LDS per workgroup);
-// launch 2 = long-row partitions (64-80 KB LDS).  Row loops keep two to eight
-// independent loads in flight per lane.
+//               workgroup (v, p) of launch 2 reads only its own buckets;
+//               with P <= 2 each workgroup reads the row and filters it.
+// Launch 1 = short-row bins + long-row key fill (23 KB LDS per workgroup);
+// launch 2 = long-row partitions (40 KB LDS with the default 2048-slot
+// table).  Row loops keep two to eight independent loads in flight per lane.
 #include "device_common.h"
 
 namespace nido {
@@ -516,16 +517,6 @@ __device__ __forceinline__ void lds_add(unsigned long long* lo, int* hi, fx_t w)
 constexpr int kFill = 6;      // entries per thread loaded ahead of their inserts
 constexpr int kOverflow = -2; // partial community of a partition that outgrew its table
 
-// S-slot LDS table per workgroup.  Narrow: 32-bit high words (S*16 bytes),
-// used when every row's weights are non-negative and sum to less than 2^31.
-// Items come longest row first, so the long poles start first.
-//
-// A row with P == 1 is read straight from the CSR.  For P > 1 the key fill
-// has bucketed each chunk of the row by partition, so workgroup (v, p) reads
-// only its own segment of every chunk: the segment lengths of up to 256
-// chunks are scanned across the workgroup and the concatenated segments are
-// processed 256 entries at a time, each thread locating its chunk by binary
-// search in the scanned starts.  Every key a workgroup reads belongs to it.
 __device__ __forceinline__ int rfl(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ int64_t rfl64(int64_t x)
 {
@@ -549,7 +540,7 @@ __device__ __forceinline__ void item_segment(const BigArgs& B, const BigItem& x,
 {
     l0 = 0;
     len = 0;
-    const int nch = x.P > 1 ? (int)((x.d + kKeyChunkArcs - 1) / kKeyChunkArcs) : 0;
+    const int nch = x.c0 >= 0 ? (int)((x.d + kKeyChunkArcs - 1) / kKeyChunkArcs) : 0;
     if (j < nch) {
         const int* cp = B.cpo + x.c0 + (int64_t)j * (x.P + 1) + x.p;
         l0 = cp[0];
@@ -557,6 +548,17 @@ __device__ __forceinline__ void item_segment(const BigArgs& B, const BigItem& x,
     }
 }
 
+// S-slot LDS table per workgroup.  Narrow: 32-bit high words (S*16 bytes),
+// used when every row's weights are non-negative and sum to less than 2^31.
+// Items come longest row first, so the long poles start first.
+//
+// A row with P <= kDirectParts is read straight from the CSR (for P = 2 each
+// of the two workgroups keeps the arcs of its partition).  Otherwise the key fill
+// has bucketed each chunk of the row by partition, so workgroup (v, p) reads
+// only its own segment of every chunk: the segment lengths of up to 256
+// chunks are scanned across the workgroup and the concatenated segments are
+// processed 256 entries at a time, each thread locating its chunk by binary
+// search in the scanned starts.  Every key a workgroup reads belongs to it.
 template <bool Narrow, int S, bool Multi>
 __global__ void __launch_bounds__(kBlock)
 lm_big_kernel(LmArgs A, BigArgs B)
@@ -598,7 +600,9 @@ lm_big_kernel(LmArgs A, BigArgs B)
         }
         const int own = A.C[v];
         const double ki = A.kd[v];
-        const int nch = P > 1 ? (int)((d + kKeyChunkArcs - 1) / kKeyChunkArcs) : 0;
+        const bool whole = c0 < 0;       // the row is read from the CSR (P = 1 or few partitions)
+        const int nch = whole ? 0 : (int)((d + kKeyChunkArcs - 1) / kKeyChunkArcs);
+        const int psh = 32 - (31 - __builtin_clz((unsigned)P));   // top log2(P) bits of the hash
 
         // Segments of chunks [g * kBlock, g * kBlock + kBlock) of partition p:
         // s_pre = exclusive scan of their lengths, s_lo = their starts.
@@ -621,11 +625,11 @@ lm_big_kernel(LmArgs A, BigArgs B)
             __syncthreads();
             return tot;
         };
-        int total = P > 1 ? scan_group(0) : (int)d;
+        int total = whole ? (int)d : scan_group(0);
         // The table needs at most 2x the partition's entries (itself a bound
         // on its distinct keys): the smallest power of two >= 2 * entries, at
         // least 256, so the clear and the sweep cost O(entries) rather than O(S).
-        const int64_t entries = P > 1 && nch > kBlock ? (int64_t)S : (int64_t)total;
+        const int64_t entries = (whole && P > 1) || nch > kBlock ? (int64_t)S : (int64_t)total;
         int lg = 8;
         while ((1ll << lg) < 2 * entries && lg < ilog2c(S)) ++lg;
         const int ts = 1 << lg;
@@ -687,7 +691,7 @@ lm_big_kernel(LmArgs A, BigArgs B)
         // Each pass loads kFill entries per thread before inserting any, so
         // their gathers are in flight together (LDS caps this kernel at two
         // workgroups per CU; registers are not the limit).
-        if (P == 1) {
+        if (whole) {
             for (int64_t i0 = 0; i0 < d; i0 += kFill * kBlock) {
                 if (s_ovf) break;
                 int us[kFill], cs[kFill];
@@ -700,6 +704,11 @@ lm_big_kernel(LmArgs A, BigArgs B)
                 }
 #pragma unroll
                 for (int r = 0; r < kFill; ++r) cs[r] = us[r] != v ? A.C[us[r]] : -1;   // self arcs weigh 0 (graph_cuda.cu:2312-2315)
+                if (P > 1) {
+#pragma unroll
+                    for (int r = 0; r < kFill; ++r)
+                        if (cs[r] >= 0 && (int)(((unsigned)cs[r] * kPartMul) >> psh) != p) cs[r] = -1;
+                }
 #pragma unroll
                 for (int r = 0; r < kFill; ++r) insert(cs[r], ws[r]);
             }
