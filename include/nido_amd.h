@@ -90,6 +90,11 @@ const char* nido_last_error(void);
  * sums are FP64; exposed so that the conversions can be checked. */
 void nido_fx_from_double(double w, uint64_t* lo, int64_t* hi);
 double nido_fx_to_double(uint64_t lo, int64_t hi);
+/* out[i] = a[i] / b[i] by the division of the gain formulas (the reciprocal
+ * of b and two FMA corrections), as compiled for the host; exact for b in
+ * [2^-399, 2^399] and a = 0 or |a| in [2^-400, 2^400].  Exposed so that its
+ * rounding can be checked against IEEE division. */
+void nido_div_rcp(int64_t n, const double* a, const double* b, double* out);
 int nido_abi_version(void);
 /* Number of HIP devices visible (0 on a machine without a GPU). */
 int nido_device_count(int* count);

@@ -12,6 +12,7 @@
 // (graph_cuda.cu:335-340 warp trees, graph_gpu.cpp:499 reduce_by_key,
 // graph_cuda.cu:1244-1248 FP64 atomics), which is what this replaces.
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -101,6 +102,27 @@ NIDO_HD double fx_to_double(fx_t x)
         r = (double)(t | sticky) * nido_bitsd((uint64_t)(1023 - n) << 52);   // * 2^-n
     }
     return neg ? -r : r;
+}
+
+// ---- division by a per-graph constant, rounded as the division is ---------
+// RN(a / b) from y = RN(1 / b) (or y / 2 for b = 2 b', exactly RN(1 / (2 b'))),
+// without the division: q0 = RN(a y) is within 1.5 ulp of a / b, the
+// correction q1 = RN(q0 + (a - b q0) y) is within one ulp (faithful), and
+// with the residual a - b q1, exact in one FMA, q2 = RN(q1 + (a - b q1) y) is
+// the correctly rounded quotient (P. Markstein, IBM J. Res. Dev. 34(1), 1990).
+// The argument needs the quotient and the residuals normal: b in
+// [2^-399, 2^399] and a = 0 or |a| in [2^-400, 2^400].  Every division of
+// the gain formulas is in that range: the sums are Q64.64 values (0 or at
+// least 2^-64 in magnitude, below 2^63), so m is in [2^-65, 2^100], k_v a_c
+// in [2^-128, 2^200], its quotient by 2m at least 2^-229, and a difference
+// e - k_v a_c / (2m) is 0 or at least one ulp of 2^-229.
+NIDO_HD double div_rcp(double a, double b, double y)
+{
+    double q = a * y;
+    double r = fma(-b, q, a);
+    q = fma(r, y, q);
+    r = fma(-b, q, a);
+    return fma(r, y, q);
 }
 
 #if defined(__HIPCC__)

@@ -315,7 +315,7 @@ struct nido_graph {
     int* d_gid0 = nullptr;               // graph of every original vertex
     int64_t* d_vb = nullptr;             // G+1
     int64_t* d_vbn = nullptr;            // G+1 (bases after renumbering)
-    double* d_mg = nullptr;              // mass per graph
+    double* d_mg = nullptr;              // {mass, 1 / mass} per graph
     double* d_twomg = nullptr;           // 2m per graph, FP64
     unsigned char* d_act = nullptr;      // per graph: moves in this iteration
     unsigned char* d_keep = nullptr;     // per graph: selected for a restore / a coarsening
@@ -751,7 +751,7 @@ int do_modularity(nido_graph* g, double* q)
 int do_lm(nido_graph* g, int b)
 {
     g->committed[(size_t)b] = 0;
-    LmArgs A{g->d_off, g->d_col32, g->d_w, g->d_C32, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m};
+    LmArgs A{g->d_off, g->d_col32, g->d_w, g->d_C32, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m, 1.0 / g->m};
     if (g->G == 1) {
         const int64_t v0 = g->parts[(size_t)b], v1 = g->parts[(size_t)b + 1];
         // an empty batch still carries the previous batch's modularity delta
@@ -1463,20 +1463,21 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         g->vb0 = g->vb;
         g->twom_g.assign((size_t)G, 0);
         g->twom_dg.assign((size_t)G, 0.0);
-        std::vector<double> mg((size_t)G), tw((size_t)G);
+        std::vector<double> mg(2 * (size_t)G), tw((size_t)G);
         graph_twom(off, w, G, vptr, g->twom_g.data());   // 2m per graph, exactly (graph_cuda.cu:1855)
         for (int i = 0; i < G; ++i) {
             tw[(size_t)i] = g->twom_dg[(size_t)i] = fx_to_double(g->twom_g[(size_t)i]);
-            mg[(size_t)i] = tw[(size_t)i] * 0.5;
+            mg[2 * (size_t)i] = tw[(size_t)i] * 0.5;
+            mg[2 * (size_t)i + 1] = 1.0 / mg[2 * (size_t)i];   // unused while the graph has no weight
         }
         if ((rc = dev_alloc(&g->d_gid, nv)) || (rc = dev_alloc(&g->d_gid0, nv)) ||
             (rc = dev_alloc(&g->d_vb, G + 1)) || (rc = dev_alloc(&g->d_vbn, G + 1)) ||
-            (rc = dev_alloc(&g->d_mg, G)) || (rc = dev_alloc(&g->d_twomg, G)) || (rc = dev_alloc(&g->d_act, G)) ||
+            (rc = dev_alloc(&g->d_mg, 2 * (int64_t)G)) || (rc = dev_alloc(&g->d_twomg, G)) || (rc = dev_alloc(&g->d_act, G)) ||
             (rc = dev_alloc(&g->d_keep, G)) || (rc = dev_alloc(&g->d_gacc, 2 * (int64_t)G)) ||
             (rc = dev_alloc(&g->d_gq, G)))
             return fail(rc);
         if (hipMemcpyAsync(g->d_vb, g->vb.data(), sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipMemcpyAsync(g->d_mg, mg.data(), sizeof(double) * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(g->d_mg, mg.data(), sizeof(double) * 2 * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess ||
             hipMemcpyAsync(g->d_twomg, tw.data(), sizeof(double) * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess)
             return fail(set_error(NIDO_ERR_HIP, "upload failed"));
         launch_fill_gid(g->d_vb, G, g->d_gid, s);

@@ -491,4 +491,9 @@ void nido_fx_from_double(double w, uint64_t* lo, int64_t* hi)
 }
 
 double nido_fx_to_double(uint64_t lo, int64_t hi) { return fx_to_double(fx_make(lo, hi)); }
+
+void nido_div_rcp(int64_t n, const double* a, const double* b, double* out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = div_rcp(a[i], b[i], 1.0 / b[i]);
+}
 }

@@ -135,9 +135,10 @@ struct LmArgs {
     const fx_t* k;           // k_v exact
     const fx_t* a;           // a_c exact
     double m;                // mass, FP64 (one graph)
+    double ym;               // RN(1 / m)
     // Several graphs in one launch (gid null: one graph of mass m): vertex
-    // v belongs to graph gid[v], of mass mg[gid[v]], and takes part only
-    // while act[gid[v]] is set.
+    // v belongs to graph gid[v], of mass mg[2 gid[v]] with RN(1 / mass) at
+    // mg[2 gid[v] + 1], and takes part only while act[gid[v]] is set.
     const int* gid;
     const double* mg;
     const unsigned char* act;

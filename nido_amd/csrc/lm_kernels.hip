@@ -28,28 +28,34 @@
 namespace nido {
 namespace {
 
+// A graph's mass m with y = RN(1/m).  Both divisions of a gain are by m or
+// 2m (with 1/(2m) = y/2 exactly), so each is a multiplication by y plus two
+// FMA corrections (div_rcp), rounded as the division is.
+struct Mass {
+    double m, y;
+};
+
 // g_c = (e_{v,c} - k_v a_c / (2m)) / m, graph_cuda.cu:1505-1506.
-__device__ __forceinline__ double gain_of(const LmArgs& A, double m, double ki, fx_t e, int c)
+__device__ __forceinline__ double gain_of(const LmArgs& A, Mass m, double ki, fx_t e, int c)
 {
     double g = fx_to_double(e);
     const double ac = fx_to_double(A.a[c]);
-    g -= ki * ac / (2. * m);
-    g /= m;
-    return g;
+    g -= div_rcp(ki * ac, 2. * m.m, 0.5 * m.y);
+    return div_rcp(g, m.m, m.y);
 }
 
 // Mass of v's graph; false when that graph sits out the iteration.  The
 // kernels are compiled twice: Multi = false (one graph) leaves out the
 // graph lookups altogether.
 template <bool Multi>
-__device__ __forceinline__ bool graph_mass(const LmArgs& A, int v, double& m)
+__device__ __forceinline__ bool graph_mass(const LmArgs& A, int v, Mass& m)
 {
     if constexpr (!Multi) {
-        m = A.m;
+        m = Mass{A.m, A.ym};
         return true;
     }
     const int gg = A.gid[v];
-    m = A.mg[gg];
+    m = Mass{A.mg[2 * gg], A.mg[2 * gg + 1]};
     return A.act[gg] != 0;
 }
 
@@ -57,33 +63,33 @@ __device__ __forceinline__ bool graph_mass(const LmArgs& A, int v, double& m)
 // graph_cuda.cu:1527-1536.
 // `rest` = a_own - k_v, loaded by the caller ahead of the row scan.
 // `self_d` = e_{v,own} rounded to FP64.
-__device__ __forceinline__ int64_t decide_d(double m, double ki, fx_t rest, int own, double self_d,
+__device__ __forceinline__ int64_t decide_d(Mass m, double ki, fx_t rest, int own, double self_d,
                                             double best_g, int best_c)
 {
     if (best_c == kNone) return own;    // gain stays -MAX_FLOAT, graph_cuda.cu:1489
     double s = self_d;
     const double ar = fx_to_double(rest);
-    s -= ki * ar / (2. * m);
-    s /= m;
+    s -= div_rcp(ki * ar, 2. * m.m, 0.5 * m.y);
+    s = div_rcp(s, m.m, m.y);
     double gain = best_g;
     gain -= s;
     return gain > 0 ? (int64_t)best_c : (int64_t)own;
 }
 
-__device__ __forceinline__ int64_t decide(double m, double ki, fx_t rest, int own, fx_t self, double best_g,
+__device__ __forceinline__ int64_t decide(Mass m, double ki, fx_t rest, int own, fx_t self, double best_g,
                                           int best_c)
 {
     return decide_d(m, ki, rest, own, fx_to_double(self), best_g, best_c);
 }
 
-__device__ __forceinline__ int64_t decide(const LmArgs& A, double m, int v, int own, fx_t self,
+__device__ __forceinline__ int64_t decide(const LmArgs& A, Mass m, int v, int own, fx_t self,
                                           double best_g, int best_c)
 {
     return decide(m, A.kd[v], A.a[own] - A.k[v], own, self, best_g, best_c);
 }
 
 // Candidate update for one table entry whose a_c the caller has loaded.
-__device__ __forceinline__ void consider_a(double m, double ki, int own, int c, fx_t e, fx_t ac,
+__device__ __forceinline__ void consider_a(Mass m, double ki, int own, int c, fx_t e, fx_t ac,
                                            fx_t& self, double& bg, int& bc)
 {
     if (c == own) {
@@ -91,8 +97,8 @@ __device__ __forceinline__ void consider_a(double m, double ki, int own, int c, 
         return;
     }
     double g = fx_to_double(e);
-    g -= ki * fx_to_double(ac) / (2. * m);
-    g /= m;
+    g -= div_rcp(ki * fx_to_double(ac), 2. * m.m, 0.5 * m.y);
+    g = div_rcp(g, m.m, m.y);
     if (better(g, c, bg, bc)) {
         bg = g;
         bc = c;
@@ -100,7 +106,7 @@ __device__ __forceinline__ void consider_a(double m, double ki, int own, int c, 
 }
 
 // Candidate update for one table entry.
-__device__ __forceinline__ void consider(const LmArgs& A, double m, double ki, int own, int c, fx_t e,
+__device__ __forceinline__ void consider(const LmArgs& A, Mass m, double ki, int own, int c, fx_t e,
                                          fx_t& self, double& bg, int& bc)
 {
     if (c == own) {
@@ -120,7 +126,7 @@ __device__ __forceinline__ void consider(const LmArgs& A, double m, double ki, i
 // row's own community stores e_{v,own} (rounded) in *self_d, so only the
 // argmax needs a cross-lane reduction.
 template <int L>
-__device__ __forceinline__ void sweep_list(const LmArgs& A, double m, double ki, int own, const int* key,
+__device__ __forceinline__ void sweep_list(const LmArgs& A, Mass m, double ki, int own, const int* key,
                                            const unsigned long long* lo, const unsigned long long* hi,
                                            const int* klist, int nk, int lane, double* self_d, double& bg, int& bc)
 {
@@ -203,7 +209,7 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
     for (int64_t i = (int64_t)bi * kBlock + threadIdx.x; i < n; i += (int64_t)nbk * kBlock) {
         const BinRow r = verts[i];
         const int v = r.v;
-        double m;
+        Mass m;
         if (!graph_mass<Multi>(A, v, m)) continue;
         const int own = A.C[v];
         const double ki = A.kd[v];
@@ -274,7 +280,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
         bool act = idx < n;
         const BinRow r = act ? verts[idx] : BinRow{0, 0, 0};
         const int v = r.v;
-        double m = 0.0;
+        Mass m{0.0, 0.0};
         act = act && graph_mass<Multi>(A, v, m);
         const int own = act ? A.C[v] : 0;
         const double ki = act ? A.kd[v] : 0.0;
@@ -323,7 +329,7 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
         }
         const BinRow r = verts[idx];
         const int v = r.v;
-        double m;
+        Mass m;
         if (!graph_mass<Multi>(A, v, m)) continue;   // uniform: the workgroup holds one vertex
         const int own = A.C[v];
         const double ki = A.kd[v];
@@ -355,7 +361,7 @@ __device__ __forceinline__ void keyfill_body(const LmArgs& A, const SmallBins& L
     int* wtot = cnt + kMaxParts + 1;
     for (int64_t i = bi; i < L.nchunks; i += nbk) {
         const KeyChunk ch = L.chunks[i];
-        double m;
+        Mass m;
         if (!graph_mass<Multi>(A, ch.v, m)) continue;   // uniform over the workgroup
         const int P = ch.P;
         const int psh = 32 - (31 - __builtin_clz((unsigned)P));   // top log2(P) bits of the hash
@@ -455,7 +461,7 @@ lm_small_kernel(LmArgs A, SmallBins L)
 
 // Whole-row aggregation in an HBM table of `size` slots (>= 2 * row length):
 // the exact fallback taken when a partition outgrows its LDS table.
-__device__ __forceinline__ void global_row_decide(const LmArgs& A, double m, int v, int* __restrict__ key,
+__device__ __forceinline__ void global_row_decide(const LmArgs& A, Mass m, int v, int* __restrict__ key,
                                   fx_t* __restrict__ tval, int64_t size, BlockRed& R)
 {
     unsigned long long* val = reinterpret_cast<unsigned long long*>(tval);
@@ -593,7 +599,7 @@ lm_big_kernel(LmArgs A, BigArgs B)
         auto prefetch_next = [&]() {
             if (itn < B.nitems) item_segment(B, nx, threadIdx.x, n_l0, n_len);
         };
-        double m;
+        Mass m;
         if (!graph_mass<Multi>(A, v, m)) {   // uniform; every partition of the row skips it
             prefetch_next();
             continue;

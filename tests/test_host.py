@@ -223,3 +223,40 @@ def test_device_conversions_match_the_oracle():
     for x in xs:
         lo, hi = x & ((1 << 64) - 1), x >> 64
         assert na.lib.nido_fx_to_double(lo, hi) == oracle.fx_to_double(x), x
+
+
+def test_reciprocal_division_is_ieee_division():
+    """div_rcp (fixed.h, built for the host), which replaces both divisions
+    of the gain formulas, against IEEE division inside its stated range:
+    random quotients of every magnitude, quotients of Q64.64-grid values as
+    the gains produce them, quotients near a rounding midpoint (and one ulp
+    either side), and zero numerators."""
+    import ctypes as C
+    f = na.lib.nido_div_rcp
+    f.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+
+    def check(a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.empty_like(a)
+        f(a.size, a.ctypes.data, b.ctypes.data, out.ctypes.data)
+        same = out.view(np.int64) == (a / b).view(np.int64)
+        assert same.all(), (a[~same][:4], b[~same][:4])
+
+    rng = np.random.default_rng(11)
+    n = 400_000
+    check(rng.uniform(-2, 2, n), rng.uniform(0.1, 3, n))
+    check(np.exp2(rng.uniform(-399, 399, n)) * rng.choice([-1.0, 1.0], n), np.exp2(rng.uniform(-398, 398, n)))
+    check(rng.integers(-10 ** 6, 10 ** 6, n).astype(float), rng.integers(1, 10 ** 7, n) / 2.0)
+    # e - k a / (2m) and its quotient by m with Q64.64-grid sums (multiples of 2^-64)
+    m = np.ldexp(rng.integers(1, 1 << 53, n).astype(float), rng.integers(-64, 40, n))
+    k = np.ldexp(rng.integers(1, 1 << 53, n).astype(float), rng.integers(-64, 30, n))
+    a = np.ldexp(rng.integers(1, 1 << 53, n).astype(float), rng.integers(-64, 30, n))
+    check(k * a, 2 * m)
+    check(k - (k * a) / (2 * m), m)
+    b = rng.uniform(1, 2, n)
+    q = rng.uniform(1, 2, n)
+    mid = b * (q + np.spacing(q) / 2)
+    for x in (mid, np.nextafter(mid, np.inf), np.nextafter(mid, -np.inf)):
+        check(x, b)
+    check([0.0, 0.0], [3.0, 2.0 ** -65])
