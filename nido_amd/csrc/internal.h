@@ -111,7 +111,7 @@ struct BigArgs {
     const int* cpo;          // per chunk: P + 1 segment starts
     int key_limit;           // distinct keys a partition may hold in LDS (<= slots/2)
     int narrow;              // every row's weights are >= 0 and sum below 2^31
-    int slots;               // LDS table size of a partition workgroup: 2048 or 4096
+    int slots;               // LDS table size of a partition workgroup: 1024, 2048 or 4096
     const int64_t* poff;     // per slot: first partial
     double* pg;              // partial best gain
     int* pc;                 // partial best community

@@ -695,8 +695,8 @@ lm_big_kernel(LmArgs A, BigArgs B)
         };
 
         // Each pass loads kFill entries per thread before inserting any, so
-        // their gathers are in flight together (LDS caps this kernel at two
-        // workgroups per CU; registers are not the limit).
+        // their gathers are in flight together (LDS caps this kernel at four
+        // workgroups per CU with 2048 slots; registers are not the limit).
         if (whole) {
             for (int64_t i0 = 0; i0 < d; i0 += kFill * kBlock) {
                 if (s_ovf) break;
