@@ -14,7 +14,9 @@
 
 #include "../../include/nido_amd.h"
 #include "errors.h"
+#include "host_bins.h"
 #include "host_check.h"
+#include "host_copy.h"
 #include "internal.h"
 
 using namespace nido;
@@ -35,8 +37,6 @@ namespace {
         if (r_ != NIDO_OK) return r_;   \
     } while (0)
 
-constexpr int64_t kSmallRow = 32;      // rows handled by one lane in the degree/modularity sweeps
-constexpr int64_t kItemArcs = 1024;    // longer rows are cut into slices of this many arcs
 
 // Device memory is stream-ordered: allocations and frees made while a
 // handle's AllocScope is active go through a library-owned memory pool on
@@ -341,15 +341,6 @@ struct nido_graph {
 
 namespace {
 
-int bin_of(int64_t d, const nido_tuning& t)
-{
-    if (d > t.big_min_deg) return BIN_BIG;
-    if (d <= t.thread_max_deg) return BIN_THREAD;
-    for (int b = BIN_G8; b < NUM_SMALL_BINS; ++b)
-        if (d <= kBinMaxDeg[b]) return b;
-    return BIN_BIG;
-}
-
 void default_tuning(nido_tuning& t)
 {
     t = nido_tuning{};
@@ -364,53 +355,18 @@ void default_tuning(nido_tuning& t)
     t.full_modularity = 0;
 }
 
-// partition_graph_edge_batch with part_on_batch=1 on one device,
-// graph_gpu.cpp:303-339.
-void batch_bounds(const int64_t* off, int64_t nv, int nb, std::vector<int64_t>& parts)
-{
-    const int64_t ne = off[nv] - off[0];
-    const int64_t ave = ne / nb;
-    std::vector<int64_t> vp;
-    vp.reserve((size_t)nb + 2);
-    vp.push_back(0);
-    int64_t start = 0;
-    for (int64_t i = 1; i <= nv; ++i) {
-        if (off[i] - off[start] > ave) {
-            vp.push_back(i);
-            start = i;
-        } else if (i == nv) {
-            vp.push_back(nv);
-        }
-    }
-    if ((int64_t)vp.size() > nb + 1) {
-        const int64_t remain = nv - vp[nb];
-        const int64_t nnv = remain / nb;
-        for (int i = 1; i < nb; ++i) vp[i] += nnv;
-        vp[nb] = nv;
-    } else {
-        while ((int64_t)vp.size() < nb + 1) vp.push_back(nv);
-    }
-    parts.assign(vp.begin(), vp.begin() + nb + 1);
-}
-
 void batch_bounds(const std::vector<int64_t>& off, int64_t nv, int nb, std::vector<int64_t>& parts)
 {
-    batch_bounds(off.data(), nv, nb, parts);
+    nido::batch_bounds(off.data(), nv, nb, parts);
 }
 
-int64_t pow2_at_least(int64_t x)
-{
-    int64_t s = 2;
-    while (s < x) s <<= 1;
-    return s;
-}
 
 template <typename T>
 int upload(T** d, const std::vector<T>& h, hipStream_t s)
 {
     dev_free(*d);
     NIDO_TRY(dev_alloc(d, (int64_t)h.size()));
-    if (!h.empty()) HIP_TRY(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    if (!h.empty()) HIP_TRY(staged_h2d(*d, h.data(), sizeof(T) * h.size(), s));
     return NIDO_OK;
 }
 
@@ -425,118 +381,33 @@ int setup_phase(nido_graph* g)
     const auto& off = g->h_off;
     drop_iteration_graph(g);
     g->inc_ok = false;
-    g->maxdeg = 0;
-    for (int64_t v = 0; v < nv; ++v) g->maxdeg = std::max(g->maxdeg, off[v + 1] - off[v]);
+    g->maxdeg = max_degree(off.data(), nv);
     g->committed.assign((size_t)g->nb, 0);
     // Batch b of the handle = batch b of every graph it holds (each graph
     // cut by its own rule, graph_gpu.cpp:303-339).
     const int G = g->G;
-    const size_t nb1 = (size_t)g->nb + 1;
-    if (G == 1) {
-        batch_bounds(off, nv, g->nb, g->parts);
-    } else {
-        g->gparts.assign((size_t)G * nb1, 0);
-        std::vector<int64_t> p;
-        for (int gg = 0; gg < G; ++gg) {
-            const int64_t v0 = g->vb[(size_t)gg], n = g->vb[(size_t)gg + 1] - v0;
-            batch_bounds(off.data() + v0, n, g->nb, p);
-            for (size_t i = 0; i < nb1; ++i) g->gparts[(size_t)gg * nb1 + i] = v0 + p[i];
-        }
-    }
-    auto for_batch = [&](int b, auto&& fn) {
-        if (G == 1) {
-            fn(g->parts[(size_t)b], g->parts[(size_t)b + 1]);
-            return;
-        }
-        for (int gg = 0; gg < G; ++gg) fn(g->gparts[(size_t)gg * nb1 + b], g->gparts[(size_t)gg * nb1 + b + 1]);
-    };
+    if (G == 1) batch_bounds(off, nv, g->nb, g->parts);
+    else batch_bounds_all(off.data(), G, g->vb.data(), g->nb, g->gparts);
 
-    // Short rows: vertices grouped by (batch, bin), ascending id inside a
-    // group, then the batch's long rows; isolated vertices are left out
-    // (they never move, and C' = C holds for them throughout), so a batch's
-    // slice of binv lists every vertex its commit has to visit.  Long rows:
-    // per batch, P = 2^ceil(log2(d / part_arcs)) partition items, their
-    // partial slots, HBM fallback tables, and for P > 1 the key-fill chunks
-    // and bucket space.
-    std::vector<BinRow> binv((size_t)std::max<int64_t>(nv, 1));
-    std::vector<BigItem> bitems;
-    std::vector<KeyChunk> chunks;
-    std::vector<int64_t> poff, toff;
-    std::vector<int64_t> item_base((size_t)g->nb), poff_base((size_t)g->nb), toff_base((size_t)g->nb),
-        chunk_base((size_t)g->nb + 1);
-    std::vector<int64_t> small_start((size_t)g->nb * NUM_SMALL_BINS), small_n((size_t)g->nb * NUM_SMALL_BINS);
-    int64_t pos = 0, tab_need = 0, part_need = 0, slot_need = 0, key_need = 0, cpo_need = 0;
-    g->short_bytes.assign((size_t)g->nb, 0.0);
-    g->long_bytes.assign((size_t)g->nb, 0.0);
-    g->blist0.assign((size_t)g->nb, 0);
-    g->blistn.assign((size_t)g->nb, 0);
-    for (int b = 0; b < g->nb; ++b) {
-        chunk_base[(size_t)b] = (int64_t)chunks.size();
-        int64_t ksum = 0, csum = 0;
-        int64_t cnt[NUM_BINS] = {0};
-        for_batch(b, [&](int64_t v0, int64_t v1) {
-            for (int64_t v = v0; v < v1; ++v)
-                if (off[v + 1] > off[v]) cnt[bin_of(off[v + 1] - off[v], g->tuning)]++;
-        });
-        int64_t cur[NUM_BINS];
-        g->blist0[(size_t)b] = pos;
-        for (int k = 0; k < NUM_BINS; ++k) {
-            if (k < NUM_SMALL_BINS) {
-                small_start[(size_t)b * NUM_SMALL_BINS + k] = pos;
-                small_n[(size_t)b * NUM_SMALL_BINS + k] = cnt[k];
-            }
-            cur[k] = pos;
-            pos += cnt[k];
-        }
-        g->blistn[(size_t)b] = pos - g->blist0[(size_t)b];
-        item_base[(size_t)b] = (int64_t)bitems.size();
-        poff_base[(size_t)b] = (int64_t)poff.size();
-        toff_base[(size_t)b] = (int64_t)toff.size();
-        int64_t psum = 0, tsum = 0;
-        int slot = 0;
-        toff.push_back(0);
-        for_batch(b, [&](int64_t v0, int64_t v1) {
-          for (int64_t v = v0; v < v1; ++v) {
-            const int64_t d = off[v + 1] - off[v];
-            const int k = bin_of(d, g->tuning);
-            // algorithmic bytes of the sweep: 24 per arc (col, weight,
-            // C[col]) + 40 per vertex (offset, C, k, a_C, C')
-            (k != BIN_BIG ? g->short_bytes : g->long_bytes)[(size_t)b] += 24.0 * (double)d + 40.0;
-            if (d == 0) continue;
-            binv[(size_t)cur[k]++] = BinRow{(int)v, (int)std::min<int64_t>(d, INT32_MAX), off[v]};
-            if (k != BIN_BIG) continue;
-            int P = 1;
-            while (P < kMaxParts && (int64_t)P * g->tuning.big_part_arcs < d) P <<= 1;
-            const bool direct = P <= kDirectParts;
-            for (int p = 0; p < P; ++p)
-                bitems.push_back(BigItem{(int)v, p, P, slot, ksum, direct ? -1 : csum, off[v], d});
-            if (!direct) {
-                for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs) {
-                    chunks.push_back(KeyChunk{(int)v, P, e, std::min(off[v + 1], e + kKeyChunkArcs),
-                                              ksum + (e - off[v]), csum});
-                    csum += P + 1;
-                }
-                ksum += d;
-            }
-            poff.push_back(psum);
-            psum += P;
-            tsum += pow2_at_least(2 * d);
-            toff.push_back(tsum);
-            ++slot;
-          }
-        });
-        // longest rows first: their partition workgroups are the long poles
-        std::stable_sort(bitems.begin() + item_base[(size_t)b], bitems.end(),
-                         [&](const BigItem& x, const BigItem& y) {
-                             return off[x.v + 1] - off[x.v] > off[y.v + 1] - off[y.v];
-                         });
-        tab_need = std::max(tab_need, tsum);
-        part_need = std::max(part_need, psum);
-        slot_need = std::max<int64_t>(slot_need, slot);
-        key_need = std::max(key_need, ksum);
-        cpo_need = std::max(cpo_need, csum);
-    }
-    chunk_base[(size_t)g->nb] = (int64_t)chunks.size();
+    PhaseBins pb;
+    build_phase_bins(off.data(), nv, g->nb, G, g->parts, g->gparts, g->tuning, pb);
+    const auto& binv = pb.binv;
+    const auto& bitems = pb.bitems;
+    const auto& chunks = pb.chunks;
+    const auto& poff = pb.poff;
+    const auto& toff = pb.toff;
+    const auto& item_base = pb.item_base;
+    const auto& poff_base = pb.poff_base;
+    const auto& toff_base = pb.toff_base;
+    const auto& chunk_base = pb.chunk_base;
+    const auto& small_start = pb.small_start;
+    const auto& small_n = pb.small_n;
+    const int64_t tab_need = pb.tab_need, part_need = pb.part_need, slot_need = pb.slot_need,
+                  key_need = pb.key_need, cpo_need = pb.cpo_need;
+    g->short_bytes = pb.short_bytes;
+    g->long_bytes = pb.long_bytes;
+    g->blist0 = pb.blist0;
+    g->blistn = pb.blistn;
     NIDO_TRY(upload(&g->d_binv, binv, g->stream));
     NIDO_TRY(upload(&g->d_big_items, bitems, g->stream));
     NIDO_TRY(upload(&g->d_key_chunks, chunks, g->stream));
@@ -641,15 +512,7 @@ int setup_phase(nido_graph* g)
 
     std::vector<int> small;
     std::vector<RowItem> items;
-    for (int64_t v = 0; v < nv; ++v) {
-        const int64_t d = off[v + 1] - off[v];
-        if (d <= kSmallRow) {
-            small.push_back((int)v);
-        } else {
-            for (int64_t e = off[v]; e < off[v + 1]; e += kItemArcs)
-                items.push_back(RowItem{v, e, std::min(off[v + 1], e + kItemArcs)});
-        }
-    }
+    build_row_lists(off.data(), nv, small, items);
     g->nsmall = (int64_t)small.size();
     g->nitems = (int64_t)items.size();
     NIDO_TRY(upload(&g->d_small, small, g->stream));
@@ -1446,17 +1309,13 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     if (hipHostMalloc(reinterpret_cast<void**>(&g->h_q), sizeof(double)) != hipSuccess)
         return fail(set_error(NIDO_ERR_NOMEM, "pinned allocation failed"));
     hipStream_t s = g->stream;
-    if (hipMemcpyAsync(g->d_off, g->h_off.data(), sizeof(int64_t) * (size_t)(nv + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
-        (ne > 0 && hipMemcpyAsync(g->d_col, col + base, sizeof(int64_t) * (size_t)ne, hipMemcpyHostToDevice, s) != hipSuccess) ||
-        (ne > 0 && hipMemcpyAsync(g->d_w, w + base, sizeof(double) * (size_t)ne, hipMemcpyHostToDevice, s) != hipSuccess))
-        return fail(set_error(NIDO_ERR_HIP, "upload failed"));
     launch_fill_fx(g->d_acc, 4, s);
     // Clustering::commIds_ starts as the identity (clustering.cpp:4-10)
-    std::vector<int64_t> ident((size_t)std::max<int64_t>(nv, 1));
-    for (int64_t i = 0; i < nv; ++i) ident[(size_t)i] = i;
-    if (nv > 0 && hipMemcpyAsync(g->d_clus, ident.data(), sizeof(int64_t) * (size_t)nv, hipMemcpyHostToDevice, s) != hipSuccess)
+    launch_iota_i64(g->d_clus, nv, s);
+    if (staged_h2d(g->d_off, g->h_off.data(), sizeof(int64_t) * (size_t)(nv + 1), s) != hipSuccess ||
+        (ne > 0 && staged_h2d(g->d_col, col + base, sizeof(int64_t) * (size_t)ne, s) != hipSuccess) ||
+        (ne > 0 && staged_h2d(g->d_w, w + base, sizeof(double) * (size_t)ne, s) != hipSuccess))
         return fail(set_error(NIDO_ERR_HIP, "upload failed"));
-    if (hipStreamSynchronize(s) != hipSuccess) return fail(set_error(NIDO_ERR_HIP, "upload sync failed"));
     if (G > 1) {
         g->G = G;
         g->vb.assign(vptr, vptr + G + 1);

@@ -1,0 +1,48 @@
+// host_bins.h -- per-phase work lists of the local-moving launches, built on
+// the host (OpenMP): batch boundaries, degree bins, long-row items and the
+// row lists of the degree and modularity sweeps.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/nido_amd.h"
+#include "internal.h"
+
+namespace nido {
+
+constexpr int64_t kSmallRow = 32;      // rows handled by one lane in the degree/modularity sweeps
+constexpr int64_t kItemArcs = 1024;    // longer rows are cut into slices of this many arcs
+
+// nb + 1 batch boundaries: partition_graph_edge_batch with part_on_batch=1
+// on one device, graph_gpu.cpp:303-339.
+void batch_bounds(const int64_t* off, int64_t nv, int nb, std::vector<int64_t>& parts);
+
+// The same for each of G graphs (graph g owns vertices [vb[g], vb[g+1])),
+// boundaries in handle-wide ids: gparts[g * (nb + 1) + i].
+void batch_bounds_all(const int64_t* off, int G, const int64_t* vb, int nb, std::vector<int64_t>& gparts);
+
+int64_t max_degree(const int64_t* off, int64_t nv);
+
+// Degree bins and long-row work of every batch.  Batch b of the handle is
+// [parts[b], parts[b+1]) for one graph, or the union over graphs g of
+// [gparts[g (nb+1) + b], gparts[g (nb+1) + b + 1]).
+struct PhaseBins {
+    std::vector<BinRow> binv;          // per batch: its rows by bin (ascending id inside a bin), long rows last
+    std::vector<BigItem> bitems;       // per batch: partition items, longest row first
+    std::vector<KeyChunk> chunks;      // per batch: key-fill chunks of rows with P > kDirectParts
+    std::vector<int64_t> poff, toff;   // per batch: partial offsets per row slot; HBM table offsets (slots + 1)
+    std::vector<int64_t> item_base, poff_base, toff_base, chunk_base;   // chunk_base has nb + 1 entries
+    std::vector<int64_t> small_start, small_n;                          // [b * NUM_SMALL_BINS + k]
+    std::vector<int64_t> blist0, blistn;                                // batch slices of binv
+    std::vector<double> short_bytes, long_bytes;                        // algorithmic bytes per batch
+    int64_t tab_need = 0, part_need = 0, slot_need = 0, key_need = 0, cpo_need = 0;
+};
+void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::vector<int64_t>& parts,
+                      const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb);
+
+// Rows of at most kSmallRow arcs (one lane each) and kItemArcs-arc slices
+// of the longer rows, both in ascending vertex order.
+void build_row_lists(const int64_t* off, int64_t nv, std::vector<int>& small, std::vector<RowItem>& items);
+
+}  // namespace nido

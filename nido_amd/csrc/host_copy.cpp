@@ -1,0 +1,50 @@
+// host_copy.cpp -- large host -> device copies through pinned staging buffers.
+#include "host_copy.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+
+namespace nido {
+
+hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    constexpr size_t kChunk = size_t(64) << 20;
+    if (bytes < 2 * kChunk) {
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+        return e != hipSuccess ? e : hipStreamSynchronize(s);
+    }
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), kChunk);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+    }
+    const char* in = static_cast<const char*>(src);
+    char* out = static_cast<char*>(dst);
+    for (size_t at = 0, i = 0; e == hipSuccess && at < bytes; at += kChunk, i ^= 1) {
+        const size_t n = std::min(kChunk, bytes - at);
+        e = hipEventSynchronize(done[i]);      // the buffer's previous copy has landed
+        if (e != hipSuccess) break;
+        constexpr size_t kPiece = size_t(1) << 20;
+        const int64_t pieces = (int64_t)((n + kPiece - 1) / kPiece);
+#pragma omp parallel for schedule(static)
+        for (int64_t p = 0; p < pieces; ++p) {
+            const size_t o = (size_t)p * kPiece;
+            std::memcpy(buf[i] + o, in + at + o, std::min(kPiece, n - o));
+        }
+        e = hipMemcpyAsync(out + at, buf[i], n, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(done[i], s);
+    }
+    const hipError_t w = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = w;
+    for (int i = 0; i < 2; ++i) {
+        if (done[i]) (void)hipEventDestroy(done[i]);
+        if (buf[i]) (void)hipHostFree(buf[i]);
+    }
+    return e;
+}
+
+}  // namespace nido

@@ -1,0 +1,14 @@
+// host_copy.h -- large host -> device copies through pinned staging buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace nido {
+
+// Copies `bytes` from pageable host memory to device memory on stream s and
+// waits for the copy.  Large copies go through two pinned buffers (host
+// copies into one split across threads while the other is in flight); a
+// pageable hipMemcpy stages through one thread.  Returns the first HIP error.
+hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+}  // namespace nido

@@ -146,6 +146,7 @@ struct LmArgs {
 
 // ---- launch helpers (kernels.hip) ----------------------------------------
 void launch_fill_i64(int64_t* p, int64_t val, int64_t n, hipStream_t s);
+void launch_iota_i64(int64_t* p, int64_t n, hipStream_t s);   // p[i] = i
 void launch_fill_i32(int32_t* p, int32_t val, int64_t n, hipStream_t s);
 void launch_fill_fx(fx_t* p, int64_t n, hipStream_t s);
 

@@ -6,6 +6,12 @@
 namespace nido {
 namespace {
 
+// p[i] = i.
+__global__ void iota_kernel(int64_t* __restrict__ p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = i;
+}
+
 template <typename T>
 __global__ void fill_kernel(T* __restrict__ p, T val, int64_t n)
 {
@@ -491,6 +497,12 @@ void launch_fill_i64(int64_t* p, int64_t val, int64_t n, hipStream_t s)
 {
     if (n <= 0) return;
     fill_kernel<int64_t><<<grid_for(n, kBlock * 4), kBlock, 0, s>>>(p, val, n);
+}
+
+void launch_iota_i64(int64_t* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    iota_kernel<<<grid_for(n, kBlock * 4), kBlock, 0, s>>>(p, n);
 }
 
 void launch_fill_i32(int32_t* p, int32_t val, int64_t n, hipStream_t s)
