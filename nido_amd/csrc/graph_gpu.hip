@@ -1291,7 +1291,7 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     g->nv = nv;
     g->ne = ne;
     g->h_off.resize((size_t)nv + 1);
-    for (int64_t i = 0; i <= nv; ++i) g->h_off[(size_t)i] = off[i] - base;
+    rebase_offsets(off, nv, g->h_off.data());
     int rc;
     if ((rc = dev_alloc(&g->d_off, nv + 1)) || (rc = dev_alloc(&g->d_col, ne)) ||
         (rc = dev_alloc(&g->d_w, ne)) || (rc = dev_alloc(&g->d_C, nv)) ||

@@ -47,4 +47,11 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
     return e;
 }
 
+void rebase_offsets(const int64_t* off, int64_t n, int64_t* out)
+{
+    const int64_t base = off[0];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i <= n; ++i) out[i] = off[i] - base;
+}
+
 }  // namespace nido

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
 
 namespace nido {
 
@@ -10,5 +11,8 @@ namespace nido {
 // copies into one split across threads while the other is in flight); a
 // pageable hipMemcpy stages through one thread.  Returns the first HIP error.
 hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// out[i] = off[i] - off[0] for i <= n, split across threads.
+void rebase_offsets(const int64_t* off, int64_t n, int64_t* out);
 
 }  // namespace nido

@@ -427,21 +427,37 @@ int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* 
     if (bad_off != INT64_MAX) return set_error(NIDO_ERR_INVALID, "offsets decrease at %lld", (long long)bad_off);
     const int64_t ne = off[nv] - base;
     if (ne > 0 && (!col || !w)) return set_error(NIDO_ERR_INVALID, "null cols/weights");
-    int64_t bad_col = INT64_MAX, bad_w = INT64_MAX;
+    // first bad arc of graph g: column outside the graph, weight not finite
+    // or too large; one parallel loop over the graphs when there are many,
+    // over each graph's arcs otherwise
     int neg = 0;
-    for (int g = 0; g < G; ++g) {
+    auto scan = [&](int g, int64_t e0, int64_t e1, int64_t& bad_col, int64_t& bad_w, int& ng, bool par) {
         const int64_t lo = vptr[g], hi = vptr[g + 1];
-        const int64_t e0 = off[lo] - base, e1 = off[hi] - base;
-#pragma omp parallel for reduction(min : bad_col, bad_w) reduction(| : neg) schedule(static)
+        int64_t bc = INT64_MAX, bw = INT64_MAX;
+        int nn = 0;
+#pragma omp parallel for reduction(min : bc, bw) reduction(| : nn) schedule(static) if (par)
         for (int64_t e = e0; e < e1; ++e) {
             const int64_t c = col[base + e];
-            if ((c < lo || c >= hi) && e < bad_col) bad_col = e;
+            if ((c < lo || c >= hi) && e < bc) bc = e;
             const double x = w[base + e];
             if (!(x == x) || x >= 0x1p62 || x <= -0x1p62) {
-                if (e < bad_w) bad_w = e;
+                if (e < bw) bw = e;
             }
-            neg |= x < 0;
+            nn |= x < 0;
         }
+        bad_col = bc;
+        bad_w = bw;
+        ng = nn;
+    };
+    std::vector<int64_t> gbc((size_t)G), gbw((size_t)G);
+    std::vector<int> gneg((size_t)G);
+    const bool many = G >= 64;
+#pragma omp parallel for schedule(dynamic, 16) if (many)
+    for (int g = 0; g < G; ++g)
+        scan(g, off[vptr[g]] - base, off[vptr[g + 1]] - base, gbc[(size_t)g], gbw[(size_t)g], gneg[(size_t)g], !many);
+    for (int g = 0; g < G; ++g) {
+        const int64_t bad_col = gbc[(size_t)g], bad_w = gbw[(size_t)g];
+        neg |= gneg[(size_t)g];
         if (bad_col != INT64_MAX && G > 1)
             return set_error(NIDO_ERR_INVALID, "arc %lld (col %lld) leaves graph %d", (long long)bad_col,
                              (long long)col[base + bad_col], g);
@@ -458,6 +474,15 @@ int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* 
 void graph_twom(const int64_t* off, const double* w, int G, const int64_t* vptr, fx_t* out)
 {
     const int64_t base = off[0];
+    if (G >= 64) {   // many graphs: one thread per graph
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int g = 0; g < G; ++g) {
+            fx_t s = 0;
+            for (int64_t e = off[vptr[g]] - base; e < off[vptr[g + 1]] - base; ++e) s += fx_from_double(w[base + e]);
+            out[g] = s;
+        }
+        return;
+    }
     for (int g = 0; g < G; ++g) {
         const int64_t e0 = off[vptr[g]] - base, e1 = off[vptr[g + 1]] - base;
         uint64_t lo = 0;
