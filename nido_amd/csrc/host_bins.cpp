@@ -87,7 +87,7 @@ void batch_bounds_all(const int64_t* off, int G, const int64_t* vb, int nb, std:
 int64_t max_degree(const int64_t* off, int64_t nv)
 {
     int64_t m = 0;
-#pragma omp parallel for reduction(max : m) schedule(static)
+#pragma omp parallel for reduction(max : m) schedule(static) if (nv >= kParallelRows)
     for (int64_t v = 0; v < nv; ++v) m = std::max(m, off[v + 1] - off[v]);
     return m;
 }
@@ -95,7 +95,6 @@ int64_t max_degree(const int64_t* off, int64_t nv)
 void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::vector<int64_t>& parts,
                       const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb)
 {
-    (void)nv;
     const size_t nb1 = (size_t)nb + 1;
     auto for_batch = [&](int b, auto&& fn) {
         if (G == 1) {
@@ -112,7 +111,7 @@ void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::
     // per batch, P = 2^ceil(log2(d / part_arcs)) partition items, their
     // partial slots, HBM fallback tables, and for P > kDirectParts the
     // key-fill chunks and bucket space.
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) if (nv >= kParallelRows)
     for (int b = 0; b < nb; ++b) {
         BatchWork& w = W[(size_t)b];
         for_batch(b, [&](int64_t v0, int64_t v1) {
@@ -208,7 +207,7 @@ void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::
     pb.chunks.resize((size_t)chunks);
     pb.poff.resize((size_t)poffs);
     pb.toff.resize((size_t)toffs);
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) if (nv >= kParallelRows)
     for (int b = 0; b < nb; ++b) {
         const BatchWork& w = W[(size_t)b];
         std::copy(w.rows.begin(), w.rows.end(), pb.binv.begin() + pb.blist0[(size_t)b]);
@@ -221,7 +220,7 @@ void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::
 
 void build_row_lists(const int64_t* off, int64_t nv, std::vector<int>& small, std::vector<RowItem>& items)
 {
-    const int T = std::max(1, omp_get_max_threads());
+    const int T = nv >= kParallelRows ? std::max(1, omp_get_max_threads()) : 1;
     std::vector<std::vector<int>> ls((size_t)T);
     std::vector<std::vector<RowItem>> li((size_t)T);
 #pragma omp parallel num_threads(T)

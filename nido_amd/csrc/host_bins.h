@@ -13,6 +13,10 @@ namespace nido {
 
 constexpr int64_t kSmallRow = 32;      // rows handled by one lane in the degree/modularity sweeps
 constexpr int64_t kItemArcs = 1024;    // longer rows are cut into slices of this many arcs
+// Host work lists are built with threads only from this many vertices up:
+// smaller graphs are the serving loop's, whose requests already run side by
+// side, one host thread each.
+constexpr int64_t kParallelRows = int64_t(1) << 22;
 
 // nb + 1 batch boundaries: partition_graph_edge_batch with part_on_batch=1
 // on one device, graph_gpu.cpp:303-339.
