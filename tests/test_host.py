@@ -179,6 +179,25 @@ def test_batch_create_validates_on_the_host():
                     nbatches=2)
 
 
+def test_batch_validation_names_the_first_bad_graph_among_many():
+    """With 64+ graphs the checks run one graph per task: the error still
+    names the first graph (in order) with a bad arc or weight."""
+    G = 100
+    vptr = np.arange(0, 2 * G + 1, 2)
+    off = np.arange(0, 2 * G + 1)
+    col = np.arange(2 * G) ^ 1                  # each graph: one edge, both directions
+    w = np.ones(2 * G)
+    bad_col, bad_w = col.copy(), w.copy()
+    bad_col[2 * 80] = 2 * 80 + 5                # graph 80 points outside itself
+    bad_w[2 * 70 + 1] = np.inf                  # graph 70 has a weight that is not finite
+    with pytest.raises(na.NidoError) as e:
+        na.BatchGPU(csr=(vptr, off, bad_col, bad_w), nbatches=2)
+    assert "weight" in str(e.value)
+    with pytest.raises(na.NidoError) as e:
+        na.BatchGPU(csr=(vptr, off, bad_col, w), nbatches=2)
+    assert "graph 80" in str(e.value)
+
+
 def test_concat_graphs_layout():
     g1 = (np.array([0, 1, 2]), np.array([1, 0]), np.array([0.5, 0.5]))
     g2 = (np.array([0, 2, 3, 4]), np.array([1, 2, 0, 0]), np.array([1.0, 2.0, 1.0, 2.0]))
