@@ -845,15 +845,21 @@ lm_big_kernel(LmArgs A, BigArgs B)
 // new community in an LDS table, and each distinct community of the chunk
 // then takes one global atomic.  Rows moving into one popular community
 // would otherwise serialise on its a_c.  The sums are exact, so the totals
-// do not depend on the grouping.
+// do not depend on the grouping.  The table is cleared once per workgroup;
+// new keys list their slots, and after each chunk only the listed slots are
+// cleared, so a chunk costs in proportion to its moves (and nothing beyond
+// one barrier when none of its vertices moved, the common case after the
+// first iterations of a phase).
 constexpr int kCommitSlots = 4 * kBlock;   // at most 2 * kBlock keys: load <= 1/2
 struct CommitTable {
     int key[kCommitSlots];
     unsigned long long lo[kCommitSlots];
     unsigned long long hi[kCommitSlots];
+    int list[2 * kBlock];                  // slots of this chunk's keys
+    int n;                                 // keys listed
 };
 
-__device__ __forceinline__ void ctab_clear(CommitTable& T)
+__device__ __forceinline__ void ctab_init(CommitTable& T)
 {
 #pragma unroll
     for (int j = 0; j < kCommitSlots / kBlock; ++j) {
@@ -862,6 +868,7 @@ __device__ __forceinline__ void ctab_clear(CommitTable& T)
         T.lo[s] = 0;
         T.hi[s] = 0;
     }
+    if (threadIdx.x == 0) T.n = 0;
 }
 
 __device__ __forceinline__ void ctab_add(CommitTable& T, int c, fx_t x)
@@ -872,51 +879,72 @@ __device__ __forceinline__ void ctab_add(CommitTable& T, int c, fx_t x)
         if (kk == c) break;
         if (kk < 0) {
             const int prev = atomicCAS(&T.key[h], -1, c);
-            if (prev < 0 || prev == c) break;
+            if (prev < 0) {
+                T.list[atomicAdd(&T.n, 1)] = (int)h;
+                break;
+            }
+            if (prev == c) break;
         }
         h = (h + 1) & (kCommitSlots - 1);
     }
     fx_atomic_add(&T.lo[h], &T.hi[h], x);
 }
 
+// Listed slot i of the chunk (i < T.n): its key and sum, and the slot cleared.
+__device__ __forceinline__ int ctab_take(CommitTable& T, int i, fx_t& x)
+{
+    const int s = T.list[i];
+    const int c = T.key[s];
+    x = fx_make(T.lo[s], (int64_t)T.hi[s]);
+    T.key[s] = -1;
+    T.lo[s] = 0;
+    T.hi[s] = 0;
+    return c;
+}
+
 // update_community_weights + update_community_ids of one batch, fused
 // (graph_cuda.cu:1234-1249 then 1191-1196): each thread touches only its own
 // vertex's C/C' entries, so the swap needs no second launch.  C32 mirrors C.
 // The vertices are list[0, n) or, with a null list, [v0, v0 + n); with gid
-// set, only vertices of graphs whose flag is set take part.
+// set, only vertices of graphs whose flag is set take part.  A vertex that
+// did not move has C = C' already, so its swap writes nothing.
 __global__ void __launch_bounds__(kBlock)
 commit_kernel(const BinRow* __restrict__ list, int64_t v0, int64_t n, int64_t* __restrict__ C,
               int64_t* __restrict__ Cn, int* __restrict__ C32, const fx_t* __restrict__ k,
               fx_t* __restrict__ a, const int* __restrict__ gid, const unsigned char* __restrict__ flag)
 {
     __shared__ CommitTable T;
+    ctab_init(T);
     for (int64_t base = blockIdx.x * (int64_t)kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
-        ctab_clear(T);
-        __syncthreads();
         const int64_t i = base + threadIdx.x;
+        int64_t v = 0, src = 0, dst = 0;
+        bool moved = false;
         if (i < n) {
-            const int64_t v = list ? (int64_t)list[i].v : v0 + i;
+            v = list ? (int64_t)list[i].v : v0 + i;
             if (!gid || flag[gid[v]]) {
-                const int64_t src = C[v];
-                const int64_t dst = Cn[v];
-                if (src != dst) {
-                    const fx_t kv = k[v];
-                    ctab_add(T, (int)src, -kv);
-                    ctab_add(T, (int)dst, kv);
-                    C32[v] = (int)dst;
-                }
-                C[v] = dst;
-                Cn[v] = src;
+                src = C[v];
+                dst = Cn[v];
+                moved = src != dst;
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kCommitSlots / kBlock; ++j) {
-            const int s = threadIdx.x + j * kBlock;
-            const int c = T.key[s];
-            const fx_t x = fx_make(T.lo[s], (int64_t)T.hi[s]);
-            if (c >= 0 && x != 0) acc_add(a + c, x);
+        if (!__syncthreads_or(moved)) continue;   // uniform; also orders the previous chunk's clears
+        if (moved) {
+            const fx_t kv = k[v];
+            ctab_add(T, (int)src, -kv);
+            ctab_add(T, (int)dst, kv);
+            C32[v] = (int)dst;
+            C[v] = dst;
+            Cn[v] = src;
         }
+        __syncthreads();
+        const int nk = T.n;
+        for (int j = threadIdx.x; j < nk; j += kBlock) {
+            fx_t x;
+            const int c = ctab_take(T, j, x);
+            if (x != 0) acc_add(a + c, x);
+        }
+        __syncthreads();                 // every thread has read T.n
+        if (threadIdx.x == 0) T.n = 0;
     }
 }
 
@@ -931,28 +959,31 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
     __shared__ CommitTable T;
     __shared__ int wtot[kBlock / 64];
     __shared__ int s_chunk0, s_comm0;
+    ctab_init(T);
     // workgroup-uniform loop: list space is reserved once per workgroup
     // (one atomic on each shared counter, not one per wavefront)
     for (int64_t base = v0 + blockIdx.x * (int64_t)kBlock; base < v1; base += (int64_t)gridDim.x * kBlock) {
-        ctab_clear(T);
-        __syncthreads();
         const int64_t v = base + threadIdx.x;
+        int64_t src = 0, dst = 0;
+        bool moved = false;
+        if (v < v1) {
+            src = C[v];
+            dst = Cn[v];
+            moved = src != dst;
+        }
+        if (!__syncthreads_or(moved)) continue;   // uniform; also orders the previous chunk's clears
         int nchunk = 0;
         int64_t e0 = 0, e1 = 0;
-        if (v < v1) {
-            const int64_t src = C[v];
-            const int64_t dst = Cn[v];
-            if (src != dst) {
-                const fx_t kv = k[v];
-                ctab_add(T, (int)src, -kv);
-                ctab_add(T, (int)dst, kv);
-                C32[v] = (int)dst;
-                e0 = off[v];
-                e1 = off[v + 1];
-                nchunk = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
-            }
+        if (moved) {
+            const fx_t kv = k[v];
+            ctab_add(T, (int)src, -kv);
+            ctab_add(T, (int)dst, kv);
+            C32[v] = (int)dst;
             C[v] = dst;
             Cn[v] = src;
+            e0 = off[v];
+            e1 = off[v + 1];
+            nchunk = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
         }
         int tc;
         const int pc = block_excl_scan_i32(nchunk, wtot, &tc);
@@ -963,17 +994,17 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
             D.chunks[s_chunk0 + pc + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
         }
         // every community a moved vertex left or joined: its total, and a
-        // place in the touched list the first time in this batch
-        constexpr int kPer = kCommitSlots / kBlock;
-        int cs[kPer];
+        // place in the touched list the first time in this batch (at most
+        // 2 * kBlock keys: two per thread)
+        const int nk = T.n;
+        int cs[2] = {-1, -1};
         int nf = 0;
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int s = threadIdx.x + j * kBlock;
-            const int c = T.key[s];
-            cs[j] = -1;
-            if (c >= 0) {
-                const fx_t x = fx_make(T.lo[s], (int64_t)T.hi[s]);
+        for (int j = 0; j < 2; ++j) {
+            const int i = threadIdx.x + j * kBlock;
+            if (i < nk) {
+                fx_t x;
+                const int c = ctab_take(T, i, x);
                 if (x != 0) acc_add(a + c, x);
                 if (atomicExch(&D.touched[c], 1) == 0) {
                     cs[j] = c;
@@ -982,12 +1013,15 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
             }
         }
         int tm;
-        int at = block_excl_scan_i32(nf, wtot, &tm);
-        if (threadIdx.x == 0 && tm) s_comm0 = atomicAdd(&D.counts[2 * b + 1], tm);
+        int at = block_excl_scan_i32(nf, wtot, &tm);   // its barrier: every thread has read T.n
+        if (threadIdx.x == 0) {
+            if (tm) s_comm0 = atomicAdd(&D.counts[2 * b + 1], tm);
+            T.n = 0;
+        }
         __syncthreads();
         at += s_comm0;
 #pragma unroll
-        for (int j = 0; j < kPer; ++j)
+        for (int j = 0; j < 2; ++j)
             if (cs[j] >= 0) D.comms[at++] = cs[j];
     }
 }
