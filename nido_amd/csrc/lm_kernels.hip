@@ -1081,7 +1081,9 @@ void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, 
 void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s)
 {
     if (B.nitems <= 0) return;
-    const unsigned g = grid_for(B.nitems, 1, 2048);
+    // one workgroup per item: the dispatcher hands items to CUs as earlier
+    // ones finish (static round-robin over fewer workgroups was 0.7% slower)
+    const unsigned g = grid_for(B.nitems, 1, 65535);
     // a 1024-slot table keeps the workgroup at ~20 KB of LDS, so it can run
     // beside the short-row workgroups (overlap_long)
     const int size = B.slots == 1024 ? 0 : B.slots == 2048 ? 1 : 2;
