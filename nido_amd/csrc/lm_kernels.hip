@@ -433,10 +433,12 @@ lm_small_kernel(LmArgs A, SmallBins L)
     __shared__ double sself[kBlock / 8];       // e_{v,own} per row, rounded
     // Every index into L is a constant (a dynamically indexed kernel
     // argument would be copied to scratch memory).
+    // Blocks are numbered backwards, so the dispatcher starts the longest
+    // workgroups (delta, key fill, whole-workgroup rows) first.
+    const int b = (int)(gridDim.x - 1 - blockIdx.x);
     int bin = 0;   // 0..5 short-row bins, 6 key fill, 7 the previous batch's modularity delta
 #pragma unroll
-    for (int k = 1; k <= NUM_SMALL_BINS + 1; ++k) bin += (int)blockIdx.x >= L.blk0[k];
-    const int b = (int)blockIdx.x;
+    for (int k = 1; k <= NUM_SMALL_BINS + 1; ++k) bin += b >= L.blk0[k];
 #define NIDO_BIN_ARGS(K) L.verts[K], L.n[K], b - L.blk0[K], L.blk0[K + 1] - L.blk0[K]
     switch (bin) {
     case BIN_THREAD: thread_body<Multi>(A, NIDO_BIN_ARGS(BIN_THREAD)); break;
