@@ -76,10 +76,12 @@ typedef struct {
     int32_t big_min_deg;      /* longer rows: hash-partitioned workgroups (0..512) */
     int32_t big_part_arcs;    /* distinct communities expected per partition, at most (>= 1) */
     int32_t big_key_limit;    /* LDS keys per partition before the exact HBM fallback (1..big_slots/2) */
-    int32_t use_graph;        /* capture each iteration in a hipGraph (0/1) */
+    int32_t use_graph;        /* replay each phase's iterations from a hipGraph: 0 never, 1 for graphs of
+                                 2^24+ arcs (default), 2 always; never while per-kernel events are on */
     int32_t big_slots;        /* LDS table of a long-row workgroup: 1024, 2048 or 4096 slots */
     int32_t split_small;      /* launch every short-row bin as a kernel of its own (0/1; profiling) */
-    int32_t overlap_long;     /* run the long rows of a batch on a second stream, beside the short rows (0/1) */
+    int32_t overlap_long;     /* run the long rows of a batch on a second stream, beside the short rows
+                                 (0/1, default 1; only while iterations are replayed from a hipGraph) */
     int32_t full_modularity;  /* 1: every iteration sweeps all arcs for Q; 0: exact incremental update (0/1) */
 } nido_tuning;
 

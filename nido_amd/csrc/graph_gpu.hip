@@ -348,10 +348,10 @@ void default_tuning(nido_tuning& t)
     t.big_min_deg = (int32_t)kBinMaxDeg[BIN_B256];
     t.big_part_arcs = (int32_t)kBigPartArcs;
     t.big_key_limit = kBigLimit;
-    t.use_graph = 0;
+    t.use_graph = 1;
     t.big_slots = kBigSlots;
     t.split_small = 0;
-    t.overlap_long = 0;
+    t.overlap_long = 1;
     t.full_modularity = 0;
 }
 
@@ -611,6 +611,8 @@ int do_modularity(nido_graph* g, double* q)
     return NIDO_OK;
 }
 
+bool replaying(const nido_graph* g);
+
 int do_lm(nido_graph* g, int b)
 {
     g->committed[(size_t)b] = 0;
@@ -635,7 +637,9 @@ int do_lm(nido_graph* g, int b)
     const SmallBins& L = g->small_bins[(size_t)b];
     const BigArgs& B = g->big_args[(size_t)b];
     const bool split = g->tuning.split_small != 0;
-    if (B.nitems > 0 && g->tuning.overlap_long) {
+    // The second stream pays off only inside a replayed graph, where the
+    // fork and join are graph edges (from the stream: -7% on R-MAT s22).
+    if (B.nitems > 0 && g->tuning.overlap_long && replaying(g)) {
         // fork: the long rows (key fill, then partitions) run on the side
         // stream while the short rows run on the main stream; join before
         // the commit
@@ -730,6 +734,15 @@ int iteration_launches(nido_graph* g, bool inc)
     return NIDO_OK;
 }
 
+constexpr int64_t kGraphMinArcs = int64_t(1) << 24;
+
+// Whether this phase's iterations are replayed from a hipGraph.
+bool replaying(const nido_graph* g)
+{
+    const int ug = g->tuning.use_graph;
+    return !g->prof.on && (g->G > 1 || g->twom != 0) && (ug == 2 || (ug == 1 && g->ne >= kGraphMinArcs));
+}
+
 void drop_iteration_graph(nido_graph* g)
 {
     for (auto& ex : g->iter_exec) {
@@ -743,6 +756,9 @@ int launch_or_replay(nido_graph* g, bool inc);
 // With use_graph set (and no per-kernel events being recorded) the
 // iteration's launches are captured into a hipGraph once per phase and
 // replayed; the kernels' arguments do not change within a phase.
+// use_graph = 1 does so for graphs of kGraphMinArcs arcs or more: a capture
+// per phase costs more than it saves on small graphs (a serving loop's
+// requests ran at 60% of their throughput), 2 always.
 int run_iteration_launches(nido_graph* g)
 {
     // The first iteration from the singleton partition moves most vertices
@@ -759,8 +775,7 @@ int run_iteration_launches(nido_graph* g)
 
 int launch_or_replay(nido_graph* g, bool inc)
 {
-    const bool use = g->tuning.use_graph && !g->prof.on && (g->G > 1 || g->twom != 0);
-    if (!use) return iteration_launches(g, inc);
+    if (!replaying(g)) return iteration_launches(g, inc);
     hipGraphExec_t& exec = g->iter_exec[inc ? 1 : 0];
     if (!exec) {
         HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
@@ -1236,7 +1251,7 @@ int do_autotune(nido_graph* g, int iters, nido_tuning* out)
             t.big_part_arcs = v;
             t.overlap_long = 1;
         });
-    for (int v : {0, 1}) try_value([&](nido_tuning& t) { t.use_graph = v; });
+    for (int v : {0, 2}) try_value([&](nido_tuning& t) { t.use_graph = v; });
     g->prof.on = prof;
     g->tuning = best;
     NIDO_TRY(setup_phase(g));
@@ -1451,6 +1466,7 @@ int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
         return set_error(NIDO_ERR_INVALID, "big_slots must be 1024, 2048 or 4096");
     if (t->big_key_limit < 1 || t->big_key_limit > t->big_slots / 2)
         return set_error(NIDO_ERR_INVALID, "big_key_limit must be in [1, big_slots/2]");
+    if (t->use_graph < 0 || t->use_graph > 2) return set_error(NIDO_ERR_INVALID, "use_graph must be 0, 1 or 2");
     AllocScope scope(g->stream);
     API_GUARD(g->tuning = *t; return setup_phase(g);)
 }

@@ -60,8 +60,8 @@ def test_batch_equals_each_graph_alone(nb):
 
 def test_batch_replayed_from_hip_graphs():
     na = _na()
-    print("case: mixed batch nb=8 use_graph=1", flush=True)
-    _check(na, _mixed_graphs(na), 8, use_graph=1)
+    print("case: mixed batch nb=8 use_graph=2", flush=True)
+    _check(na, _mixed_graphs(na), 8, use_graph=2)
 
 
 def test_batch_of_one_graph_is_the_plain_run():

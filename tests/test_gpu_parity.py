@@ -127,12 +127,13 @@ KNOBS = [
     {"split_small": 1},                                             # one launch per short-row bin
     {"overlap_long": 0},                                            # long rows on the main stream
     {"overlap_long": 0, "split_small": 1},
-    {"use_graph": 1},                                               # iterations replayed from a hipGraph
-    {"use_graph": 1, "overlap_long": 1},
+    {"use_graph": 2},                                               # iterations replayed from a hipGraph
+    {"use_graph": 2, "overlap_long": 1},                            # ... with the long rows on a second stream
+    {"use_graph": 0, "overlap_long": 0},                            # stream launches only
     {"big_slots": 1024, "big_key_limit": 512, "big_part_arcs": 384},
     {"big_slots": 1024, "big_key_limit": 512, "big_part_arcs": 384, "overlap_long": 1},
     {"full_modularity": 1},                                         # every iteration sweeps all arcs for Q
-    {"full_modularity": 1, "use_graph": 1},
+    {"full_modularity": 1, "use_graph": 2},
 ]
 
 
