@@ -334,6 +334,7 @@ struct nido_graph {
     fx_t* d_term = nullptr;              // capacity n0
     fx_t* d_iacc = nullptr;              // 2
     bool inc_ok = false;                 // d_term / d_iacc describe the current partition
+    bool ident = false;                  // C is the singleton partition: set only by do_singleton
     int64_t iters_since_singleton = 0;   // the first iteration moves most vertices: full sweep there
     ModDelta delta() const { return ModDelta{d_dcounts, d_dchunks, d_dcomms, d_touched, d_term, d_iacc}; }
     std::vector<BatchResult> bres;       // per graph, from the last batched run
@@ -381,6 +382,7 @@ int setup_phase(nido_graph* g)
     const auto& off = g->h_off;
     drop_iteration_graph(g);
     g->inc_ok = false;
+    g->ident = false;
     g->maxdeg = max_degree(off.data(), nv);
     g->committed.assign((size_t)g->nb, 0);
     // Batch b of the handle = batch b of every graph it holds (each graph
@@ -583,9 +585,11 @@ void free_graph_arrays(nido_graph* g)
 int do_singleton(nido_graph* g)
 {
     g->inc_ok = false;
+    g->ident = false;
     g->iters_since_singleton = 0;
     launch_singleton(g->d_C, g->d_Cn, g->d_C32, g->d_a, g->d_k, g->nv, g->stream);
     HIP_TRY(hipGetLastError());
+    g->ident = true;
     std::fill(g->committed.begin(), g->committed.end(), 0);
     return NIDO_OK;
 }
@@ -600,7 +604,7 @@ int do_modularity(nido_graph* g, double* q)
         ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
         launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d,
                           g->d_part, g->d_q, g->stream, g->G == 1 ? g->d_term : nullptr,
-                          g->G == 1 ? g->d_iacc : nullptr);
+                          g->G == 1 ? g->d_iacc : nullptr, g->ident);
     }
     g->inc_ok = g->G == 1;
     HIP_TRY(hipGetLastError());
@@ -674,6 +678,7 @@ int do_lm(nido_graph* g, int b)
 int do_commit(nido_graph* g, int b)
 {
     g->inc_ok = false;
+    g->ident = false;
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
         if (g->G == 1)
@@ -691,6 +696,7 @@ int do_commit(nido_graph* g, int b)
 int do_restore(nido_graph* g)
 {
     g->inc_ok = false;
+    g->ident = false;
     launch_commit(0, g->nv, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->stream);
     HIP_TRY(hipGetLastError());
     return NIDO_OK;
@@ -704,6 +710,7 @@ int do_restore(nido_graph* g)
 // deltas) instead of a sweep over every arc.
 int iteration_launches(nido_graph* g, bool inc)
 {
+    g->ident = false;
     const ModDelta D = g->delta();
     for (int b = 0; b < g->nb; ++b) {
         NIDO_TRY(do_lm(g, b));
@@ -775,6 +782,7 @@ int run_iteration_launches(nido_graph* g)
 
 int launch_or_replay(nido_graph* g, bool inc)
 {
+    g->ident = false;
     if (!replaying(g)) return iteration_launches(g, inc);
     hipGraphExec_t& exec = g->iter_exec[inc ? 1 : 0];
     if (!exec) {
@@ -875,6 +883,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
 // aggregation, graph_gpu.cpp:1382-1545, on the device.
 int do_aggregation(nido_graph* g, int* done)
 {
+    g->ident = false;
     hipStream_t s = g->stream;
     const int64_t nv = g->nv;
     int64_t* newid = g->d_work;
@@ -1028,6 +1037,7 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
 {
     const int G = g->G;
     const double t0 = now_s();
+    g->ident = false;
     g->bres.assign((size_t)G, BatchResult{});
     std::vector<unsigned char> alive((size_t)G, 1), act((size_t)G), flag((size_t)G);
     std::vector<int64_t> base((size_t)G, 0);   // subtracted from clus to give graph-local ids

@@ -187,9 +187,10 @@ void launch_commit_filtered(const BinRow* list, int64_t v0, int64_t n, int64_t* 
 // part holds 2 * kPartBlocks partials.  With term/acc set it also leaves
 // every community's square term in term[] and the exact internal weight and
 // sum of squares in acc[0], acc[1] (the state the incremental form updates).
+// ident: the caller guarantees C[v] = v for every v (C is then not read).
 void launch_modularity(const int* src, const int* col, const double* w, const int* C,
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
-                       hipStream_t s, fx_t* term = nullptr, fx_t* acc = nullptr);
+                       hipStream_t s, fx_t* term = nullptr, fx_t* acc = nullptr, bool ident = false);
 
 void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* Cn, int* C32, const fx_t* k,
                            fx_t* a, const int64_t* off, const ModDelta& D, hipStream_t s);

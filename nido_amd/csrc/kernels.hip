@@ -137,10 +137,14 @@ src_items_kernel(const RowItem* __restrict__ items, int64_t n, int* __restrict__
 // Modularity (graph_cuda.cu:2078-2134): Q = I/(2m) - sum_c (a_c/(2m))^2.
 // Arc-parallel over the whole CSR (tiles of 256 x kModUnroll arcs, every load
 // coalesced except the two C gathers), then the community sweep; each block
-// leaves two partials and mod_finalize_kernel adds them up.
+// leaves two partials and mod_finalize_kernel adds them up.  Ident: C is
+// the identity (the singleton partition every phase starts from), so an arc
+// is internal exactly when it is a self arc and the C gathers are left out;
+// the sums are the same integers either way.
 // ---------------------------------------------------------------------------
 constexpr int kModUnroll = 8;
 
+template <bool Ident>
 __global__ void __launch_bounds__(kBlock)
 mod_kernel(const int* __restrict__ src, const int* __restrict__ col,
            const double* __restrict__ w, const int* __restrict__ C, int64_t ne,
@@ -163,7 +167,8 @@ mod_kernel(const int* __restrict__ src, const int* __restrict__ col,
 #pragma unroll
         for (int j = 0; j < kModUnroll; ++j) {
             const int64_t e = t0 + j * kBlock + threadIdx.x;
-            in[j] = e < ne && C[s[j]] == C[u[j]];    // self arcs are internal
+            if constexpr (Ident) in[j] = e < ne && s[j] == u[j];
+            else in[j] = e < ne && C[s[j]] == C[u[j]];    // self arcs are internal
         }
 #pragma unroll
         for (int j = 0; j < kModUnroll; ++j)
@@ -558,12 +563,13 @@ void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s)
 
 void launch_modularity(const int* src, const int* col, const double* w, const int* C,
                        int64_t ne, const fx_t* a, int64_t nv, double twom_d, fx_t* part, double* q,
-                       hipStream_t s, fx_t* term, fx_t* acc)
+                       hipStream_t s, fx_t* term, fx_t* acc, bool ident)
 {
     const int64_t tiles = (ne + (int64_t)kBlock * kModUnroll - 1) / ((int64_t)kBlock * kModUnroll);
     const int64_t need = tiles > (nv + kBlock - 1) / kBlock ? tiles : (nv + kBlock - 1) / kBlock;
     const unsigned g = grid_for(need, 1, kPartBlocks);
-    mod_kernel<<<g, kBlock, 0, s>>>(src, col, w, C, ne, a, nv, twom_d, part, term);
+    if (ident) mod_kernel<true><<<g, kBlock, 0, s>>>(src, col, w, C, ne, a, nv, twom_d, part, term);
+    else mod_kernel<false><<<g, kBlock, 0, s>>>(src, col, w, C, ne, a, nv, twom_d, part, term);
     mod_finalize_kernel<<<1, kBlock, 0, s>>>(part, (int)g, twom_d, q, acc);
 }
 
