@@ -963,7 +963,9 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
     __shared__ int s_chunk0, s_comm0;
     ctab_init(T);
     // workgroup-uniform loop: list space is reserved once per workgroup
-    // (one atomic on each shared counter, not one per wavefront)
+    // and chunk, both lists with one 64-bit atomic on the batch's counter
+    // pair {chunks, communities} (no carry crosses: each count stays below
+    // 2^31)
     for (int64_t base = v0 + blockIdx.x * (int64_t)kBlock; base < v1; base += (int64_t)gridDim.x * kBlock) {
         const int64_t v = base + threadIdx.x;
         int64_t src = 0, dst = 0;
@@ -987,17 +989,11 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
             e1 = off[v + 1];
             nchunk = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
         }
-        int tc;
-        const int pc = block_excl_scan_i32(nchunk, wtot, &tc);
-        if (threadIdx.x == 0 && tc) s_chunk0 = atomicAdd(&D.counts[2 * b], tc);
-        __syncthreads();                  // also: the table is complete
-        for (int i = 0; i < nchunk; ++i) {
-            const int64_t s0 = e0 + (int64_t)i * kDeltaArcs;
-            D.chunks[s_chunk0 + pc + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
-        }
+        __syncthreads();                  // the table is complete
         // every community a moved vertex left or joined: its total, and a
         // place in the touched list the first time in this batch (at most
-        // 2 * kBlock keys: two per thread)
+        // 2 * kBlock keys: two per thread); a flag already seen set needs no
+        // exchange (flags go back to 0 only after the batch's commit)
         const int nk = T.n;
         int cs[2] = {-1, -1};
         int nf = 0;
@@ -1008,19 +1004,32 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
                 fx_t x;
                 const int c = ctab_take(T, i, x);
                 if (x != 0) acc_add(a + c, x);
-                if (atomicExch(&D.touched[c], 1) == 0) {
+                if (ld_i32_agent(&D.touched[c]) == 0 && atomicExch(&D.touched[c], 1) == 0) {
                     cs[j] = c;
                     ++nf;
                 }
             }
         }
+        int tc;
+        const int pc = block_excl_scan_i32(nchunk, wtot, &tc);   // its barrier: every thread has read T.n
+        __syncthreads();                  // wtot read before the next scan writes it
         int tm;
-        int at = block_excl_scan_i32(nf, wtot, &tm);   // its barrier: every thread has read T.n
+        int at = block_excl_scan_i32(nf, wtot, &tm);
         if (threadIdx.x == 0) {
-            if (tm) s_comm0 = atomicAdd(&D.counts[2 * b + 1], tm);
+            if (tc | tm) {
+                const unsigned long long old =
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&D.counts[2 * b]),
+                              (unsigned long long)(unsigned)tc | ((unsigned long long)(unsigned)tm << 32));
+                s_chunk0 = (int)(unsigned)old;
+                s_comm0 = (int)(old >> 32);
+            }
             T.n = 0;
         }
         __syncthreads();
+        for (int i = 0; i < nchunk; ++i) {
+            const int64_t s0 = e0 + (int64_t)i * kDeltaArcs;
+            D.chunks[s_chunk0 + pc + i] = RowItem{v, s0, s0 + kDeltaArcs < e1 ? s0 + kDeltaArcs : e1};
+        }
         at += s_comm0;
 #pragma unroll
         for (int j = 0; j < 2; ++j)
