@@ -72,7 +72,7 @@ typedef struct {
 /* Knobs of the degree-binned local-moving kernels (results do not depend
  * on them; only speed does). */
 typedef struct {
-    int32_t thread_max_deg;   /* rows up to this length: one lane per vertex (0..8) */
+    int32_t thread_max_deg;   /* rows up to this length: one lane per vertex (0..2) */
     int32_t big_min_deg;      /* longer rows: hash-partitioned workgroups (0..512) */
     int32_t big_part_arcs;    /* distinct communities expected per partition, at most (>= 1) */
     int32_t big_key_limit;    /* LDS keys per partition before the exact HBM fallback (1..big_slots/2) */

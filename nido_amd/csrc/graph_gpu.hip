@@ -1241,7 +1241,7 @@ int do_autotune(nido_graph* g, int iters, nido_tuning* out)
             tb = x;
         }
     };
-    for (int v : {4, 6, 8}) try_value([&](nido_tuning& t) { t.thread_max_deg = v; });
+    for (int v : {0, 1, 2}) try_value([&](nido_tuning& t) { t.thread_max_deg = v; });
     for (int v : {128, 256, 512}) try_value([&](nido_tuning& t) { t.big_min_deg = v; });
     // table size and partition size move together: a partition of up to
     // big_part_arcs arcs fits a table of twice as many slots

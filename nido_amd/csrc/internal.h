@@ -21,8 +21,11 @@ enum LmBin : int {
 };
 constexpr int NUM_SMALL_BINS = BIN_BIG;
 
-constexpr int kThreadMaxDeg = 8;          // registers hold up to this many arcs
-constexpr int64_t kBinMaxDeg[NUM_BINS] = {8, 16, 32, 64, 128, 512, INT64_MAX};
+// Registers hold up to this many arcs.  Rows longer than two arcs run
+// faster with 8 lanes each (one arc per lane, LDS table) than with one lane
+// holding the whole row: R-MAT s22 end to end, +5% over a bound of 8.
+constexpr int kThreadMaxDeg = 2;
+constexpr int64_t kBinMaxDeg[NUM_BINS] = {kThreadMaxDeg, 16, 32, 64, 128, 512, INT64_MAX};
 constexpr int64_t kBigPartArcs = 1024;    // expected distinct keys per partition, at most
 constexpr int kBigSlots = 2048;           // LDS table of a partition workgroup (40/48 KB: 4 or 3 per CU)
 constexpr int kBigLimit = kBigSlots / 2;  // distinct keys before the exact HBM fallback

@@ -9,7 +9,7 @@
 // community id; the FP64 gain argmax (min-id ties) is taken over the table.
 //
 // Degree bins (rows of d arcs):
-//   d <= 8      one lane per vertex, neighbours in registers
+//   d <= 2      one lane per vertex, neighbours in registers
 //   d <= 16/32/64/128   8/16/32/64 lanes per vertex, LDS table of 2d..4d slots
 //   d <= 512    one workgroup per vertex, 1024-slot LDS table
 //   d >  512    long rows, split by a hash of the community id into

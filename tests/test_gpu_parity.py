@@ -120,7 +120,7 @@ def test_rmat_full_run(scale, ef, weighted, nb):
 
 KNOBS = [
     {"thread_max_deg": 0},
-    {"thread_max_deg": 4},
+    {"thread_max_deg": 1},
     {"big_min_deg": 16, "big_part_arcs": 4},                        # many partitions per row
     {"big_min_deg": 0, "big_part_arcs": 1},                         # every row partitioned
     {"big_min_deg": 32, "big_part_arcs": 64, "big_key_limit": 2},   # forces the HBM fallback
