@@ -77,7 +77,7 @@ typedef struct {
     int32_t big_part_arcs;    /* distinct communities expected per partition, at most (>= 1) */
     int32_t big_key_limit;    /* LDS keys per partition before the exact HBM fallback (1..big_slots/2) */
     int32_t use_graph;        /* replay each phase's iterations from a hipGraph: 0 never, 1 for graphs of
-                                 2^24+ arcs (default), 2 always; never while per-kernel events are on */
+                                 2^28+ arcs (default), 2 always; never while per-kernel events are on */
     int32_t big_slots;        /* LDS table of a long-row workgroup: 1024, 2048 or 4096 slots */
     int32_t split_small;      /* launch every short-row bin as a kernel of its own (0/1; profiling) */
     int32_t overlap_long;     /* run the long rows of a batch on a second stream, beside the short rows

@@ -741,7 +741,7 @@ int iteration_launches(nido_graph* g, bool inc)
     return NIDO_OK;
 }
 
-constexpr int64_t kGraphMinArcs = int64_t(1) << 24;
+constexpr int64_t kGraphMinArcs = int64_t(1) << 28;
 
 // Whether this phase's iterations are replayed from a hipGraph.
 bool replaying(const nido_graph* g)
@@ -765,7 +765,10 @@ int launch_or_replay(nido_graph* g, bool inc);
 // replayed; the kernels' arguments do not change within a phase.
 // use_graph = 1 does so for graphs of kGraphMinArcs arcs or more: a capture
 // per phase costs more than it saves on small graphs (a serving loop's
-// requests ran at 60% of their throughput), 2 always.
+// requests ran at 60% of their throughput), and a graph of 128 M arcs
+// (R-MAT scale 22) ran 1.8% slower replayed with its long rows beside the
+// short rows than from the stream, a graph of 1 G arcs (Chung-Lu 2^25)
+// 0.9% faster; 2 always.
 int run_iteration_launches(nido_graph* g)
 {
     // The first iteration from the singleton partition moves most vertices
