@@ -269,6 +269,42 @@ def test_profiling_events_sample_every_stride_th_launch():
     g.close()
 
 
+@pytest.mark.parametrize("use_graph", [0, 2])
+def test_modularity_after_singleton_iteration_and_restore(use_graph):
+    """compute_modularity right after singleton_partition counts self arcs
+    alone (C is the identity there); after an iteration, a restore or a
+    second singleton reset it must match the oracle's full sweep again."""
+    na = _na()
+    off, col, w = na.gen_rmat(13, 8, 0.57, 0.19, 0.19, True, 9).arrays()
+    nb = 4
+    print(f"case: modularity after singleton/iteration/restore, rmat13 -b {nb} use_graph={use_graph}", flush=True)
+    g = na.GraphGPU((off, col, w), nb)
+    g.set_tuning(use_graph=use_graph)
+    o = oracle.Oracle(off, col, w, nb)
+    for _ in range(2):
+        g.singleton_partition()
+        o.singleton_partition()
+        q = g.compute_modularity()
+        assert q == o.compute_modularity()
+        qa, dq, qe = g.louvain_iteration(q)
+        ra, rdq, rqe = o.louvain_iteration(q)
+        assert (qa, dq, qe) == (ra, rdq, rqe)
+        assert g.compute_modularity() == o.compute_modularity()
+        for b in range(nb):
+            g.louvain_update(b)
+            o.louvain_update(b)
+            g.update_community_weights(b)
+            g.update_community_ids(b)
+            o.update_community_weights(b)
+            o.update_community_ids(b)
+        assert g.compute_modularity() == o.compute_modularity()
+        g.restore_community()
+        o.restore_community()
+        assert np.array_equal(g.communities(), o.communities())
+        assert g.compute_modularity() == o.compute_modularity()
+    g.close()
+
+
 def _hubs_at_bin_edges(seed=17):
     """Hubs whose degrees sit on both sides of every bin and partition edge,
     each joined to its own leaves (so in the first iteration every hub sees
