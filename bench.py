@@ -240,7 +240,8 @@ def bench_serve(args):
     log(f"serve: {len(t_arr)} arrivals over {args.duration:g}s at {args.rate:g}/s; {len(pool)} graphs generated "
         f"in {time.perf_counter() - t0:.1f}s")
     # warm-up request
-    srv = na.Server(slots=args.slots, nbatches=args.batches, hbm_budget=args.hbm_budget << 30, tol=args.tol)
+    srv = na.Server(slots=args.slots, nbatches=args.batches, hbm_budget=args.hbm_budget << 30, tol=args.tol,
+                    group=args.group)
     rid = srv.submit(pool[min(pool)], borrow=True)
     srv.wait(rid)
     srv.release(rid)
@@ -272,7 +273,8 @@ def bench_serve(args):
         "config": {"workload": f"serving: Poisson arrivals {args.rate:g}/s for {args.duration:g}s, R-MAT "
                                f"(0.57,0.19,0.19,0.05) scale U[10,20], edge factor {{8,16,32}}, unit weights, "
                                f"-b {args.batches} -t {args.tol:g}, full runs",
-                   "slots": args.slots, "hbm_budget_gib": args.hbm_budget, "max_queue": args.max_queue,
+                   "slots": args.slots, "group": args.group, "hbm_budget_gib": args.hbm_budget,
+                   "max_queue": args.max_queue,
                    "arrivals": int(len(t_arr)), "admitted": len(ids), "turned_away": turned_away,
                    "arcs_completed": int(sum(arcs)), "peak_concurrent": peak},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
@@ -289,7 +291,9 @@ def main():
     ap.add_argument("--workload", choices=["rmat", "chunglu", "batched", "serve"], default="rmat")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
-    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent requests")
+    ap.add_argument("--slots", type=int, default=8, help="serve: workers (streams)")
+    ap.add_argument("--group", type=int, default=64,
+                    help="serve: small requests one worker runs together in a batched handle (1: none)")
     ap.add_argument("--hbm-budget", type=int, default=200, help="serve: GiB of HBM for admitted requests")
     ap.add_argument("--max-queue", type=int, default=64, help="serve: waiting requests before arrivals are turned away")
     ap.add_argument("--graphs", type=int, default=4096, help="batched workload: number of graphs")

@@ -280,12 +280,14 @@ int nido_batch_clustering(const nido_graph* g, int graph, int64_t* comm);
  * is the one it gets alone. */
 typedef struct nido_server nido_server;
 typedef struct {
-    int32_t slots;         /* concurrent requests (1..64) */
+    int32_t slots;         /* workers, each with its own stream (1..64) */
     int32_t nbatches;      /* -b */
     int64_t hbm_budget;    /* bytes of device memory the admitted requests may use together */
     int64_t max_loops;     /* -i */
     double tol;            /* -t */
     double tol_phase;
+    int32_t group;         /* small requests (up to 2^22 arcs) a worker runs together in one batched
+                              handle, at most (0 or 1: one request per worker; up to 4096) */
 } nido_serve_config;
 int nido_serve_create(const nido_serve_config* cfg, nido_server** out);
 int64_t nido_serve_footprint(int64_t nv, int64_t ne);   /* admission estimate, bytes */

@@ -136,7 +136,8 @@ _sig("nido_batch_clustering", C.c_int, C.c_void_p, C.c_int, _pi64)
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("nbatches", C.c_int32), ("hbm_budget", C.c_int64),
-                ("max_loops", C.c_int64), ("tol", C.c_double), ("tol_phase", C.c_double)]
+                ("max_loops", C.c_int64), ("tol", C.c_double), ("tol_phase", C.c_double),
+                ("group", C.c_int32)]
 
 
 _sig("nido_serve_create", C.c_int, C.POINTER(ServeConfig), C.POINTER(C.c_void_p))
@@ -586,8 +587,10 @@ class Server:
     the graph and returns at once; wait() blocks for that request."""
 
     def __init__(self, slots: int = 8, nbatches: int = DEFAULT_BATCHES, hbm_budget: int = 200 << 30,
-                 max_loops: int = DEFAULT_ITERATIONS, tol: float = DEFAULT_THRESHOLD, tol_phase: float | None = None):
-        cfg = ServeConfig(slots, nbatches, hbm_budget, max_loops, tol, tol if tol_phase is None else tol_phase)
+                 max_loops: int = DEFAULT_ITERATIONS, tol: float = DEFAULT_THRESHOLD, tol_phase: float | None = None,
+                 group: int = 1):
+        cfg = ServeConfig(slots, nbatches, hbm_budget, max_loops, tol, tol if tol_phase is None else tol_phase,
+                          group)
         h = C.c_void_p()
         _check(lib.nido_serve_create(C.byref(cfg), C.byref(h)), "nido_serve_create")
         self._h = h

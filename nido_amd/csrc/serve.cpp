@@ -7,8 +7,13 @@
 // whole LouvainGPU::run (louvain_gpu.cpp:6-139) for its request there, with
 // every phase's state resident on the device, so admitted requests overlap
 // on the GPU.  A finished request frees its slot and its share of the
-// budget, and the next ones are admitted at once.  A request's result does
-// not depend on what runs beside it.
+// budget, and the next ones are admitted at once.  With `group` > 1 a worker
+// that takes a small request (up to kGroupArcs arcs) also takes the small
+// requests queued right behind it, up to `group` of them and while their
+// footprints fit, and runs them together in one batched handle
+// (nido_batch_create / nido_batch_run): one launch sequence serves them all,
+// where one request would leave most of the GPU idle.  A request's result
+// does not depend on what runs beside it, in a group or not.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -28,6 +33,9 @@
 using nido::set_error;
 
 namespace {
+
+constexpr int64_t kGroupArcs = int64_t(1) << 22;        // requests up to this size are grouped
+constexpr int64_t kGroupTotalArcs = int64_t(1) << 26;   // arcs of one group, at most
 
 double now_s()
 {
@@ -69,13 +77,86 @@ struct nido_server {
     bool stop = false;
 
     void worker(int slot);
+    void run_one(Request* r, hipStream_t st);
+    void run_group(const std::vector<Request*>& grp, hipStream_t st);
 };
+
+namespace {
+
+const int64_t* req_off(const Request* r) { return r->boff ? r->boff : r->off.data(); }
+const int64_t* req_col(const Request* r) { return r->boff ? r->bcol : r->col.data(); }
+const double* req_w(const Request* r) { return r->boff ? r->bw : r->w.data(); }
+
+}  // namespace
+
+void nido_server::run_one(Request* r, hipStream_t st)
+{
+    nido_graph* g = nullptr;
+    int rc = nido_graph_create(r->nv, r->ne, req_off(r), req_col(r), req_w(r), cfg.nbatches, st, &g);
+    if (rc == NIDO_OK) rc = nido_louvain_run(g, cfg.max_loops, cfg.tol, cfg.tol_phase, 0, 0, &r->st);
+    if (rc == NIDO_OK) {
+        r->clustering.resize((size_t)r->nv);
+        rc = r->nv ? nido_get_clustering(g, r->clustering.data()) : NIDO_OK;
+    }
+    if (rc != NIDO_OK) r->err = nido_last_error();
+    if (g) nido_graph_destroy(g);
+    r->rc = rc;
+}
+
+// The group's graphs side by side in one CSR (graph i owns vertices
+// [vptr[i], vptr[i+1])), run to completion together; each request then
+// takes its own graph's statistics and clustering.
+void nido_server::run_group(const std::vector<Request*>& grp, hipStream_t st)
+{
+    const int G = (int)grp.size();
+    std::vector<int64_t> vptr((size_t)G + 1, 0);
+    int64_t ne = 0;
+    for (int i = 0; i < G; ++i) {
+        vptr[(size_t)i + 1] = vptr[(size_t)i] + grp[(size_t)i]->nv;
+        ne += grp[(size_t)i]->ne;
+    }
+    std::vector<int64_t> off((size_t)vptr[(size_t)G] + 1), col((size_t)ne);
+    std::vector<double> w((size_t)ne);
+    int64_t e = 0;
+    for (int i = 0; i < G; ++i) {
+        const Request* r = grp[(size_t)i];
+        const int64_t* ro = req_off(r);
+        const int64_t* rc = req_col(r);
+        const double* rw = req_w(r);
+        const int64_t v0 = vptr[(size_t)i];
+        for (int64_t v = 0; v < r->nv; ++v) off[(size_t)(v0 + v)] = e + (ro[v] - ro[0]);
+        for (int64_t j = 0; j < r->ne; ++j) {
+            col[(size_t)(e + j)] = rc[ro[0] + j] + v0;
+            w[(size_t)(e + j)] = rw[ro[0] + j];
+        }
+        e += r->ne;
+    }
+    off[(size_t)vptr[(size_t)G]] = e;
+    nido_graph* g = nullptr;
+    nido_run_stats all{};
+    int rc = nido_batch_create(G, vptr.data(), off.data(), col.data(), w.data(), cfg.nbatches, st, &g);
+    if (rc == NIDO_OK) rc = nido_batch_run(g, cfg.max_loops, cfg.tol, cfg.tol_phase, &all);
+    const std::string err = rc != NIDO_OK ? std::string(nido_last_error()) : std::string();
+    for (int i = 0; i < G; ++i) {
+        Request* r = grp[(size_t)i];
+        int ri = rc;
+        if (ri == NIDO_OK) ri = nido_batch_graph_stats(g, i, &r->st);
+        if (ri == NIDO_OK) {
+            r->clustering.resize((size_t)r->nv);
+            ri = r->nv ? nido_batch_clustering(g, i, r->clustering.data()) : NIDO_OK;
+        }
+        r->rc = ri;
+        if (ri != NIDO_OK) r->err = rc != NIDO_OK ? err : std::string(nido_last_error());
+    }
+    if (g) nido_graph_destroy(g);
+}
 
 void nido_server::worker(int slot)
 {
     (void)hipSetDevice(device);
     for (;;) {
-        Request* r = nullptr;
+        std::vector<Request*> grp;
+        int64_t fp = 0;
         {
             std::unique_lock<std::mutex> lock(mu);
             // a request fits once enough running ones finish (its footprint
@@ -85,39 +166,43 @@ void nido_server::worker(int slot)
                 return (!queue.empty() && queue.front()->footprint <= budget_free) || (stop && queue.empty());
             });
             if (queue.empty()) return;
-            r = queue.front();
-            queue.pop_front();
-            budget_free -= r->footprint;
-            r->state = 1;
-            ++running;
+            // the head request, then (if it is small) the small ones right
+            // behind it, in arrival order, while they fit
+            int64_t arcs = 0;
+            do {
+                Request* r = queue.front();
+                if (!grp.empty() && (r->ne > kGroupArcs || fp + r->footprint > budget_free ||
+                                     arcs + r->ne > kGroupTotalArcs))
+                    break;
+                queue.pop_front();
+                grp.push_back(r);
+                fp += r->footprint;
+                arcs += r->ne;
+                r->state = 1;
+            } while (grp.front()->ne <= kGroupArcs && (int)grp.size() < cfg.group && !queue.empty());
+            budget_free -= fp;
+            running += (int64_t)grp.size();
             if (running > peak_running) peak_running = running;
         }
-        r->t_start = now_s();
-        nido_graph* g = nullptr;
-        const bool borrowed = r->boff != nullptr;
-        int rc = nido_graph_create(r->nv, r->ne, borrowed ? r->boff : r->off.data(),
-                                   borrowed ? r->bcol : r->col.data(), borrowed ? r->bw : r->w.data(),
-                                   cfg.nbatches, streams[(size_t)slot], &g);
-        if (rc == NIDO_OK)
-            rc = nido_louvain_run(g, cfg.max_loops, cfg.tol, cfg.tol_phase, 0, 0, &r->st);
-        if (rc == NIDO_OK) {
-            r->clustering.resize((size_t)r->nv);
-            rc = r->nv ? nido_get_clustering(g, r->clustering.data()) : NIDO_OK;
+        const double t0 = now_s();
+        for (Request* r : grp) r->t_start = t0;
+        if (grp.size() == 1) run_one(grp[0], streams[(size_t)slot]);
+        else run_group(grp, streams[(size_t)slot]);
+        for (Request* r : grp) {   // the input is no longer needed
+            std::vector<int64_t>().swap(r->off);
+            std::vector<int64_t>().swap(r->col);
+            std::vector<double>().swap(r->w);
         }
-        if (rc != NIDO_OK) r->err = nido_last_error();
-        if (g) nido_graph_destroy(g);
-        r->rc = rc;
-        // the input is no longer needed
-        std::vector<int64_t>().swap(r->off);
-        std::vector<int64_t>().swap(r->col);
-        std::vector<double>().swap(r->w);
         {
             std::lock_guard<std::mutex> lock(mu);
-            r->t_end = now_s();
-            r->state = 2;
-            budget_free += r->footprint;
-            --running;
-            ++completed;
+            const double t1 = now_s();
+            for (Request* r : grp) {
+                r->t_end = t1;
+                r->state = 2;
+            }
+            budget_free += fp;
+            running -= (int64_t)grp.size();
+            completed += (int64_t)grp.size();
         }
         cv_done.notify_all();
         cv_work.notify_all();
@@ -140,6 +225,7 @@ int nido_serve_create(const nido_serve_config* cfg, nido_server** out)
     if (cfg->slots < 1 || cfg->slots > 64) return set_error(NIDO_ERR_INVALID, "slots must be in [1, 64]");
     if (cfg->nbatches < 1) return set_error(NIDO_ERR_INVALID, "nbatches must be >= 1");
     if (cfg->hbm_budget <= 0) return set_error(NIDO_ERR_INVALID, "hbm_budget must be > 0");
+    if (cfg->group < 0 || cfg->group > 4096) return set_error(NIDO_ERR_INVALID, "group must be in [0, 4096]");
     nido_server* s = new (std::nothrow) nido_server;
     if (!s) return set_error(NIDO_ERR_NOMEM, "out of host memory");
     s->cfg = *cfg;

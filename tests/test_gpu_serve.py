@@ -1,5 +1,6 @@
 """Request loop (nido_serve_*): requests that arrive while others run are
-admitted as slots and the HBM budget free up and overlap on the device; each
+admitted as slots and the HBM budget free up and overlap on the device,
+small ones grouped into one batched handle per worker when `group` > 1; each
 result must equal the result of the same graph run alone."""
 import numpy as np
 import pytest
@@ -25,12 +26,13 @@ def _requests(na):
     return gs
 
 
-@pytest.mark.parametrize("slots,budget", [(4, 200 << 30), (3, 150 << 20)])
-def test_each_result_is_the_same_alone_and_in_the_mix(slots, budget):
+@pytest.mark.parametrize("slots,budget,group", [(4, 200 << 30, 1), (3, 150 << 20, 1), (2, 200 << 30, 64),
+                                                (3, 150 << 20, 8), (1, 200 << 30, 4)])
+def test_each_result_is_the_same_alone_and_in_the_mix(slots, budget, group):
     na = _na()
     reqs = _requests(na)
-    print(f"case: serve {len(reqs)} requests, {slots} slots, budget {budget}", flush=True)
-    srv = na.Server(slots=slots, nbatches=32, hbm_budget=budget)
+    print(f"case: serve {len(reqs)} requests, {slots} slots, budget {budget}, group {group}", flush=True)
+    srv = na.Server(slots=slots, nbatches=32, hbm_budget=budget, group=group)
     ids = [srv.submit(g) for g in reqs]
     got = []
     for rid in ids:
@@ -40,7 +42,8 @@ def test_each_result_is_the_same_alone_and_in_the_mix(slots, budget):
         srv.release(rid)
     status = srv.status()
     assert status["completed"] == len(reqs) and status["queued"] == 0
-    assert 1 <= status["peak_running"] <= slots
+    # running counts requests: a worker's group holds up to `group` of them
+    assert 1 <= status["peak_running"] <= slots * group
     srv.close()
     for i, (off, col, w) in enumerate(reqs):
         alone = run_gpu(off, col, w, 32)
