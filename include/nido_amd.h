@@ -286,7 +286,7 @@ typedef struct {
     int64_t max_loops;     /* -i */
     double tol;            /* -t */
     double tol_phase;
-    int32_t group;         /* small requests (up to 2^22 arcs) a worker runs together in one batched
+    int32_t group;         /* small requests (up to 2^23 arcs) a worker runs together in one batched
                               handle, at most (0 or 1: one request per worker; up to 4096) */
 } nido_serve_config;
 int nido_serve_create(const nido_serve_config* cfg, nido_server** out);

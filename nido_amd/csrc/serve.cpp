@@ -34,7 +34,7 @@ using nido::set_error;
 
 namespace {
 
-constexpr int64_t kGroupArcs = int64_t(1) << 22;        // requests up to this size are grouped
+constexpr int64_t kGroupArcs = int64_t(1) << 23;        // requests up to this size are grouped
 constexpr int64_t kGroupTotalArcs = int64_t(1) << 26;   // arcs of one group, at most
 
 double now_s()
