@@ -26,14 +26,16 @@ def _requests(na):
     return gs
 
 
-@pytest.mark.parametrize("slots,budget,group", [(4, 200 << 30, 1), (3, 150 << 20, 1), (2, 200 << 30, 64),
-                                                (3, 150 << 20, 8), (1, 200 << 30, 4)])
-def test_each_result_is_the_same_alone_and_in_the_mix(slots, budget, group):
+@pytest.mark.parametrize("slots,budget,group,borrow", [(4, 200 << 30, 1, False), (3, 150 << 20, 1, False),
+                                                       (2, 200 << 30, 64, False), (2, 200 << 30, 64, True),
+                                                       (3, 150 << 20, 8, False), (1, 200 << 30, 4, True)])
+def test_each_result_is_the_same_alone_and_in_the_mix(slots, budget, group, borrow):
     na = _na()
     reqs = _requests(na)
-    print(f"case: serve {len(reqs)} requests, {slots} slots, budget {budget}, group {group}", flush=True)
+    print(f"case: serve {len(reqs)} requests, {slots} slots, budget {budget}, group {group}, borrow {borrow}",
+          flush=True)
     srv = na.Server(slots=slots, nbatches=32, hbm_budget=budget, group=group)
-    ids = [srv.submit(g) for g in reqs]
+    ids = [srv.submit(g, borrow=borrow) for g in reqs]
     got = []
     for rid in ids:
         st, lat = srv.wait(rid)
