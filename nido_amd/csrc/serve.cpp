@@ -121,12 +121,12 @@ void nido_server::run_group(const std::vector<Request*>& grp, hipStream_t st)
     for (int i = 0; i < G; ++i) {
         const Request* r = grp[(size_t)i];
         const int64_t* ro = req_off(r);
-        const int64_t* rc = req_col(r);
+        const int64_t* rcol = req_col(r);
         const double* rw = req_w(r);
         const int64_t v0 = vptr[(size_t)i];
         for (int64_t v = 0; v < r->nv; ++v) off[(size_t)(v0 + v)] = e + (ro[v] - ro[0]);
         for (int64_t j = 0; j < r->ne; ++j) {
-            col[(size_t)(e + j)] = rc[ro[0] + j] + v0;
+            col[(size_t)(e + j)] = rcol[ro[0] + j] + v0;
             w[(size_t)(e + j)] = rw[ro[0] + j];
         }
         e += r->ne;
