@@ -1341,9 +1341,11 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     // Clustering::commIds_ starts as the identity (clustering.cpp:4-10)
     launch_iota_i64(g->d_clus, nv, s);
     if (staged_h2d(g->d_off, g->h_off.data(), sizeof(int64_t) * (size_t)(nv + 1), s) != hipSuccess ||
-        (ne > 0 && staged_h2d(g->d_col, col + base, sizeof(int64_t) * (size_t)ne, s) != hipSuccess) ||
+        (ne > 0 && staged_h2d_ids(g->d_col32, col + base, ne, s) != hipSuccess) ||
         (ne > 0 && staged_h2d(g->d_w, w + base, sizeof(double) * (size_t)ne, s) != hipSuccess))
         return fail(set_error(NIDO_ERR_HIP, "upload failed"));
+    // ids crossed the host link as int32 (all below nv < 2^31 - 1, checked)
+    launch_widen_ids(g->d_col32, g->d_col, ne, s);
     if (G > 1) {
         g->G = G;
         g->vb.assign(vptr, vptr + G + 1);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace nido {
 
@@ -36,6 +37,42 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
             std::memcpy(buf[i] + o, in + at + o, std::min(kPiece, n - o));
         }
         e = hipMemcpyAsync(out + at, buf[i], n, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(done[i], s);
+    }
+    const hipError_t w = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = w;
+    for (int i = 0; i < 2; ++i) {
+        if (done[i]) (void)hipEventDestroy(done[i]);
+        if (buf[i]) (void)hipHostFree(buf[i]);
+    }
+    return e;
+}
+
+hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s)
+{
+    constexpr int64_t kChunk = int64_t(16) << 20;   // ids per pinned buffer (64 MiB)
+    if (n <= kChunk) {   // small: one pageable copy, no pinned allocation
+        std::vector<int> t((size_t)n);
+        for (int64_t j = 0; j < n; ++j) t[(size_t)j] = (int)src[j];
+        hipError_t e = hipMemcpyAsync(dst, t.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s);
+        return e != hipSuccess ? e : hipStreamSynchronize(s);
+    }
+    int* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), sizeof(int) * (size_t)kChunk);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+    }
+    for (int64_t at = 0, i = 0; e == hipSuccess && at < n; at += kChunk, i ^= 1) {
+        const int64_t m = std::min(kChunk, n - at);
+        e = hipEventSynchronize(done[i]);      // the buffer's previous copy has landed
+        if (e != hipSuccess) break;
+        int* b = buf[i];
+        const int64_t* in = src + at;
+#pragma omp parallel for schedule(static) if (m >= (int64_t(1) << 20))
+        for (int64_t j = 0; j < m; ++j) b[j] = (int)in[j];
+        e = hipMemcpyAsync(dst + at, b, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(done[i], s);
     }
     const hipError_t w = hipStreamSynchronize(s);

@@ -170,6 +170,7 @@ void launch_singleton(int64_t* C, int64_t* Cn, int* C32, fx_t* a, const fx_t* k,
                       hipStream_t s);
 // col32[e] = (int)col[e]
 void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s);
+void launch_widen_ids(const int* col32, int64_t* col, int64_t n, hipStream_t s);
 
 // Fills L.blk0 from L.n and L.nchunks; returns the number of blocks of the
 // fused launch.

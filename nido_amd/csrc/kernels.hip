@@ -112,6 +112,14 @@ narrow_kernel(const int64_t* __restrict__ col, int* __restrict__ col32, int64_t 
         col32[i] = (int)col[i];
 }
 
+__global__ void __launch_bounds__(kBlock)
+widen_kernel(const int* __restrict__ col32, int64_t* __restrict__ col, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        col[i] = col32[i];
+}
+
 // src[e] = the row of arc e, from the per-phase row lists.
 __global__ void __launch_bounds__(kBlock)
 src_small_kernel(const int64_t* __restrict__ off, const int* __restrict__ small, int64_t n,
@@ -559,6 +567,12 @@ void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s)
 {
     if (n <= 0) return;
     narrow_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(col, col32, n);
+}
+
+void launch_widen_ids(const int* col32, int64_t* col, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    widen_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(col32, col, n);
 }
 
 void launch_modularity(const int* src, const int* col, const double* w, const int* C,
