@@ -51,8 +51,11 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
 hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s)
 {
     constexpr int64_t kChunk = int64_t(16) << 20;   // ids per pinned buffer (64 MiB)
-    if (n <= kChunk) {   // small: one pageable copy, no pinned allocation
+    if (n < (int64_t(1) << 28)) {   // below 1 GiB of int32: one pageable copy, no pinned allocation
         std::vector<int> t((size_t)n);
+        // threads only for graphs larger than a serving request, whose
+        // requests already run side by side
+#pragma omp parallel for schedule(static) if (n >= (int64_t(1) << 26))
         for (int64_t j = 0; j < n; ++j) t[(size_t)j] = (int)src[j];
         hipError_t e = hipMemcpyAsync(dst, t.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s);
         return e != hipSuccess ? e : hipStreamSynchronize(s);

@@ -13,7 +13,8 @@ namespace nido {
 hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // The same for n ids below 2^31 stored as int64 on the host, sent as int32
-// (narrowed while staging): half the bytes over the host link.
+// (narrowed while staging): half the bytes over the host link.  Pinned
+// staging from 2^28 ids up, as for staged_h2d.
 hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s);
 
 // out[i] = off[i] - off[0] for i <= n, split across threads.
