@@ -380,7 +380,11 @@ __device__ __forceinline__ void coarse_insert_one(int64_t i, int j, fx_t wf,
 // Every fine arc (u->x, w) goes into coarse row new(u) under key new(x);
 // internal arcs and old self-loops become the coarse self-loop
 // (graph_gpu.cpp:1233-1250).  Grid-stride over vertices, one wavefront per
-// vertex so that long rows stay coalesced.
+// vertex so that long rows stay coalesced.  The arcs of a row that land in
+// its own coarse vertex (most of them once communities have formed) are
+// summed across the wavefront first and inserted once: otherwise they all
+// take device atomics on the same self-loop slot.  The sums are exact
+// integers, so the grouping does not change them.
 __global__ void __launch_bounds__(kBlock)
 coarse_insert_kernel(const int64_t* __restrict__ off, const int64_t* __restrict__ col,
                      const double* __restrict__ w, const int64_t* __restrict__ C,
@@ -396,8 +400,17 @@ coarse_insert_kernel(const int64_t* __restrict__ off, const int64_t* __restrict_
         if (gid && !keep[gid[v]]) continue;   // uniform in the wavefront
         const int64_t i = newid[C[v]];
         const int64_t e1 = off[v + 1];
-        for (int64_t e = off[v] + lane; e < e1; e += 64)
-            coarse_insert_one(i, (int)newid[C[col[e]]], fx_from_double(w[e]), toff, tkey, tval);
+        for (int64_t e0 = off[v]; e0 < e1; e0 += 64) {   // uniform trip count in the wavefront
+            const int64_t e = e0 + lane;
+            const bool on = e < e1;
+            const int j = on ? (int)newid[C[col[e]]] : -1;
+            const fx_t x = on ? fx_from_double(w[e]) : fx_t(0);
+            const bool self = on && j == (int)i;
+            const fx_t sx = wave_sum_fx(self ? x : fx_t(0));   // every lane takes part
+            const bool any_self = __ballot(self) != 0;
+            if (on && !self) coarse_insert_one(i, j, x, toff, tkey, tval);
+            if (lane == 0 && any_self) coarse_insert_one(i, (int)i, sx, toff, tkey, tval);
+        }
     }
 }
 
