@@ -161,8 +161,8 @@ int nido_graph_batch_bounds(const nido_graph* g, int64_t* parts);
 int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t);
 int nido_graph_get_tuning(const nido_graph* g, nido_tuning* t);
 /* Autotuner (no counterpart in the reference): times `iterations` local-
- * moving iterations of the current phase from the singleton partition for
- * candidate knob values, one knob at a time, keeps the fastest, applies it
+ * moving iterations of the current phase, after its first two from the
+ * singleton partition, for candidate knob values, one knob at a time, keeps the fastest, applies it
  * and reports it in *chosen (may be NULL).  Results never depend on the
  * knobs.  Leaves the handle in the singleton partition of its phase. */
 int nido_autotune(nido_graph* g, int iterations, nido_tuning* chosen);

@@ -1201,8 +1201,9 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
 
 // ---- autotuner ----------------------------------------------------------
 // Picks the launch knobs (degree-bin bounds, long-row partition size and
-// table, stream overlap, graph replay) by timing the first iterations of the
-// current phase from the singleton partition: one knob at a time, keeping a
+// table, stream overlap, graph replay) by timing `iters` iterations of the
+// current phase after its first two from the singleton partition: one knob
+// at a time, keeping a
 // value only when it is at least 2% faster.  Results never depend on the
 // knobs, so any choice is safe; the handle is left in the singleton
 // partition of its current phase.
@@ -1215,9 +1216,12 @@ double tune_trial(nido_graph* g, const nido_tuning& t, int iters)
         for (int i = 0; i < g->G; ++i) act[(size_t)i] = g->twom_g[(size_t)i] != 0;
         if (upload_flags(g, g->d_act, act) != NIDO_OK) return -1.0;
     }
-    // one untimed pass (first launches, graph capture), then the timed ones
-    if (run_iteration_launches(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
-    if (do_singleton(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
+    // Two untimed passes (the first iterations from the singleton partition,
+    // which move most vertices and say little about the later ones; first
+    // launches; the capture of both graph forms when replaying), then the
+    // timed ones continue the same trajectory.
+    for (int i = 0; i < 2; ++i)
+        if (run_iteration_launches(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
     const double t0 = now_s();
     for (int i = 0; i < iters; ++i)
         if (run_iteration_launches(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
