@@ -37,7 +37,7 @@ extern "C" {
 #define NIDO_ERR_STATE    -5   /* call out of order */
 #define NIDO_ERR_IO       -6   /* file could not be read or written */
 
-#define NIDO_ABI_VERSION   1
+#define NIDO_ABI_VERSION   2   /* 2: nido_serve_config.group */
 
 #define NIDO_MAX_PHASES  200   /* MAX_PHASES, types.hpp:24-26 */
 #define NIDO_DEFAULT_ITERATIONS 500    /* types.hpp:28-30 */

@@ -77,6 +77,10 @@ def _sig(name, res, *args):
 
 _sig("nido_last_error", C.c_char_p)
 _sig("nido_abi_version", C.c_int)
+ABI_VERSION = 2   # the struct layouts below
+if lib.nido_abi_version() != ABI_VERSION:
+    raise ImportError(f"{LIB_PATH} has ABI {lib.nido_abi_version()}, this binding needs {ABI_VERSION}: "
+                      f"rebuild it with `make -C {_HERE}`")
 _sig("nido_device_count", C.c_int, C.POINTER(C.c_int))
 _sig("nido_hgraph_read", C.c_int, C.c_char_p, C.POINTER(C.c_void_p))
 _sig("nido_hgraph_write", C.c_int, C.c_void_p, C.c_char_p)
