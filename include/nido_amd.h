@@ -100,6 +100,13 @@ void nido_div_rcp(int64_t n, const double* a, const double* b, double* out);
 int nido_abi_version(void);
 /* Number of HIP devices visible (0 on a machine without a GPU). */
 int nido_device_count(int* count);
+/* Graph creation copies large inputs to the device through two pinned
+ * staging buffers (ids narrowed to int32 on the way): from min_bytes crossing
+ * the host link, chunk_bytes per buffer (a multiple of 4096); 0 restores the
+ * defaults, 1 GiB and 64 MiB.  Process-wide.  No counterpart in the reference
+ * (its uploads are plain pageable copies, graph_gpu.cpp:967-1027); lowering
+ * it lets small inputs take the pinned path. */
+int nido_set_upload_staging(int64_t min_bytes, int64_t chunk_bytes);
 
 /* ---- host graph: Graph (graph.hpp:14-102) ------------------------------- */
 
@@ -146,7 +153,11 @@ int nido_gen_chung_lu(int64_t nv, double alpha, double dmin, double dmax, double
 /* GraphGPU::GraphGPU(graph, nbatches, ...), graph_gpu.cpp:19-185: uploads the
  * CSR, computes k_v (sum_vertex_weights), 2m (compute_mass), the maximum
  * order and the -b batch boundaries (partition_graph_edge_batch).  Offsets
- * must be non-decreasing, cols in [0, nv), weights finite with |w| < 2^62.
+ * must be non-decreasing, cols in [0, nv), weights finite with |w| < 2^62,
+ * their |w| totalling less than 2^62 (each exact sum -- k_v, a_c, 2m, coarse
+ * weights -- is then below 2^62), and with any negative weight the total of
+ * |w| less than 2^31 |2m| (modularity square terms below 2^62); in a batched
+ * handle these hold per graph.
  * stream: a hipStream_t to run on, or NULL for a stream owned by the
  * handle.  nv must be below 2^31. */
 int nido_graph_create(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,

@@ -16,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 __all__ = [
-    "lib", "LIB_PATH", "NidoError", "Graph", "GraphGPU", "LouvainGPU", "PhaseStats", "RunStats",
+    "lib", "LIB_PATH", "NidoError", "set_upload_staging", "Graph", "GraphGPU", "LouvainGPU", "PhaseStats", "RunStats",
     "gen_sbm", "gen_rmat", "gen_chung_lu", "device_count", "batch_bounds_host",
     "DEFAULT_BATCHES", "DEFAULT_ITERATIONS", "DEFAULT_THRESHOLD", "MAX_PHASES",
 ]
@@ -82,6 +82,7 @@ if lib.nido_abi_version() != ABI_VERSION:
     raise ImportError(f"{LIB_PATH} has ABI {lib.nido_abi_version()}, this binding needs {ABI_VERSION}: "
                       f"rebuild it with `make -C {_HERE}`")
 _sig("nido_device_count", C.c_int, C.POINTER(C.c_int))
+_sig("nido_set_upload_staging", C.c_int, _i64, _i64)
 _sig("nido_hgraph_read", C.c_int, C.c_char_p, C.POINTER(C.c_void_p))
 _sig("nido_hgraph_write", C.c_int, C.c_void_p, C.c_char_p)
 _sig("nido_hgraph_from_csr", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.POINTER(C.c_void_p))
@@ -180,6 +181,12 @@ def device_count() -> int:
     n = C.c_int(0)
     _check(lib.nido_device_count(C.byref(n)), "nido_device_count")
     return n.value
+
+
+def set_upload_staging(min_bytes: int = 0, chunk_bytes: int = 0) -> None:
+    """Pinned staging of graph uploads from min_bytes in chunk_bytes pieces
+    (0: the defaults, 1 GiB and 64 MiB); process-wide."""
+    _check(lib.nido_set_upload_staging(int(min_bytes), int(chunk_bytes)), "nido_set_upload_staging")
 
 
 def batch_bounds_host(off: np.ndarray, nbatches: int) -> np.ndarray:

@@ -1474,7 +1474,7 @@ int nido_graph_batch_bounds(const nido_graph* g, int64_t* parts)
 
 int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
 {
-    if (!g || !t) return set_error(NIDO_ERR_INVALID, "null argument");
+    if (!t) return set_error(NIDO_ERR_INVALID, "null argument");
     if (t->thread_max_deg < 0 || t->thread_max_deg > kThreadMaxDeg)
         return set_error(NIDO_ERR_INVALID, "thread_max_deg must be in [0, %d]", kThreadMaxDeg);
     if (t->big_min_deg < 0 || t->big_min_deg > kBinMaxDeg[BIN_B256])
@@ -1486,6 +1486,10 @@ int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
     if (t->big_key_limit < 1 || t->big_key_limit > t->big_slots / 2)
         return set_error(NIDO_ERR_INVALID, "big_key_limit must be in [1, big_slots/2]");
     if (t->use_graph < 0 || t->use_graph > 2) return set_error(NIDO_ERR_INVALID, "use_graph must be 0, 1 or 2");
+    if ((t->split_small | 1) != 1) return set_error(NIDO_ERR_INVALID, "split_small must be 0 or 1");
+    if ((t->overlap_long | 1) != 1) return set_error(NIDO_ERR_INVALID, "overlap_long must be 0 or 1");
+    if ((t->full_modularity | 1) != 1) return set_error(NIDO_ERR_INVALID, "full_modularity must be 0 or 1");
+    if (!g) return set_error(NIDO_ERR_INVALID, "null handle");
     AllocScope scope(g->stream);
     API_GUARD(g->tuning = *t; return setup_phase(g);)
 }

@@ -4,15 +4,29 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <vector>
 
+#include "../../include/nido_amd.h"
+#include "errors.h"
+
 namespace nido {
+
+namespace {
+// Copies of at least kMinBytes go through two pinned buffers of kChunkBytes
+// each (below 1 GiB the runtime's own staging is as fast); both can be
+// lowered so that tests reach the pinned path with small inputs.
+constexpr int64_t kDefaultMinBytes = int64_t(1) << 30;
+constexpr int64_t kDefaultChunkBytes = int64_t(64) << 20;
+std::atomic<int64_t> g_min_bytes{kDefaultMinBytes};
+std::atomic<int64_t> g_chunk_bytes{kDefaultChunkBytes};
+}  // namespace
 
 hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
 {
-    constexpr size_t kChunk = size_t(64) << 20;
-    if (bytes < (size_t(1) << 30)) {   // below 1 GiB the runtime's own staging is as fast
+    const size_t kChunk = (size_t)g_chunk_bytes.load();
+    if (bytes < (size_t)g_min_bytes.load()) {
         hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
         return e != hipSuccess ? e : hipStreamSynchronize(s);
     }
@@ -50,8 +64,8 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
 
 hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s)
 {
-    constexpr int64_t kChunk = int64_t(16) << 20;   // ids per pinned buffer (64 MiB)
-    if (n < (int64_t(1) << 28)) {   // below 1 GiB of int32: one pageable copy, no pinned allocation
+    const int64_t kChunk = g_chunk_bytes.load() / (int64_t)sizeof(int);   // ids per pinned buffer
+    if (n * (int64_t)sizeof(int) < g_min_bytes.load()) {   // one pageable copy, no pinned allocation
         std::vector<int> t((size_t)n);
         // threads only for graphs larger than a serving request, whose
         // requests already run side by side
@@ -87,6 +101,15 @@ hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s
     return e;
 }
 
+int set_staging(int64_t min_bytes, int64_t chunk_bytes)
+{
+    if (min_bytes < 0 || chunk_bytes < 0 || (chunk_bytes > 0 && (chunk_bytes < 4096 || chunk_bytes % 4096)))
+        return set_error(NIDO_ERR_INVALID, "min_bytes >= 0; chunk_bytes 0 or a multiple of 4096");
+    g_min_bytes.store(min_bytes ? min_bytes : kDefaultMinBytes);
+    g_chunk_bytes.store(chunk_bytes ? chunk_bytes : kDefaultChunkBytes);
+    return NIDO_OK;
+}
+
 void rebase_offsets(const int64_t* off, int64_t n, int64_t* out)
 {
     const int64_t base = off[0];
@@ -95,3 +118,8 @@ void rebase_offsets(const int64_t* off, int64_t n, int64_t* out)
 }
 
 }  // namespace nido
+
+extern "C" int nido_set_upload_staging(int64_t min_bytes, int64_t chunk_bytes)
+{
+    return nido::set_staging(min_bytes, chunk_bytes);
+}

@@ -428,33 +428,61 @@ int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* 
     const int64_t ne = off[nv] - base;
     if (ne > 0 && (!col || !w)) return set_error(NIDO_ERR_INVALID, "null cols/weights");
     // first bad arc of graph g: column outside the graph, weight not finite
-    // or too large; one parallel loop over the graphs when there are many,
-    // over each graph's arcs otherwise
+    // or too large; and the graph's exact total of |fx(w)| (saturating at
+    // 2^126 units) and of fx(w).  One parallel loop over the graphs when
+    // there are many, over each graph's arcs otherwise.
     int neg = 0;
-    auto scan = [&](int g, int64_t e0, int64_t e1, int64_t& bad_col, int64_t& bad_w, int& ng, bool par) {
+    const ufx_t cap = (ufx_t)1 << 126;   // 2^62 in units of 2^-64
+    auto sat_add = [&](ufx_t x, ufx_t y) { return x >= cap || y >= cap || x + y >= cap ? cap : x + y; };
+    auto scan = [&](int g, int64_t e0, int64_t e1, int64_t& bad_col, int64_t& bad_w, int& ng, ufx_t& sabs,
+                    fx_t& ssum, bool par) {
         const int64_t lo = vptr[g], hi = vptr[g + 1];
         int64_t bc = INT64_MAX, bw = INT64_MAX;
         int nn = 0;
-#pragma omp parallel for reduction(min : bc, bw) reduction(| : nn) schedule(static) if (par)
-        for (int64_t e = e0; e < e1; ++e) {
-            const int64_t c = col[base + e];
-            if ((c < lo || c >= hi) && e < bc) bc = e;
-            const double x = w[base + e];
-            if (!(x == x) || x >= 0x1p62 || x <= -0x1p62) {
-                if (e < bw) bw = e;
+        ufx_t sa = 0, ss = 0;   // ss: fx(w) summed modulo 2^128
+#pragma omp parallel if (par)
+        {
+            int64_t tbc = INT64_MAX, tbw = INT64_MAX;
+            int tn = 0;
+            ufx_t ta = 0, ts = 0;
+#pragma omp for schedule(static) nowait
+            for (int64_t e = e0; e < e1; ++e) {
+                const int64_t c = col[base + e];
+                if ((c < lo || c >= hi) && e < tbc) tbc = e;
+                const double x = w[base + e];
+                if (!(x == x) || x >= 0x1p62 || x <= -0x1p62) {
+                    if (e < tbw) tbw = e;
+                    continue;
+                }
+                tn |= x < 0;
+                const fx_t f = fx_from_double(x);
+                ta = sat_add(ta, f < 0 ? (ufx_t)(-f) : (ufx_t)f);
+                ts += (ufx_t)f;   // exact while the total of |w| stays below 2^62
             }
-            nn |= x < 0;
+#pragma omp critical
+            {
+                bc = std::min(bc, tbc);
+                bw = std::min(bw, tbw);
+                nn |= tn;
+                sa = sat_add(sa, ta);
+                ss += ts;
+            }
         }
         bad_col = bc;
         bad_w = bw;
         ng = nn;
+        sabs = sa;
+        ssum = (fx_t)ss;
     };
     std::vector<int64_t> gbc((size_t)G), gbw((size_t)G);
     std::vector<int> gneg((size_t)G);
+    std::vector<ufx_t> gabs((size_t)G);
+    std::vector<fx_t> gsum((size_t)G);
     const bool many = G >= 64;
 #pragma omp parallel for schedule(dynamic, 16) if (many)
     for (int g = 0; g < G; ++g)
-        scan(g, off[vptr[g]] - base, off[vptr[g + 1]] - base, gbc[(size_t)g], gbw[(size_t)g], gneg[(size_t)g], !many);
+        scan(g, off[vptr[g]] - base, off[vptr[g + 1]] - base, gbc[(size_t)g], gbw[(size_t)g], gneg[(size_t)g],
+             gabs[(size_t)g], gsum[(size_t)g], !many);
     for (int g = 0; g < G; ++g) {
         const int64_t bad_col = gbc[(size_t)g], bad_w = gbw[(size_t)g];
         neg |= gneg[(size_t)g];
@@ -466,6 +494,17 @@ int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* 
                              (long long)col[base + bad_col]);
         if (bad_w != INT64_MAX)
             return set_error(NIDO_ERR_INVALID, "weight[%lld] is not finite or |w| >= 2^62", (long long)bad_w);
+        // every exact sum (k_v, e_vc, a_c, 2m, coarse weights, the internal
+        // weight) is bounded by the total of |w|: below 2^62 it fits Q64.64
+        if (gabs[(size_t)g] >= cap)
+            return set_error(NIDO_ERR_INVALID, "graph %d: the total of |w| over its arcs reaches 2^62", g);
+        // with negative weights |a_c / 2m| <= sum|w| / |2m|: below 2^31 the
+        // square terms of the modularity stay below 2^62
+        const fx_t t = gsum[(size_t)g];
+        const ufx_t tabs = t < 0 ? (ufx_t)(-t) : (ufx_t)t;
+        if (gneg[(size_t)g] && tabs != 0 && (gabs[(size_t)g] >> 31) >= tabs)
+            return set_error(NIDO_ERR_INVALID,
+                             "graph %d: the total of |w| is 2^31 or more times |2m| (negative weights)", g);
     }
     if (nonneg) *nonneg = !neg;
     return NIDO_OK;

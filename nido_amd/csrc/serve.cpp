@@ -29,6 +29,7 @@
 
 #include "../../include/nido_amd.h"
 #include "errors.h"
+#include "host_check.h"
 
 using nido::set_error;
 
@@ -36,6 +37,7 @@ namespace {
 
 constexpr int64_t kGroupArcs = int64_t(1) << 23;        // requests up to this size are grouped
 constexpr int64_t kGroupTotalArcs = int64_t(1) << 26;   // arcs of one group, at most
+constexpr int64_t kMaxVertices = (int64_t(1) << 31) - 1;  // vertices of one handle, below this
 
 double now_s()
 {
@@ -135,6 +137,16 @@ void nido_server::run_group(const std::vector<Request*>& grp, hipStream_t st)
     nido_graph* g = nullptr;
     nido_run_stats all{};
     int rc = nido_batch_create(G, vptr.data(), off.data(), col.data(), w.data(), cfg.nbatches, st, &g);
+    if (rc == NIDO_ERR_INVALID || rc == NIDO_ERR_CAPACITY || rc == NIDO_ERR_NOMEM) {
+        // every request was checked on its own at submission, so this is the
+        // group's doing (its size or memory): run the members one at a time,
+        // each with the result it gets alone
+        std::vector<int64_t>().swap(off);
+        std::vector<int64_t>().swap(col);
+        std::vector<double>().swap(w);
+        for (Request* r : grp) run_one(r, st);
+        return;
+    }
     if (rc == NIDO_OK) rc = nido_batch_run(g, cfg.max_loops, cfg.tol, cfg.tol_phase, &all);
     const std::string err = rc != NIDO_OK ? std::string(nido_last_error()) : std::string();
     for (int i = 0; i < G; ++i) {
@@ -168,16 +180,17 @@ void nido_server::worker(int slot)
             if (queue.empty()) return;
             // the head request, then (if it is small) the small ones right
             // behind it, in arrival order, while they fit
-            int64_t arcs = 0;
+            int64_t arcs = 0, verts = 0;
             do {
                 Request* r = queue.front();
                 if (!grp.empty() && (r->ne > kGroupArcs || fp + r->footprint > budget_free ||
-                                     arcs + r->ne > kGroupTotalArcs))
+                                     arcs + r->ne > kGroupTotalArcs || verts + r->nv >= kMaxVertices))
                     break;
                 queue.pop_front();
                 grp.push_back(r);
                 fp += r->footprint;
                 arcs += r->ne;
+                verts += r->nv;
                 r->state = 1;
             } while (grp.front()->ne <= kGroupArcs && (int)grp.size() < cfg.group && !queue.empty());
             budget_free -= fp;
@@ -186,8 +199,18 @@ void nido_server::worker(int slot)
         }
         const double t0 = now_s();
         for (Request* r : grp) r->t_start = t0;
-        if (grp.size() == 1) run_one(grp[0], streams[(size_t)slot]);
-        else run_group(grp, streams[(size_t)slot]);
+        try {
+            if (grp.size() == 1) run_one(grp[0], streams[(size_t)slot]);
+            else run_group(grp, streams[(size_t)slot]);
+        } catch (const std::bad_alloc&) {
+            // a host copy of the group or a clustering did not fit: the
+            // requests fail, and their slot and budget share are freed below
+            for (Request* r : grp) {
+                r->rc = NIDO_ERR_NOMEM;
+                r->err = "out of host memory";
+                std::vector<int64_t>().swap(r->clustering);
+            }
+        }
         for (Request* r : grp) {   // the input is no longer needed
             std::vector<int64_t>().swap(r->off);
             std::vector<int64_t>().swap(r->col);
@@ -252,8 +275,16 @@ int nido_serve_submit(nido_server* s, int64_t nv, const int64_t* off, const int6
                       int flags, int64_t* id)
 {
     if (!s || !off || !id || nv < 0) return set_error(NIDO_ERR_INVALID, "invalid argument");
+    if (nv >= kMaxVertices) return set_error(NIDO_ERR_CAPACITY, "nv must be below 2^31-1");
     const int64_t ne = off[nv] - off[0];
     if (ne < 0 || (ne > 0 && (!col || !w))) return set_error(NIDO_ERR_INVALID, "invalid CSR");
+    {
+        // the whole check of nido_graph_create, on this request alone: a
+        // malformed request is refused here, so it never fails a group
+        const int64_t whole[2] = {0, nv};
+        const int rc = nido::check_csr(nv, off, col, w, 1, whole, nullptr);
+        if (rc != NIDO_OK) return rc;
+    }
     Request* r = new (std::nothrow) Request;
     if (!r) return set_error(NIDO_ERR_NOMEM, "out of host memory");
     r->nv = nv;

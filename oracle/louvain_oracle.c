@@ -305,6 +305,26 @@ static void phase_setup(orc_ctx* c)
         err |= e1;
     }
     if (err && !c->err) c->err = 1;
+    /* Every exact sum (k_v, e_vc, a_c, 2m, coarse weights) is bounded by the
+     * total of |w|, which must stay below 2^62; with negative weights
+     * |a_c / 2m| <= sum|w| / |2m| must stay below 2^31 so that each square
+     * term of the modularity stays below 2^62.  Serial and saturating. */
+    if (!err && p->ne > 0) {
+        const ufx_t cap = (ufx_t)1 << 126;
+        ufx_t sabs = 0, ssum = 0;
+        int neg = 0;
+        for (int64_t e = 0; e < p->ne && sabs < cap; ++e) {
+            fx_t f = p->wf[e];
+            ufx_t af = f < 0 ? (ufx_t)(-f) : (ufx_t)f;
+            neg |= f < 0;
+            sabs = af >= cap - sabs ? cap : sabs + af;
+            ssum += (ufx_t)f;
+        }
+        fx_t t = (fx_t)ssum;
+        ufx_t tabs = t < 0 ? (ufx_t)(-t) : (ufx_t)t;
+        if (sabs >= cap && !c->err) c->err = 3;
+        else if (neg && tabs != 0 && (sabs >> 31) >= tabs && !c->err) c->err = 4;
+    }
     int64_t maxdeg = 0;
     #pragma omp parallel for schedule(dynamic, 1024) reduction(max:maxdeg)
     for (int64_t v = 0; v < nv; ++v) {

@@ -358,3 +358,23 @@ def test_rows_at_bin_and_partition_edges(knobs):
         got["phases"].append(p)
     g.close()
     assert_runs_equal(ref, got)
+
+
+def test_min_id_tie_break_closed_form_on_the_device():
+    """Vertex 0 of the oracle tests' tie graph has two candidates (5 then 3
+    in its row) with exactly equal gains, (1 - 2*2/10)/5; the smallest id,
+    3, must win on the device too, and again with 3 and 5 relabelled (the
+    tie in the other arc order).  A weight-2 edge 0-5 breaks the tie in 5's
+    favour."""
+    na = _na()
+    off = np.array([0, 2, 3, 4, 6, 7, 9, 10], np.int64)
+    col = np.array([5, 3, 3, 5, 0, 1, 6, 0, 2, 4], np.int64)
+    col_swapped = np.array([3, 5, 5, 3, 0, 2, 6, 0, 1, 4], np.int64)
+    w2 = np.ones(10)
+    w2[[0, 7]] = 2.0                     # arcs 0 -> 5 and 5 -> 0
+    for c, w, want in ((col, np.ones(10), 3), (col_swapped, np.ones(10), 3), (col, w2, 5)):
+        g = na.GraphGPU((off, c, w), 1)
+        g.singleton_partition()
+        g.louvain_update(0)
+        assert g.new_communities()[0] == want, (list(c), list(w))
+        g.close()

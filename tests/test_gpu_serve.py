@@ -67,3 +67,37 @@ def test_request_larger_than_the_budget_is_refused():
     with pytest.raises(na.NidoError):
         srv.submit((off, col, w))
     srv.close()
+
+
+def test_malformed_requests_are_refused_and_the_group_still_runs():
+    """Each request is checked on its own at submission: a column outside
+    the graph, a weight that is not finite, or weights whose total would
+    wrap the exact sums are refused there, and the well-formed requests queued
+    around them (grouped into one batched handle) get their own results."""
+    na = _na()
+    reqs = _requests(na)[:5]
+    srv = na.Server(slots=1, nbatches=8, group=64)
+    off, col, w = reqs[0]
+    bad_col = col.copy()
+    bad_col[3] = len(off) + 5
+    bad_w = w.copy()
+    bad_w[1] = np.nan
+    path = (np.array([0, 1, 3, 4]), np.array([1, 0, 2, 1]), np.full(4, 3e18))
+    ids = []
+    for i, g in enumerate(reqs):
+        ids.append(srv.submit(g))
+        if i == 1:
+            for bad in ((off, bad_col, w), (off, col, bad_w), path):
+                with pytest.raises(na.NidoError) as e:
+                    srv.submit(bad)
+                assert e.value.code == -1
+    got = []
+    for rid in ids:
+        st, _ = srv.wait(rid)
+        got.append((st.q, srv.clustering(rid)))
+        srv.release(rid)
+    srv.close()
+    for i, (o, c, x) in enumerate(reqs):
+        alone = run_gpu(o, c, x, 8)
+        assert abs(got[i][0] - alone["q"]) <= Q_TOL, f"request {i}"
+        assert np.array_equal(got[i][1], alone["clustering"]), f"request {i}"

@@ -160,6 +160,65 @@ def test_device_entry_points_validate_before_touching_the_gpu():
         na.GraphGPU((off, np.array([1, 0]), np.array([1.0, 1.0])), 0)
 
 
+def test_weight_totals_that_would_wrap_the_exact_sums_are_refused():
+    """Each |w| below 2^62 is not enough: the Q64.64 sums (2m, k_v, a_c) need
+    the total of |w| below 2^62 too.  The path 0-1-2 with two edges of
+    weight 3e18 (2m = 1.2e19) used to wrap to a negative mass; both the
+    library and the oracle now refuse it, before any device work.  With
+    negative weights, sum|w| / |2m| must also stay below 2^31 (square terms
+    of the modularity)."""
+    import oracle
+    off = np.array([0, 1, 3, 4])
+    col = np.array([1, 0, 2, 1])
+    big = np.full(4, 3e18)
+    with pytest.raises(na.NidoError) as e:
+        na.GraphGPU((off, col, big), 2)
+    assert e.value.code == -1 and "2^62" in str(e.value)
+    with pytest.raises(ValueError):
+        oracle.Oracle(off, col, big, 2)
+    # just below the bound on both sides: accepted by the oracle, and the
+    # library gets past its checks (it then needs a device)
+    ok = np.full(4, 2.0 ** 60 - 2.0 ** 8)
+    o = oracle.Oracle(off, col, ok, 2)
+    assert o.mass == 2.0 * ok[0] * 2 / 2
+    o.close()
+    # negative weights that nearly cancel: |2m| tiny against sum|w|
+    neg = np.array([1.0, 1.0, -1.0 + 2.0 ** -40, -1.0 + 2.0 ** -40])
+    with pytest.raises(na.NidoError) as e:
+        na.GraphGPU((off, col, neg), 2)
+    assert "negative" in str(e.value)
+    with pytest.raises(ValueError):
+        oracle.Oracle(off, col, neg, 2)
+    # the same shape with a moderate ratio is accepted by the oracle
+    mild = np.array([1.0, 1.0, -0.5, -0.5])
+    oracle.Oracle(off, col, mild, 2).close()
+    # per graph in a batched handle: graph 1 alone exceeds the bound
+    vptr = np.array([0, 3, 6])
+    off2 = np.array([0, 1, 3, 4, 5, 7, 8])
+    col2 = np.array([1, 0, 2, 1, 4, 3, 5, 4])
+    w2 = np.concatenate([np.ones(4), big])
+    with pytest.raises(na.NidoError) as e:
+        na.BatchGPU(csr=(vptr, off2, col2, w2), nbatches=2)
+    assert "graph 1" in str(e.value)
+
+
+def test_tuning_flags_accept_only_zero_or_one():
+    """split_small, overlap_long and full_modularity are 0/1 switches; the
+    knob values are checked before the handle, so a null handle shows which
+    check refused the struct without a device."""
+    import ctypes as C
+    base = dict(thread_max_deg=2, big_min_deg=512, big_part_arcs=1024, big_key_limit=1024, use_graph=1,
+                big_slots=2048, split_small=0, overlap_long=1, full_modularity=0)
+    t = na.Tuning(**base)
+    assert na.lib.nido_graph_set_tuning(None, C.byref(t)) == -1
+    assert b"null handle" in na.lib.nido_last_error()
+    for knob in ("split_small", "overlap_long", "full_modularity"):
+        for bad in (2, -1, 7):
+            t = na.Tuning(**dict(base, **{knob: bad}))
+            assert na.lib.nido_graph_set_tuning(None, C.byref(t)) == -1
+            assert knob.encode() in na.lib.nido_last_error(), (knob, bad)
+
+
 def test_batch_bounds_host_matches_oracle():
     import oracle
     off, _, _ = na.gen_rmat(10, 8, 0.57, 0.19, 0.19, False, 1).arrays()

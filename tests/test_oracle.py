@@ -147,6 +147,67 @@ def test_two_cliques_with_a_bridge():
     assert abs(r["q"] - float(q_exact)) < 1e-15
 
 
+def _tie_graph():
+    """Vertex 0 joined to vertices 5 and 3 (in that arc order) by unit
+    edges; 3 and 5 are also joined to 1 and 2, one each, and 4 to 6 (so no
+    vertex is isolated).  In the singleton partition (-b 1, every vertex
+    decides on the same snapshot) k_0 = k_3 = k_5 = 2, 2m = 10, m = 5, and
+    vertex 0 sees two candidates with the same e = 1 and a = 2:
+        g_3 = g_5 = (1 - 2 * 2 / 10) / 5 = 3/25,   s' = 0,
+    an exact tie (both gains are computed from identical operands, so they
+    round identically).  The smallest id wins: C'[0] = 3, not 5 (its first
+    arc) and not the larger id."""
+    off = np.array([0, 2, 3, 4, 6, 7, 9, 10], np.int64)
+    col = np.array([5, 3, 3, 5, 0, 1, 6, 0, 2, 4], np.int64)
+    w = np.ones(len(col))
+    return off, col, w
+
+
+def test_min_id_tie_break_closed_form():
+    off, col, w = _tie_graph()
+    # the CSR is symmetric and k, m are as stated
+    pairs = {(v, int(u)) for v in range(7) for u in col[off[v]:off[v + 1]]}
+    assert all((u, v) in pairs for v, u in pairs)
+    o = oracle.Oracle(off, col, w, 1)
+    o.singleton_partition()
+    assert list(o.vertex_weights()) == [2, 1, 1, 2, 1, 2, 1] and o.mass == 5.0
+    o.louvain_update(0)
+    cn = o.new_communities()
+    assert cn[0] == 3
+    # vertex 4's only candidate is 6 (e = 1, a = 1): g = (1 - 1/10) / 5 > 0;
+    # 1 and 2 have one candidate each
+    assert cn[4] == 6 and cn[1] == 3 and cn[2] == 5
+    # relabelling 3 <-> 5 puts the same tie in the other arc order ([3, 5]):
+    # 3 again, so neither the first nor the last arc decides
+    perm = np.array([0, 1, 2, 5, 4, 3, 6])
+    inv = np.argsort(perm)
+    off2 = np.zeros(8, np.int64)
+    rows = [col[off[inv[v]]:off[inv[v] + 1]] for v in range(7)]
+    off2[1:] = np.cumsum([len(r) for r in rows])
+    col2 = np.concatenate([perm[r] for r in rows])
+    o2 = oracle.Oracle(off2, col2, np.ones(len(col2)), 1)
+    o2.singleton_partition()
+    o2.louvain_update(0)
+    assert o2.new_communities()[0] == 3
+    o.close()
+    o2.close()
+
+
+def test_higher_gain_beats_a_smaller_id():
+    # vertex 0 with a weight-2 edge to 5 and a weight-1 edge to 3: k_0 =
+    # k_5 = 3, 2m = 12, g_5 = (2 - 9/12) / 6 > g_3 = (1 - 6/12) / 6, so 5
+    # wins although 3 is smaller
+    off, col, w = _tie_graph()
+    w = w.copy()
+    w[0] = 2.0          # arc 0 -> 5
+    w[7] = 2.0          # arc 5 -> 0
+    o = oracle.Oracle(off, col, w, 1)
+    o.singleton_partition()
+    o.louvain_update(0)
+    assert o.new_communities()[0] == 5
+    o.close()
+
+
 def test_empty_and_isolated_graphs():
     for nv in (0, 5):
         off = np.zeros(nv + 1, np.int64)
