@@ -184,6 +184,21 @@ int nido_graph_sync(nido_graph* g);
 int nido_singleton_partition(nido_graph* g);
 /* GraphGPU::compute_modularity, graph_gpu.cpp:862-905 (blocks for the value). */
 int nido_compute_modularity(nido_graph* g, double* q);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the modularity nido_compute_modularity gives, for the
+ * partition comm of a CSR graph, all arrays device pointers (offsets nv+1,
+ * cols/weights ne, comm nv with ids in [0, nv), *q one device double).  It
+ * allocates nothing: its working memory is a fixed device array of the
+ * library (16 B per vertex for NIDO_NOALLOC_MAX_VERTICES vertices, held on
+ * every GPU the library runs on until the process ends); more vertices give
+ * NIDO_ERR_CAPACITY.  Asynchronous on `stream` (a hipStream_t; NULL = the
+ * null stream): *q is written when the stream reaches it, NaN if an id or
+ * a row lies outside the graph.  Calls on one GPU share the working memory:
+ * calls whose launches overlap (unordered streams) race. */
+#define NIDO_NOALLOC_MAX_VERTICES (INT64_C(1) << 22)
+int nido_modularity_noalloc(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,
+                            const double* weights, const int64_t* comm, double* q, void* stream);
 /* GraphGPU::louvain_update(batch, g), graph_gpu.cpp:602-624: decides the
  * moves of one batch (writes newCommIds_). */
 int nido_louvain_update(nido_graph* g, int batch);

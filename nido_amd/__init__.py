@@ -17,7 +17,7 @@ import numpy as np
 
 __all__ = [
     "lib", "LIB_PATH", "NidoError", "set_upload_staging", "Graph", "GraphGPU", "LouvainGPU", "PhaseStats", "RunStats",
-    "gen_sbm", "gen_rmat", "gen_chung_lu", "device_count", "batch_bounds_host",
+    "gen_sbm", "gen_rmat", "gen_chung_lu", "device_count", "batch_bounds_host", "modularity_noalloc",
     "DEFAULT_BATCHES", "DEFAULT_ITERATIONS", "DEFAULT_THRESHOLD", "MAX_PHASES",
 ]
 
@@ -118,6 +118,9 @@ _sig("nido_get_graph", C.c_int, C.c_void_p, _pi64, _pi64, _pdbl)
 _sig("nido_get_clustering", C.c_int, C.c_void_p, _pi64)
 _sig("nido_dump_partition", C.c_int, C.c_void_p, C.c_char_p)
 _sig("nido_louvain_iteration", C.c_int, C.c_void_p, _pdbl, _pdbl, _pdbl)
+_sig("nido_modularity_noalloc", C.c_int, _i64, _i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+     C.c_void_p, C.c_void_p)
+NOALLOC_MAX_VERTICES = 1 << 22   # NIDO_NOALLOC_MAX_VERTICES
 _sig("nido_graph_set_profiling", C.c_int, C.c_void_p, C.c_int)
 _sig("nido_graph_set_profiling_classes", C.c_int, C.c_void_p, C.c_uint)
 _sig("nido_graph_set_profiling_stride", C.c_int, C.c_void_p, C.c_int)
@@ -187,6 +190,15 @@ def set_upload_staging(min_bytes: int = 0, chunk_bytes: int = 0) -> None:
     """Pinned staging of graph uploads from min_bytes in chunk_bytes pieces
     (0: the defaults, 1 GiB and 64 MiB); process-wide."""
     _check(lib.nido_set_upload_staging(int(min_bytes), int(chunk_bytes)), "nido_set_upload_staging")
+
+
+def modularity_noalloc(nv: int, ne: int, offsets: int, cols: int, weights: int, comm: int, q: int,
+                       stream: int = 0) -> None:
+    """nido_modularity_noalloc: every array argument is a device address
+    (int); *q (a device double) receives the modularity of partition comm
+    when `stream` reaches it.  Allocates no device memory."""
+    _check(lib.nido_modularity_noalloc(nv, ne, offsets, cols, weights, comm, q, stream or None),
+           "nido_modularity_noalloc")
 
 
 def batch_bounds_host(off: np.ndarray, nbatches: int) -> np.ndarray:
