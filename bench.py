@@ -36,7 +36,8 @@ equal the batched ones.
 --rate graphs/s for --duration s, each an R-MAT (0.57, 0.19, 0.19, 0.05) with
 scale uniform in [10, 20] and edge factor in {8, 16, 32}, unit weights, from a
 seeded stream; admission by HBM footprint into --slots concurrent workers;
-arrivals that find --max-queue requests waiting are turned away.  value =
+arrivals that find --max-queue requests waiting are turned away (reported
+beside the latencies, which cover the admitted requests only).  value =
 completed graphs/s over the whole run; p50/p99 latency (submission to result)
 of the completed requests.  One graph per (scale, edge factor) is generated
 up front (seed derived from the pair), since generating a scale-20 graph
@@ -294,6 +295,7 @@ def bench_serve(args):
                    "slots": args.slots, "group": args.group, "hbm_budget_gib": args.hbm_budget,
                    "max_queue": args.max_queue,
                    "arrivals": int(len(t_arr)), "admitted": len(ids), "turned_away": turned_away,
+                   "turned_away_frac": round(turned_away / max(len(t_arr), 1), 4),
                    "arcs_completed": int(sum(arcs)), "peak_concurrent": peak},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "mean": round(float(lat.mean()), 4), "max": round(float(lat.max()), 4)},

@@ -37,7 +37,7 @@ extern "C" {
 #define NIDO_ERR_STATE    -5   /* call out of order */
 #define NIDO_ERR_IO       -6   /* file could not be read or written */
 
-#define NIDO_ABI_VERSION   2   /* 2: nido_serve_config.group */
+#define NIDO_ABI_VERSION   3   /* 2: nido_serve_config.group; 3: nido_tuning.fixed_short */
 
 #define NIDO_MAX_PHASES  200   /* MAX_PHASES, types.hpp:24-26 */
 #define NIDO_DEFAULT_ITERATIONS 500    /* types.hpp:28-30 */
@@ -83,6 +83,9 @@ typedef struct {
     int32_t overlap_long;     /* run the long rows of a batch on a second stream, beside the short rows
                                  (0/1, default 1; only while iterations are replayed from a hipGraph) */
     int32_t full_modularity;  /* 1: every iteration sweeps all arcs for Q; 0: exact incremental update (0/1) */
+    int32_t fixed_short;      /* 1 (default): rows up to big_min_deg arcs sum in 64-bit fixed point when every
+                                 weight is a multiple of one power of two 2^-F and those rows' sums stay below
+                                 2^63 in units of 2^-F (the same exact sums); 0: always Q64.64 (0/1) */
 } nido_tuning;
 
 const char* nido_last_error(void);
@@ -128,11 +131,11 @@ int nido_hgraph_arrays(const nido_hgraph* g, const int64_t** offsets, const int6
                        const double** weights);
 void nido_hgraph_destroy(nido_hgraph* g);
 
-/* Seeded synthetic inputs.  All graphs are symmetric, without self-loops or
- * duplicate arcs, rows sorted by neighbour id; one weight per undirected edge
- * copied to both arcs; weighted=0 gives unit weights, weighted=1 weights in
- * (0,1] on a 2^-53 grid.  (The reference's own generator is the -r BA graph,
- * graph.cpp:76-160.) */
+/* Seeded synthetic inputs (the workloads' graphs).  All are symmetric,
+ * without self-loops or duplicate arcs, rows sorted by neighbour id; one
+ * weight per undirected edge copied to both arcs; weighted=0 gives unit
+ * weights, weighted=1 weights in (0,1] on a 2^-53 grid.  The reference's own
+ * generator, the -r BA graph, is nido_gen_ba below. */
 /* Planted partition: vertex v in block v/block_size, pairs u<v taken in
  * lexicographic order, Bernoulli(p_in) inside a block, Bernoulli(p_out)
  * across blocks. */
@@ -147,6 +150,28 @@ int nido_gen_rmat(int scale, int64_t edge_factor, double a, double b, double c, 
  * proportion to the expected degrees; self-loops dropped, deduplicated. */
 int nido_gen_chung_lu(int64_t nv, double alpha, double dmin, double dmax, double mean,
                       int weighted, uint64_t seed, nido_hgraph** out);
+/* Graph::Graph(nv, m0) -> create_random_network_ba, graph.cpp:76-110 and
+ * 127-148 (the reference's -r "<nv> <m0>"): a Barabasi-Albert graph drawn
+ * with the reference's random-number objects in the reference's order (two
+ * default std::default_random_engine, weights |N(1, 1)|, attachment to the
+ * first endpoint of a uniformly drawn earlier arc), duplicates kept, arcs
+ * ordered by (tail, head); 2*m0*(nv-m0) arcs.  Needs 1 <= m0 < nv < 2^31. */
+int nido_gen_ba(int64_t nv, int64_t m0, nido_hgraph** out);
+/* Graph::coloring, graph.cpp:547-694 (the reference's -c), in place: random
+ * weights r (engine t = std::mt19937_64(1 << t), t < engines, each with a
+ * std::uniform_int_distribution<int64_t>(0, 4 nv); vertex i draws from
+ * engine i % engines, each engine's draws taken in vertex order), the
+ * colouring of the reference's loop as one thread runs it -- colour(i) is the
+ * smallest colour held by no neighbour j < i with r_j >= r_i -- computed on
+ * the device, then the vertices renumbered in (colour, id) order and the CSR
+ * permuted to match (rows keep their arc order).  new_orders[nv] receives the
+ * old id of every new vertex, colors[nv] (if not NULL) the colour of every
+ * old vertex, *num_colors (if not NULL) the colour count.  With engines > 1
+ * the reference's own result depends on thread timing (its threads share
+ * the engines and the colour array without synchronisation); engines = 1
+ * is its reproducible case.  1 <= engines <= 31. */
+int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* new_orders,
+                         int64_t* num_colors);
 
 /* ---- device state: GraphGPU (graph_gpu.hpp:168-247) -------------------- */
 
@@ -230,6 +255,10 @@ int nido_get_clustering(nido_graph* g, int64_t* comm);
 /* GraphGPU::dump_partition / Clustering::dump_partition, clustering.cpp:23-46:
  * one "%lld\n" line per original vertex. */
 int nido_dump_partition(nido_graph* g, const char* path);
+/* The same with the order from nido_hgraph_coloring (main.cpp:162-163,
+ * clustering.cpp:25-37): the line of original vertex new_orders[i] holds
+ * the community of renumbered vertex i; new_orders NULL = no renumbering. */
+int nido_dump_partition_orders(nido_graph* g, const char* path, const int64_t* new_orders);
 
 /* One pass of the LouvainGPU::run loop body (louvain_gpu.cpp:55-90): every
  * batch's louvain_update / update_community_weights / update_community_ids,

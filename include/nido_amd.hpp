@@ -43,6 +43,8 @@ class Graph {
     {
         check(nido_hgraph_from_csr(nv, offsets[nv] - offsets[0], offsets, cols, weights, &h_), "Graph(csr)");
     }
+    // Graph(nv, m0): the -r Barabasi-Albert generator (graph.cpp:76-110, 127-148)
+    Graph(GraphElem totalVertices, GraphElem m0) { check(nido_gen_ba(totalVertices, m0, &h_), "Graph(nv, m0)"); }
     explicit Graph(nido_hgraph* h) : h_(h) {}
     Graph(const Graph&) = delete;
     Graph& operator=(const Graph&) = delete;
@@ -54,10 +56,20 @@ class Graph {
     const GraphElem* get_edges() const { const int64_t* p; check(nido_hgraph_arrays(h_, nullptr, &p, nullptr), "get_edges"); return p; }
     const GraphWeight* get_edge_weights() const { const double* p; check(nido_hgraph_arrays(h_, nullptr, nullptr, &p), "get_edge_weights"); return p; }
     void write(const std::string& binfile) const { check(nido_hgraph_write(h_, binfile.c_str()), "Graph::write"); }
+    // Graph::coloring (graph.cpp:547-694): renumbers the graph in place and
+    // returns new_orders (old id of every new vertex).
+    std::vector<GraphElem> coloring(int engines = 1)
+    {
+        std::vector<GraphElem> no((size_t)get_num_vertices());
+        check(nido_hgraph_coloring(h_, engines, nullptr, no.data(), &num_colors_), "coloring");
+        return no;
+    }
+    GraphElem get_num_colors() const { return num_colors_; }
     nido_hgraph* handle() const { return h_; }
 
   private:
     nido_hgraph* h_ = nullptr;
+    int64_t num_colors_ = 0;
 };
 
 // Device-resident state (graph_gpu.hpp:168-247).
@@ -80,7 +92,10 @@ class GraphGPU {
     void update_community_ids(const int& batch) { check(nido_update_community_ids(g_, batch), "update_community_ids"); }
     void restore_community() { check(nido_restore_community(g_), "restore_community"); }
     bool aggregation() { int done = 0; check(nido_aggregation(g_, &done), "aggregation"); return done != 0; }
-    void dump_partition(const std::string& filename) { check(nido_dump_partition(g_, filename.c_str()), "dump_partition"); }
+    void dump_partition(const std::string& filename, const GraphElem* new_orders = nullptr)
+    {
+        check(nido_dump_partition_orders(g_, filename.c_str(), new_orders), "dump_partition");
+    }
 
     GraphElem get_num_vertices() const { int64_t n = 0; check(nido_graph_info(g_, &n, nullptr, nullptr, nullptr), "info"); return n; }
     GraphElem get_num_edges() const { int64_t e = 0; check(nido_graph_info(g_, nullptr, &e, nullptr, nullptr), "info"); return e; }

@@ -17,7 +17,7 @@ import numpy as np
 
 __all__ = [
     "lib", "LIB_PATH", "NidoError", "set_upload_staging", "Graph", "GraphGPU", "LouvainGPU", "PhaseStats", "RunStats",
-    "gen_sbm", "gen_rmat", "gen_chung_lu", "device_count", "batch_bounds_host", "modularity_noalloc",
+    "gen_sbm", "gen_rmat", "gen_chung_lu", "gen_ba", "device_count", "batch_bounds_host", "modularity_noalloc",
     "DEFAULT_BATCHES", "DEFAULT_ITERATIONS", "DEFAULT_THRESHOLD", "MAX_PHASES",
 ]
 
@@ -62,7 +62,7 @@ class Tuning(C.Structure):
     _fields_ = [("thread_max_deg", C.c_int32), ("big_min_deg", C.c_int32),
                 ("big_part_arcs", C.c_int32), ("big_key_limit", C.c_int32),
                 ("use_graph", C.c_int32), ("big_slots", C.c_int32), ("split_small", C.c_int32),
-                ("overlap_long", C.c_int32), ("full_modularity", C.c_int32)]
+                ("overlap_long", C.c_int32), ("full_modularity", C.c_int32), ("fixed_short", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -77,7 +77,7 @@ def _sig(name, res, *args):
 
 _sig("nido_last_error", C.c_char_p)
 _sig("nido_abi_version", C.c_int)
-ABI_VERSION = 2   # the struct layouts below
+ABI_VERSION = 3   # the struct layouts below
 if lib.nido_abi_version() != ABI_VERSION:
     raise ImportError(f"{LIB_PATH} has ABI {lib.nido_abi_version()}, this binding needs {ABI_VERSION}: "
                       f"rebuild it with `make -C {_HERE}`")
@@ -95,6 +95,9 @@ _sig("nido_gen_rmat", C.c_int, C.c_int, _i64, C.c_double, C.c_double, C.c_double
      C.c_uint64, C.POINTER(C.c_void_p))
 _sig("nido_gen_chung_lu", C.c_int, _i64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
      C.c_uint64, C.POINTER(C.c_void_p))
+_sig("nido_gen_ba", C.c_int, _i64, _i64, C.POINTER(C.c_void_p))
+_sig("nido_hgraph_coloring", C.c_int, C.c_void_p, C.c_int, _pi64, _pi64, _pi64)
+_sig("nido_dump_partition_orders", C.c_int, C.c_void_p, C.c_char_p, _pi64)
 _sig("nido_graph_create", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.c_int, C.c_void_p,
      C.POINTER(C.c_void_p))
 _sig("nido_graph_destroy", C.c_int, C.c_void_p)
@@ -274,6 +277,18 @@ class Graph:
             return off.copy(), col.copy(), w.copy()
         return off, col, w
 
+    def coloring(self, engines: int = 1):
+        """Graph::coloring (graph.cpp:547-694), in place: returns
+        (new_orders, colors, num_colors); new_orders[i] is the old id of new
+        vertex i, colors[v] the colour of old vertex v."""
+        nv = self.num_vertices
+        no = np.empty(max(nv, 1), np.int64)
+        colors = np.empty(max(nv, 1), np.int64)
+        k = C.c_int64()
+        _check(lib.nido_hgraph_coloring(self._h, engines, _p64(colors), _p64(no), C.byref(k)),
+               "nido_hgraph_coloring")
+        return no[:nv], colors[:nv], k.value
+
     def close(self) -> None:
         if self._h:
             lib.nido_hgraph_destroy(self._h)
@@ -297,6 +312,14 @@ def gen_rmat(scale: int, edge_factor: int, a: float, b: float, c: float, weighte
     h = C.c_void_p()
     _check(lib.nido_gen_rmat(scale, edge_factor, a, b, c, int(weighted), seed, C.byref(h)),
            "nido_gen_rmat")
+    return Graph(h)
+
+
+def gen_ba(nv: int, m0: int) -> Graph:
+    """The reference's -r generator (graph.cpp:76-110): Barabasi-Albert,
+    nv vertices, m0 attachments per new vertex, weights |N(1, 1)|."""
+    h = C.c_void_p()
+    _check(lib.nido_gen_ba(nv, m0, C.byref(h)), "nido_gen_ba")
     return Graph(h)
 
 
@@ -461,8 +484,13 @@ class GraphGPU:
         _check(lib.nido_get_clustering(self._h, _p64(out)), "nido_get_clustering")
         return out
 
-    def dump_partition(self, path: str) -> None:
-        _check(lib.nido_dump_partition(self._h, os.fsencode(path)), "nido_dump_partition")
+    def dump_partition(self, path: str, new_orders=None) -> None:
+        if new_orders is None:
+            _check(lib.nido_dump_partition(self._h, os.fsencode(path)), "nido_dump_partition")
+        else:
+            no = np.ascontiguousarray(new_orders, np.int64)
+            _check(lib.nido_dump_partition_orders(self._h, os.fsencode(path), _p64(no)),
+                   "nido_dump_partition_orders")
 
     # ---- run history ----
     def phase(self, i: int) -> PhaseStats:

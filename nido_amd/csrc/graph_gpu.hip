@@ -6,6 +6,7 @@
 // and the host only reads back one scalar per modularity evaluation.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <mutex>
 #include <cstdio>
 #include <cstring>
@@ -257,6 +258,10 @@ struct nido_graph {
     int64_t* d_work = nullptr;     // n0+1 int64 scratch (renumbering)
     int* d_src = nullptr;          // row of every arc, capacity = phase-0 arcs
     int* d_col32 = nullptr;        // (int)d_col, capacity = phase-0 arcs
+    int64_t* d_wq = nullptr;       // d_w in units of 2^-F when the short rows sum in fixed point
+    int64_t wq_cap = 0;
+    bool fixed = false;            // this phase's short rows sum in 64-bit fixed point (d_wq)
+    double wscale = 1.0;           // 2^-F
     fx_t* d_part = nullptr;        // 2 * kPartBlocks reduction partials
     void* d_scan_tmp = nullptr;
     // scalars
@@ -354,6 +359,7 @@ void default_tuning(nido_tuning& t)
     t.split_small = 0;
     t.overlap_long = 1;
     t.full_modularity = 0;
+    t.fixed_short = 1;
 }
 
 void batch_bounds(const std::vector<int64_t>& off, int64_t nv, int nb, std::vector<int64_t>& parts)
@@ -541,6 +547,38 @@ int setup_phase(nido_graph* g)
     const int narrow = g->nonneg && maxk < 2147483648.0 - 256.0;
     for (auto& B : g->big_args) B.narrow = narrow;
     for (auto& L : g->small_bins) L.twom_d = g->twom_d;
+    // The short rows sum in 64-bit fixed point when every weight is a
+    // multiple of 2^-F (F <= 62) and no short row's sum can reach 2^63 in
+    // units of 2^-F: the sums are then the same exact values as in Q64.64
+    // (scaled by 2^(64 - F)), one word and no carry per LDS update.
+    g->fixed = false;
+    if (g->tuning.fixed_short && g->ne > 0 && g->twom != 0) {
+        int* d_fneed = reinterpret_cast<int*>(g->d_tot);
+        unsigned long long* d_wmax = reinterpret_cast<unsigned long long*>(g->d_tot + 1);
+        launch_weight_grid(g->d_w, g->ne, d_fneed, d_wmax, g->stream);
+        HIP_TRY(hipGetLastError());
+        int fneed = 0;
+        unsigned long long wbits = 0;
+        HIP_TRY(hipMemcpyAsync(&fneed, d_fneed, sizeof(fneed), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(&wbits, d_wmax, sizeof(wbits), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        const int F = std::max(fneed, 0);
+        const double wq_max = std::ldexp(nido_bitsd(wbits), F);   // exact: a power-of-two scaling
+        const uint64_t dmax = (uint64_t)std::max<int64_t>(g->tuning.big_min_deg, kThreadMaxDeg);
+        if (fneed <= 62 && wq_max < 0x1p62 &&
+            (unsigned __int128)(uint64_t)wq_max * dmax < ((unsigned __int128)1 << 63)) {
+            if (g->ne > g->wq_cap) {
+                dev_free(g->d_wq);
+                g->wq_cap = 0;
+                NIDO_TRY(dev_alloc(&g->d_wq, g->ne));
+                g->wq_cap = g->ne;
+            }
+            launch_to_fixed(g->d_w, g->d_wq, g->ne, std::ldexp(1.0, F), g->stream);
+            HIP_TRY(hipGetLastError());
+            g->fixed = true;
+            g->wscale = std::ldexp(1.0, -F);
+        }
+    }
     if (G > 1) {
         // per-graph modularity items: arc and vertex chunks of each graph
         std::vector<SegItem> seg;
@@ -621,6 +659,8 @@ int do_lm(nido_graph* g, int b)
 {
     g->committed[(size_t)b] = 0;
     LmArgs A{g->d_off, g->d_col32, g->d_w, g->d_C32, g->d_Cn, g->d_kd, g->d_k, g->d_a, g->m, 1.0 / g->m};
+    A.wq = g->fixed ? g->d_wq : nullptr;
+    A.wscale = g->wscale;
     if (g->G == 1) {
         const int64_t v0 = g->parts[(size_t)b], v1 = g->parts[(size_t)b + 1];
         // an empty batch still carries the previous batch's modularity delta
@@ -729,9 +769,9 @@ int iteration_launches(nido_graph* g, bool inc)
     if (inc && g->nb > 0)   // the earlier batches' deltas rode in the next batch's short-row launch
         launch_mod_delta(g->nb - 1, g->parts[(size_t)g->nb - 1], g->parts[(size_t)g->nb], g->d_Cn, g->d_C32,
                          g->d_col32, g->d_w, g->d_a, g->twom_d, D, g->stream);
-    if (g->G > 1)
+    if (g->G > 1)   // the graphs that sat out this iteration keep their modularity
         launch_modularity_seg(g->d_seg, g->nseg, g->d_src, g->d_col32, g->d_w, g->d_C32, g->d_a, g->d_twomg,
-                              g->G, g->d_gacc, g->d_gq, g->stream);
+                              g->G, g->d_gacc, g->d_gq, g->stream, g->d_act);
     else if (g->twom != 0 && inc)
         launch_mod_finish(D, g->nb, g->twom_d, g->d_q, g->stream);
     else if (g->twom != 0)
@@ -1434,7 +1474,7 @@ int nido_graph_destroy(nido_graph* g)
     free_graph_arrays(g);
     dev_free(g->d_C); dev_free(g->d_Cn); dev_free(g->d_k); dev_free(g->d_a); dev_free(g->d_kd);
     dev_free(g->d_clus); dev_free(g->d_work); dev_free(g->d_acc); dev_free(g->d_q); dev_free(g->d_tot);
-    dev_free(g->d_src); dev_free(g->d_part); dev_free(g->d_col32); dev_free(g->d_C32);
+    dev_free(g->d_src); dev_free(g->d_part); dev_free(g->d_col32); dev_free(g->d_C32); dev_free(g->d_wq);
     dev_free(g->d_gid); dev_free(g->d_gid0); dev_free(g->d_vb); dev_free(g->d_vbn); dev_free(g->d_mg);
     dev_free(g->d_twomg); dev_free(g->d_act); dev_free(g->d_keep); dev_free(g->d_gacc); dev_free(g->d_gq);
     dev_free(g->d_seg);
@@ -1489,6 +1529,7 @@ int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
     if ((t->split_small | 1) != 1) return set_error(NIDO_ERR_INVALID, "split_small must be 0 or 1");
     if ((t->overlap_long | 1) != 1) return set_error(NIDO_ERR_INVALID, "overlap_long must be 0 or 1");
     if ((t->full_modularity | 1) != 1) return set_error(NIDO_ERR_INVALID, "full_modularity must be 0 or 1");
+    if ((t->fixed_short | 1) != 1) return set_error(NIDO_ERR_INVALID, "fixed_short must be 0 or 1");
     if (!g) return set_error(NIDO_ERR_INVALID, "null handle");
     AllocScope scope(g->stream);
     API_GUARD(g->tuning = *t; return setup_phase(g);)
@@ -1600,10 +1641,26 @@ int nido_get_clustering(nido_graph* g, int64_t* comm)
 
 int nido_dump_partition(nido_graph* g, const char* path)
 {
+    return nido_dump_partition_orders(g, path, nullptr);
+}
+
+int nido_dump_partition_orders(nido_graph* g, const char* path, const int64_t* new_orders)
+{
     if (!g || !path) return set_error(NIDO_ERR_INVALID, "null argument");
     API_GUARD(
         std::vector<int64_t> c((size_t)std::max<int64_t>(g->n0, 1));
         NIDO_TRY(d2h(g, c.data(), g->d_clus, g->n0));
+        if (new_orders) {   // clustering.cpp:25-37: line new_orders[i] holds vertex i's community
+            std::vector<int64_t> t((size_t)std::max<int64_t>(g->n0, 1), -1);
+            for (int64_t i = 0; i < g->n0; ++i) {
+                const int64_t id = new_orders[i];
+                if (id < 0 || id >= g->n0 || t[(size_t)id] >= 0)
+                    return set_error(NIDO_ERR_INVALID, "new_orders is not a permutation of [0, %lld)",
+                                     (long long)g->n0);
+                t[(size_t)id] = c[(size_t)i];
+            }
+            c.swap(t);
+        }
         FILE* f = fopen(path, "wt");
         if (!f) return set_error(NIDO_ERR_IO, "cannot open %s", path);
         for (int64_t i = 0; i < g->n0; ++i) fprintf(f, "%lld\n", (long long)c[(size_t)i]);

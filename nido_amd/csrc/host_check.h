@@ -12,4 +12,9 @@ int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* 
               const int64_t* vptr, bool* nonneg);
 // Exact 2m of every graph: the Q64.64 sum of its arc weights.
 void graph_twom(const int64_t* off, const double* w, int G, const int64_t* vptr, fx_t* out);
+// The greedy colouring of Graph::coloring run by one thread (coloring.hip):
+// colors[i] = smallest colour not held by a neighbour j < i with
+// r[j] >= r[i]; off[nv + 1], col[off[nv]] (ids in [0, nv)), host arrays.
+int device_coloring(int64_t nv, const int64_t* off, const int* col, const int64_t* r, int* colors,
+                    int64_t* rounds);
 }  // namespace nido

@@ -9,6 +9,7 @@
 #include <vector>
 #include <omp.h>
 #include <parallel/algorithm>
+#include <random>
 
 #include "../../include/nido_amd.h"
 #include "errors.h"
@@ -405,6 +406,185 @@ int nido_gen_chung_lu(int64_t nv, double alpha, double dmin, double dmax, double
         sort_unique(e);
         if (!e.empty() && e.back() == ~0ull) e.pop_back();
         *out = csr_from_sorted_pairs(nv, e, weighted, seed ^ 0xA5A5A5A5DEADBEEFull);
+    } catch (const std::bad_alloc&) {
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    }
+    return NIDO_OK;
+}
+
+// Graph::create_random_network_ba(m0), graph.cpp:76-110, the reference's -r
+// generator, with the same random-number objects drawn in the same order:
+// two default-constructed std::default_random_engine (one for the weights,
+// one for the endpoints), one std::normal_distribution<double>(1, 1) whose
+// absolute values are the weights, and a fresh std::uniform_int_distribution
+// over the arcs listed so far for every new vertex.  Vertices 0..m0 start as
+// a star around 0; each later vertex i attaches m0 times to the first
+// endpoint of a uniformly chosen earlier arc (so by degree), duplicates
+// kept.  The arcs are then ordered by (tail, head), as the reference's heap
+// sort orders them (graph.cpp:62-72); the order among duplicate arcs, which
+// its heap leaves unspecified, is the draw order here and changes no result
+// (every sum is exact).  The draws are libstdc++'s (the C++ library this is
+// built with); 2 * m0 * (nv - m0) arcs.
+int nido_gen_ba(int64_t nv, int64_t m0, nido_hgraph** out)
+{
+    int rc = check_out(out);
+    if (rc) return rc;
+    if (m0 < 1 || nv < m0 + 1 || nv >= (1ll << 31) || m0 > (1ll << 20) ||
+        (double)m0 * (double)(nv - m0) * 2.0 >= 0x1p40)
+        return set_error(NIDO_ERR_INVALID, "invalid BA parameters (nv %lld, m0 %lld): need 1 <= m0 < nv < 2^31",
+                         (long long)nv, (long long)m0);
+    try {
+        struct Tuple {
+            int64_t x, y;
+            double w;
+        };
+        const int64_t ne = 2 * m0 * (nv - m0);
+        std::vector<Tuple> el((size_t)ne);
+        std::default_random_engine rand_gen1, rand_gen2;
+        std::normal_distribution<double> gaussian(1.0, 1.0);
+        int64_t total = 0;
+        for (int64_t i = 0; i < m0; ++i) {
+            const double w = std::fabs(gaussian(rand_gen1));
+            el[(size_t)total + 0] = {0, i + 1, w};
+            el[(size_t)total + 1] = {i + 1, 0, w};
+            total += 2;
+        }
+        for (int64_t i = m0 + 1; i < nv; ++i) {
+            std::uniform_int_distribution<int64_t> uniform(0, total - 1);
+            for (int64_t j = 0; j < m0; ++j) {
+                const int64_t t = el[(size_t)uniform(rand_gen2)].x;
+                const double w = std::fabs(gaussian(rand_gen1));
+                el[(size_t)(total + 2 * j + 0)] = {i, t, w};
+                el[(size_t)(total + 2 * j + 1)] = {t, i, w};
+            }
+            total += 2 * m0;
+        }
+        // (tail, head) order; counting sort by tail keeps the draw order
+        // within a row, then each row sorted by head
+        nido_hgraph* g = new nido_hgraph;
+        g->nv = nv;
+        g->ne = ne;
+        g->off.assign((size_t)nv + 1, 0);
+        for (const Tuple& t : el) g->off[(size_t)t.x + 1]++;
+        for (int64_t i = 0; i < nv; ++i) g->off[(size_t)i + 1] += g->off[(size_t)i];
+        g->col.resize((size_t)ne);
+        g->w.resize((size_t)ne);
+        {
+            std::vector<int64_t> cur(g->off.begin(), g->off.end() - 1);
+            for (const Tuple& t : el) {
+                const int64_t p = cur[(size_t)t.x]++;
+                g->col[(size_t)p] = t.y;
+                g->w[(size_t)p] = t.w;
+            }
+        }
+        std::vector<Tuple>().swap(el);
+        #pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t v = 0; v < nv; ++v) {
+            const int64_t a = g->off[(size_t)v], b = g->off[(size_t)v + 1];
+            std::vector<std::pair<int64_t, double>> row((size_t)(b - a));
+            for (int64_t e = a; e < b; ++e) row[(size_t)(e - a)] = {g->col[(size_t)e], g->w[(size_t)e]};
+            std::stable_sort(row.begin(), row.end(),
+                             [](const std::pair<int64_t, double>& p, const std::pair<int64_t, double>& q) {
+                                 return p.first < q.first;
+                             });
+            for (int64_t e = a; e < b; ++e) {
+                g->col[(size_t)e] = row[(size_t)(e - a)].first;
+                g->w[(size_t)e] = row[(size_t)(e - a)].second;
+            }
+        }
+        *out = g;
+    } catch (const std::bad_alloc&) {
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    }
+    return NIDO_OK;
+}
+
+// Graph::coloring, graph.cpp:547-694 (the reference's -c): random weights,
+// the greedy colouring (on the device), then the vertices renumbered in
+// (colour, id) order with the CSR permuted to match.
+//
+// Random weights (graph.cpp:558-571): `engines` std::mt19937_64 seeded
+// 1 << t, each with a std::uniform_int_distribution<int64_t>(0, 4 nv);
+// vertex i takes the next draw of engine i % engines.  With one engine this
+// is the reference's sequence exactly; with several, the reference's threads
+// share the engines without synchronisation (a data race), and this takes
+// each engine's draws in vertex order, the race-free reading.  The colouring
+// itself is the reference's loop run by one thread (graph.cpp:573-605):
+// colour(i) = the smallest colour not held by a neighbour j < i with
+// r_j >= r_i; with several threads the reference's result depends on their
+// timing.
+int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* new_orders,
+                         int64_t* num_colors)
+{
+    if (!g || !new_orders) return set_error(NIDO_ERR_INVALID, "null argument");
+    if (engines < 1 || engines > 31) return set_error(NIDO_ERR_INVALID, "engines must be in [1, 31]");
+    const int64_t nv = g->nv, ne = g->ne;
+    if (nv >= (1ll << 31) - 1) return set_error(NIDO_ERR_CAPACITY, "coloring needs fewer than 2^31-1 vertices");
+    if (nv == 0) {
+        if (num_colors) *num_colors = 0;
+        return NIDO_OK;
+    }
+    try {
+        for (int64_t e = 0; e < ne; ++e)
+            if (g->col[(size_t)e] < 0 || g->col[(size_t)e] >= nv)
+                return set_error(NIDO_ERR_INVALID, "arc %lld: column outside [0, %lld)", (long long)e, (long long)nv);
+        std::vector<int64_t> r((size_t)nv);
+        {
+            std::vector<std::mt19937_64> eng((size_t)engines);
+            for (int t = 0; t < engines; ++t) eng[(size_t)t] = std::mt19937_64(1ull << t);
+            std::uniform_int_distribution<int64_t> dist(0, nv * 4);
+            for (int64_t i = 0; i < nv; ++i) {
+                // stored through a double as the reference does (exact below 2^53)
+                const double x = (double)dist(eng[(size_t)(i % engines)]);
+                r[(size_t)i] = (int64_t)x;
+            }
+        }
+        std::vector<int> col32((size_t)ne);
+        #pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < ne; ++e) col32[(size_t)e] = (int)g->col[(size_t)e];
+        std::vector<int> color((size_t)nv);
+        int64_t rounds = 0;
+        int rc = device_coloring(nv, g->off.data(), col32.data(), r.data(), color.data(), &rounds);
+        if (rc != NIDO_OK) return rc;
+        std::vector<int>().swap(col32);
+        // (colour, id) order, graph.cpp:609-629: a counting sort by colour
+        // keeps ids ascending within a colour
+        int maxc = 0;
+        for (int64_t i = 0; i < nv; ++i) maxc = std::max(maxc, color[(size_t)i]);
+        const int64_t K = (int64_t)maxc + 1;
+        std::vector<int64_t> start((size_t)K + 1, 0);
+        for (int64_t i = 0; i < nv; ++i) start[(size_t)color[(size_t)i] + 1]++;
+        for (int64_t c = 0; c < K; ++c) start[(size_t)c + 1] += start[(size_t)c];
+        std::vector<int64_t> orders((size_t)nv);
+        for (int64_t i = 0; i < nv; ++i) {
+            const int64_t pos = start[(size_t)color[(size_t)i]]++;
+            orders[(size_t)i] = pos;
+            new_orders[pos] = i;
+        }
+        if (colors)
+            for (int64_t i = 0; i < nv; ++i) colors[i] = color[(size_t)i];
+        // renumbered CSR, graph.cpp:631-691: row pos is old row new_orders[pos]
+        // with its arcs in their order, every column mapped through orders
+        std::vector<int64_t> noff((size_t)nv + 1, 0);
+        for (int64_t p = 0; p < nv; ++p) {
+            const int64_t v = new_orders[p];
+            noff[(size_t)p + 1] = noff[(size_t)p] + (g->off[(size_t)v + 1] - g->off[(size_t)v]);
+        }
+        std::vector<int64_t> ncol((size_t)ne);
+        std::vector<double> nw((size_t)ne);
+        #pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t p = 0; p < nv; ++p) {
+            const int64_t v = new_orders[p];
+            const int64_t a = g->off[(size_t)v], d = g->off[(size_t)v + 1] - a, b = noff[(size_t)p];
+            for (int64_t k = 0; k < d; ++k) {
+                ncol[(size_t)(b + k)] = orders[(size_t)g->col[(size_t)(a + k)]];
+                nw[(size_t)(b + k)] = g->w[(size_t)(a + k)];
+            }
+        }
+        g->off.swap(noff);
+        g->col.swap(ncol);
+        g->w.swap(nw);
+        if (num_colors) *num_colors = K;
     } catch (const std::bad_alloc&) {
         return set_error(NIDO_ERR_NOMEM, "host allocation failed");
     }

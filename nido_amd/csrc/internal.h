@@ -145,6 +145,12 @@ struct LmArgs {
     const int* gid;
     const double* mg;
     const unsigned char* act;
+    // Short rows in 64-bit fixed point (wq null: Q64.64): every weight is
+    // wq[e] * 2^-F exactly and every short row's sums stay below 2^63 in
+    // those units, so the sums are the same exact values with one 64-bit
+    // word instead of two; wscale = 2^-F.
+    const int64_t* wq;
+    double wscale;
 };
 
 // ---- launch helpers (kernels.hip) ----------------------------------------
@@ -168,6 +174,12 @@ void launch_row_ids(const int64_t* off, const int* small, int64_t nsmall, const 
                     int64_t nitems, int* src, hipStream_t s);
 void launch_singleton(int64_t* C, int64_t* Cn, int* C32, fx_t* a, const fx_t* k, int64_t nv,
                       hipStream_t s);
+// *fneed = the fractional bits the finest weight needs (max over nonzero
+// weights; 2000 for a subnormal or non-finite one, INT_MIN when every weight
+// is 0), *wmax = the bits of max |w|.
+void launch_weight_grid(const double* w, int64_t ne, int* fneed, unsigned long long* wmax, hipStream_t s);
+// wq[e] = w[e] * scale, exact for weights on the 1/scale grid.
+void launch_to_fixed(const double* w, int64_t* wq, int64_t ne, double scale, hipStream_t s);
 // col32[e] = (int)col[e]
 void launch_narrow_ids(const int64_t* col, int* col32, int64_t n, hipStream_t s);
 void launch_widen_ids(const int* col32, int64_t* col, int64_t n, hipStream_t s);
@@ -209,10 +221,12 @@ struct SegItem {
     int g;
     int64_t e0, e1, v0, v1;
 };
-// Per-graph modularity: acc holds 2 accumulators per graph, q[g] receives Q_g.
+// Per-graph modularity: acc holds 2 accumulators per graph, q[g] receives Q_g;
+// with act set, only for the graphs g with act[g] set (the others' q[g] is
+// left as it was).
 void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src, const int* col,
                            const double* w, const int* C, const fx_t* a, const double* twom_g, int G,
-                           fx_t* acc, double* q, hipStream_t s);
+                           fx_t* acc, double* q, hipStream_t s, const unsigned char* act = nullptr);
 // out[g] = newid[vb[g]] for g <= G (the total where vb[g] == nv).
 void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64_t* vb, int G, int64_t nv,
                          int64_t* out, hipStream_t s);

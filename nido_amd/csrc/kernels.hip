@@ -120,6 +120,49 @@ widen_kernel(const int* __restrict__ col32, int64_t* __restrict__ col, int64_t n
         col[i] = col32[i];
 }
 
+// The grid every weight lies on: *fneed = max over nonzero weights of the
+// fractional bits the weight needs (w = sig 2^(ex - 1075), lowest set bit
+// 2^(ex - 1075 + ctz(sig)); 2000 for a subnormal or non-finite weight) and
+// *wmax = the bits of max |w| (|x| bit patterns order like the values).
+__global__ void __launch_bounds__(kBlock)
+weight_grid_kernel(const double* __restrict__ w, int64_t ne, int* fneed, unsigned long long* wmax)
+{
+    int f = INT_MIN;
+    unsigned long long mx = 0;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+        const unsigned long long b = nido_dbits(w[e]) & 0x7fffffffffffffffull;
+        if (b == 0) continue;
+        const int ex = (int)(b >> 52);
+        int need = 2000;
+        if (ex != 0 && ex != 0x7ff) {
+            const unsigned long long sig = (b & 0x000fffffffffffffull) | 0x0010000000000000ull;
+            need = 1075 - ex - __builtin_ctzll(sig);
+        }
+        f = need > f ? need : f;
+        mx = b > mx ? b : mx;
+    }
+#pragma unroll
+    for (int msk = 32; msk > 0; msk >>= 1) {
+        const int of = __shfl_xor(f, msk, 64);
+        const unsigned long long om = __shfl_xor(mx, msk, 64);
+        f = of > f ? of : f;
+        mx = om > mx ? om : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (f != INT_MIN) atomicMax(fneed, f);
+        if (mx) atomicMax(wmax, mx);
+    }
+}
+
+// wq[e] = w[e] * 2^F, exact: every weight is a multiple of 2^-F and
+// |w| 2^F < 2^63 (checked by the caller).
+__global__ void __launch_bounds__(kBlock)
+to_fixed_kernel(const double* __restrict__ w, int64_t* __restrict__ wq, int64_t ne, double scale)
+{
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock)
+        wq[e] = (int64_t)(w[e] * scale);
+}
+
 // src[e] = the row of arc e, from the per-phase row lists.
 __global__ void __launch_bounds__(kBlock)
 src_small_kernel(const int64_t* __restrict__ off, const int* __restrict__ small, int64_t n,
@@ -465,11 +508,13 @@ coarse_emit_kernel(const int64_t* __restrict__ toff, const int* __restrict__ tke
 __global__ void __launch_bounds__(kBlock)
 mod_seg_kernel(const SegItem* __restrict__ items, int64_t nitems, const int* __restrict__ src,
                const int* __restrict__ col, const double* __restrict__ w, const int* __restrict__ C,
-               const fx_t* __restrict__ a, const double* __restrict__ twom_g, fx_t* __restrict__ acc)
+               const fx_t* __restrict__ a, const double* __restrict__ twom_g, fx_t* __restrict__ acc,
+               const unsigned char* __restrict__ act)
 {
     __shared__ fx_t red[kBlock / 64];
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const SegItem s = items[it];
+        if (act && !act[s.g]) continue;   // uniform: a graph that sat out kept its partition
         fx_t I = 0, T = 0;
         for (int64_t e = s.e0 + threadIdx.x; e < s.e1; e += kBlock)
             if (C[src[e]] == C[col[e]]) I += fx_from_double(w[e]);    // self arcs are internal
@@ -489,9 +534,10 @@ mod_seg_kernel(const SegItem* __restrict__ items, int64_t nitems, const int* __r
 
 __global__ void __launch_bounds__(kBlock)
 mod_seg_finalize_kernel(const fx_t* __restrict__ acc, const double* __restrict__ twom_g, int G,
-                        double* __restrict__ q)
+                        double* __restrict__ q, const unsigned char* __restrict__ act)
 {
     for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock) {
+        if (act && !act[g]) continue;     // q[g] keeps the value of its last evaluation
         const double tw = twom_g[g];
         q[g] = tw == 0 ? 0.0 : fx_to_double(acc[2 * g]) / tw - fx_to_double(acc[2 * g + 1]);
     }
@@ -523,6 +569,18 @@ void launch_fill_i64(int64_t* p, int64_t val, int64_t n, hipStream_t s)
 {
     if (n <= 0) return;
     fill_kernel<int64_t><<<grid_for(n, kBlock * 4), kBlock, 0, s>>>(p, val, n);
+}
+
+void launch_weight_grid(const double* w, int64_t ne, int* fneed, unsigned long long* wmax, hipStream_t s)
+{
+    launch_fill_i32(fneed, INT_MIN, 1, s);
+    launch_fill_i64(reinterpret_cast<int64_t*>(wmax), 0, 1, s);
+    if (ne > 0) weight_grid_kernel<<<grid_for(ne, kBlock * 8, 2048), kBlock, 0, s>>>(w, ne, fneed, wmax);
+}
+
+void launch_to_fixed(const double* w, int64_t* wq, int64_t ne, double scale, hipStream_t s)
+{
+    if (ne > 0) to_fixed_kernel<<<grid_for(ne, kBlock * 4), kBlock, 0, s>>>(w, wq, ne, scale);
 }
 
 void launch_iota_i64(int64_t* p, int64_t n, hipStream_t s)
@@ -679,14 +737,14 @@ void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* 
 
 void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src, const int* col,
                            const double* w, const int* C, const fx_t* a, const double* twom_g, int G,
-                           fx_t* acc, double* q, hipStream_t s)
+                           fx_t* acc, double* q, hipStream_t s, const unsigned char* act)
 {
     if (G <= 0) return;
     launch_fill_fx(acc, 2 * (int64_t)G, s);
     if (nitems > 0)
         mod_seg_kernel<<<grid_for(nitems, 1, 8192), kBlock, 0, s>>>(items, nitems, src, col, w, C, a,
-                                                                    twom_g, acc);
-    mod_seg_finalize_kernel<<<grid_for(G, kBlock), kBlock, 0, s>>>(acc, twom_g, G, q);
+                                                                    twom_g, acc, act);
+    mod_seg_finalize_kernel<<<grid_for(G, kBlock), kBlock, 0, s>>>(acc, twom_g, G, q, act);
 }
 
 void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64_t* vb, int G, int64_t nv,

@@ -23,6 +23,8 @@
 // Launch 1 = short-row bins + long-row key fill (23 KB LDS per workgroup);
 // launch 2 = long-row partitions (40 KB LDS with the default 2048-slot
 // table).  Row loops keep two to eight independent loads in flight per lane.
+#include <type_traits>
+
 #include "device_common.h"
 
 namespace nido {
@@ -120,12 +122,36 @@ __device__ __forceinline__ void consider(const LmArgs& A, Mass m, double ki, int
     }
 }
 
+// Candidate update for a candidate other than the own community, from
+// e_{v,c} already rounded to FP64.
+__device__ __forceinline__ void consider_e(Mass m, double ki, int c, double e, fx_t ac, double& bg, int& bc)
+{
+    double g = e;
+    g -= div_rcp(ki * fx_to_double(ac), 2. * m.m, 0.5 * m.y);
+    g = div_rcp(g, m.m, m.y);
+    if (better(g, c, bg, bc)) {
+        bg = g;
+        bc = c;
+    }
+}
+
+// e_{v,c} of table slot s rounded to FP64: a Q64.64 sum in {lo, hi}, or
+// (Fixed) a 64-bit sum in units of 2^-F in lo, scaled by wscale = 2^-F
+// (the int64 -> FP64 conversion rounds once; the scaling is exact).
+template <bool Fixed>
+__device__ __forceinline__ double slot_sum(const LmArgs& A, const unsigned long long* lo,
+                                           const unsigned long long* hi, int s)
+{
+    if constexpr (Fixed) return (double)(int64_t)lo[s] * A.wscale;
+    else return fx_to_double(fx_make(lo[s], (int64_t)hi[s]));
+}
+
 // Scores the listed slots of one row's table with L lanes: a row of a
 // short-row bin has at most 2L arcs, hence at most 2L keys, so each lane
 // takes at most two, with both a_c gathers in flight.  The lane holding the
 // row's own community stores e_{v,own} (rounded) in *self_d, so only the
 // argmax needs a cross-lane reduction.
-template <int L>
+template <int L, bool Fixed>
 __device__ __forceinline__ void sweep_list(const LmArgs& A, Mass m, double ki, int own, const int* key,
                                            const unsigned long long* lo, const unsigned long long* hi,
                                            const int* klist, int nk, int lane, double* self_d, double& bg, int& bc)
@@ -136,10 +162,16 @@ __device__ __forceinline__ void sweep_list(const LmArgs& A, Mass m, double ki, i
     const int c2 = s2 >= 0 ? key[s2] : -1;
     const fx_t a1 = A.a[c1 >= 0 ? c1 : 0];
     const fx_t a2 = A.a[c2 >= 0 ? c2 : 0];
-    fx_t self = 0;
-    if (c1 >= 0) consider_a(m, ki, own, c1, fx_make(lo[s1], (int64_t)hi[s1]), a1, self, bg, bc);
-    if (c2 >= 0) consider_a(m, ki, own, c2, fx_make(lo[s2], (int64_t)hi[s2]), a2, self, bg, bc);
-    if (c1 == own || c2 == own) *self_d = fx_to_double(self);
+    if (c1 >= 0) {
+        const double e = slot_sum<Fixed>(A, lo, hi, s1);
+        if (c1 == own) *self_d = e;
+        else consider_e(m, ki, c1, e, a1, bg, bc);
+    }
+    if (c2 >= 0) {
+        const double e = slot_sum<Fixed>(A, lo, hi, s2);
+        if (c2 == own) *self_d = e;
+        else consider_e(m, ki, c2, e, a2, bg, bc);
+    }
 }
 
 // Makes one lane's LDS writes visible to the other lanes of its wavefront.
@@ -180,10 +212,32 @@ __device__ __forceinline__ void lds_insert(int* key, unsigned long long* lo, uns
     fx_atomic_add(&lo[h], &hi[h], wf);
 }
 
+// The same with a 64-bit fixed-point value (no carry word).
+template <int S>
+__device__ __forceinline__ void lds_insert64(int* key, unsigned long long* lo, int* nkeys, int* klist, int c,
+                                             int64_t x)
+{
+    unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(S));
+    for (;;) {
+        const int kk = key[h];
+        if (kk == c) break;
+        if (kk < 0) {
+            const int prev = atomicCAS(&key[h], -1, c);
+            if (prev < 0) {
+                klist[atomicAdd(nkeys, 1)] = (int)h;
+                break;
+            }
+            if (prev == c) break;
+        }
+        h = (h + 1) & (S - 1);
+    }
+    atomicAdd(&lo[h], (unsigned long long)x);
+}
+
 // Scans row v with `lanes` lanes (lane index `lane`), two arcs per lane per
 // step, inserting every non-self arc into the S-slot table; the slots of new
 // keys are appended to klist (nkeys counts them).
-template <int S>
+template <int S, bool Fixed>
 __device__ __forceinline__ void scan_row(const LmArgs& A, int v, int64_t e0, int64_t e1, int lane, int lanes, int* key,
                                          unsigned long long* lo, unsigned long long* hi, int* nkeys, int* klist)
 {
@@ -192,18 +246,25 @@ __device__ __forceinline__ void scan_row(const LmArgs& A, int v, int64_t e0, int
         const bool two = f < e1;
         const int u1 = (int)A.col[e];
         const int u2 = two ? (int)A.col[f] : v;
-        const double w1 = A.w[e];
-        const double w2 = two ? A.w[f] : 0.0;
         const int c1 = u1 != v ? (int)A.C[u1] : -1;     // self arcs weigh 0 (graph_cuda.cu:2312-2315)
         const int c2 = u2 != v ? (int)A.C[u2] : -1;
-        if (c1 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c1, fx_from_double(w1));
-        if (c2 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c2, fx_from_double(w2));
+        if constexpr (Fixed) {
+            const int64_t w1 = A.wq[e];
+            const int64_t w2 = two ? A.wq[f] : 0;
+            if (c1 >= 0) lds_insert64<S>(key, lo, nkeys, klist, c1, w1);
+            if (c2 >= 0) lds_insert64<S>(key, lo, nkeys, klist, c2, w2);
+        } else {
+            const double w1 = A.w[e];
+            const double w2 = two ? A.w[f] : 0.0;
+            if (c1 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c1, fx_from_double(w1));
+            if (c2 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c2, fx_from_double(w2));
+        }
     }
 }
 
 // ---- launch 1 bodies (blocks [bi, bi+nbk) of the fused launch) -------------
 
-template <bool Multi>
+template <bool Multi, bool Fixed>
 __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk)
 {
     for (int64_t i = (int64_t)bi * kBlock + threadIdx.x; i < n; i += (int64_t)nbk * kBlock) {
@@ -217,17 +278,19 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
         const int64_t e0 = r.e0;
         const int d = r.d;
         const fx_t rest = A.a[own] - kv;
+        using W = typename std::conditional<Fixed, int64_t, double>::type;
         int uu[kThreadMaxDeg];
-        double wd[kThreadMaxDeg];
+        W wd[kThreadMaxDeg];
 #pragma unroll
         for (int j = 0; j < kThreadMaxDeg; ++j) {
             uu[j] = j < d ? (int)A.col[e0 + j] : v;
-            wd[j] = j < d ? A.w[e0 + j] : 0.0;
+            if constexpr (Fixed) wd[j] = j < d ? A.wq[e0 + j] : 0;
+            else wd[j] = j < d ? A.w[e0 + j] : 0.0;
         }
         int cc[kThreadMaxDeg];
 #pragma unroll
         for (int j = 0; j < kThreadMaxDeg; ++j) cc[j] = uu[j] != v ? (int)A.C[uu[j]] : -1;
-        fx_t self = 0;
+        double self_d = 0.0;
         double bg = -kMaxFloat;
         int bc = kNone;
 #pragma unroll
@@ -236,19 +299,33 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
 #pragma unroll
             for (int q = 0; q < j; ++q) first = first && (cc[q] != cc[j]);
             if (!first) continue;
-            fx_t e = fx_from_double(wd[j]);
+            double e_d;
+            if constexpr (Fixed) {
+                int64_t e = wd[j];
 #pragma unroll
-            for (int q = j + 1; q < kThreadMaxDeg; ++q)
-                if (cc[q] == cc[j]) e += fx_from_double(wd[q]);
-            consider(A, m, ki, own, cc[j], e, self, bg, bc);
+                for (int q = j + 1; q < kThreadMaxDeg; ++q)
+                    if (cc[q] == cc[j]) e += wd[q];
+                e_d = (double)e * A.wscale;
+            } else {
+                fx_t e = fx_from_double(wd[j]);
+#pragma unroll
+                for (int q = j + 1; q < kThreadMaxDeg; ++q)
+                    if (cc[q] == cc[j]) e += fx_from_double(wd[q]);
+                e_d = fx_to_double(e);
+            }
+            if (cc[j] == own) self_d = e_d;
+            else {
+                const fx_t ac = A.a[cc[j]];
+                consider_e(m, ki, cc[j], e_d, ac, bg, bc);
+            }
         }
-        A.Cn[v] = decide(m, ki, rest, own, self, bg, bc);
+        A.Cn[v] = decide_d(m, ki, rest, own, self_d, bg, bc);
     }
 }
 
 // G lanes per vertex (G <= 64), S-slot LDS table per vertex; 256/G vertices
 // per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
-template <int G, int S, bool Multi>
+template <int G, int S, bool Multi, bool Fixed>
 __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
                            char* lds, int* kcnt, int* klist_all, double* sself)
 {
@@ -270,7 +347,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
         for (int s = lane; s < S; s += G) {
             key[s] = -1;
             lo[s] = 0;
-            hi[s] = 0;
+            if constexpr (!Fixed) hi[s] = 0;
         }
         if (lane == 0) {
             *nkeys = 0;
@@ -286,12 +363,12 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
         const double ki = act ? A.kd[v] : 0.0;
         const fx_t rest = act ? A.a[own] - A.k[v] : fx_t(0);
         wave_lds_sync();
-        if (act) scan_row<S>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi, nkeys, klist);
+        if (act) scan_row<S, Fixed>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi, nkeys, klist);
         wave_lds_sync();
         if (act) {   // uniform within the group
             double bg = -kMaxFloat;
             int bc = kNone;
-            sweep_list<G>(A, m, ki, own, key, lo, hi, klist, *nkeys, lane, self_d, bg, bc);
+            sweep_list<G, Fixed>(A, m, ki, own, key, lo, hi, klist, *nkeys, lane, self_d, bg, bc);
 #pragma unroll
             for (int msk = G / 2; msk > 0; msk >>= 1) {
                 const double og = __shfl_xor(bg, msk, G);
@@ -309,7 +386,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
 }
 
 // One workgroup per vertex, S-slot LDS table.
-template <int S, bool Multi>
+template <int S, bool Multi, bool Fixed>
 __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
                            char* lds, BlockRed& R, int* kcnt, int* klist, double* self_d)
 {
@@ -321,7 +398,7 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
         for (int s = threadIdx.x; s < S; s += kBlock) {
             key[s] = -1;
             lo[s] = 0;
-            hi[s] = 0;
+            if constexpr (!Fixed) hi[s] = 0;
         }
         if (threadIdx.x == 0) {
             *kcnt = 0;
@@ -335,11 +412,11 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
         const double ki = A.kd[v];
         const fx_t rest = A.a[own] - A.k[v];
         __syncthreads();
-        scan_row<S>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi, kcnt, klist);
+        scan_row<S, Fixed>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi, kcnt, klist);
         __syncthreads();
         double bg = -kMaxFloat;
         int bc = kNone;
-        sweep_list<kBlock>(A, m, ki, own, key, lo, hi, klist, *kcnt, threadIdx.x, self_d, bg, bc);
+        sweep_list<kBlock, Fixed>(A, m, ki, own, key, lo, hi, klist, *kcnt, threadIdx.x, self_d, bg, bc);
         block_argmax(R, bg, bc);         // its barriers also publish *self_d
         if (threadIdx.x == 0) A.Cn[v] = decide_d(m, ki, rest, own, *self_d, bg, bc);
     }
@@ -422,7 +499,7 @@ __device__ __forceinline__ void keyfill_body(const LmArgs& A, const SmallBins& L
 constexpr int kSmallLds = 20480;   // (256/G)*S*20 for every short-row bin
 static_assert((kMaxParts + 1 + kBlock / 64) * 4 <= kSmallLds, "key-fill counters exceed the LDS buffer");
 
-template <bool Multi>
+template <bool Multi, bool Fixed>
 __global__ void __launch_bounds__(kBlock, 7)
 lm_small_kernel(LmArgs A, SmallBins L)
 {
@@ -441,12 +518,12 @@ lm_small_kernel(LmArgs A, SmallBins L)
     for (int k = 1; k <= NUM_SMALL_BINS + 1; ++k) bin += b >= L.blk0[k];
 #define NIDO_BIN_ARGS(K) L.verts[K], L.n[K], b - L.blk0[K], L.blk0[K + 1] - L.blk0[K]
     switch (bin) {
-    case BIN_THREAD: thread_body<Multi>(A, NIDO_BIN_ARGS(BIN_THREAD)); break;
-    case BIN_G8: group_body<8, 32, Multi>(A, NIDO_BIN_ARGS(BIN_G8), lds, kcnt, klist, sself); break;
-    case BIN_G16: group_body<16, 64, Multi>(A, NIDO_BIN_ARGS(BIN_G16), lds, kcnt, klist, sself); break;
-    case BIN_G32: group_body<32, 128, Multi>(A, NIDO_BIN_ARGS(BIN_G32), lds, kcnt, klist, sself); break;
-    case BIN_G64: group_body<64, 256, Multi>(A, NIDO_BIN_ARGS(BIN_G64), lds, kcnt, klist, sself); break;
-    case BIN_B256: block_body<1024, Multi>(A, NIDO_BIN_ARGS(BIN_B256), lds, R, kcnt, klist, sself); break;
+    case BIN_THREAD: thread_body<Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_THREAD)); break;
+    case BIN_G8: group_body<8, 32, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G8), lds, kcnt, klist, sself); break;
+    case BIN_G16: group_body<16, 64, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G16), lds, kcnt, klist, sself); break;
+    case BIN_G32: group_body<32, 128, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G32), lds, kcnt, klist, sself); break;
+    case BIN_G64: group_body<64, 256, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G64), lds, kcnt, klist, sself); break;
+    case BIN_B256: block_body<1024, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_B256), lds, R, kcnt, klist, sself); break;
     case NUM_SMALL_BINS:
         keyfill_body<Multi>(A, L, b - L.blk0[NUM_SMALL_BINS], L.blk0[NUM_SMALL_BINS + 1] - L.blk0[NUM_SMALL_BINS],
                             lds);
@@ -1060,8 +1137,13 @@ int small_bins_blocks(SmallBins& L, unsigned cap_per_bin)
 
 static void small_launch(const LmArgs& A, const SmallBins& L, int blocks, hipStream_t s)
 {
-    if (A.gid) lm_small_kernel<true><<<blocks, kBlock, 0, s>>>(A, L);
-    else lm_small_kernel<false><<<blocks, kBlock, 0, s>>>(A, L);
+    if (A.gid) {
+        if (A.wq) lm_small_kernel<true, true><<<blocks, kBlock, 0, s>>>(A, L);
+        else lm_small_kernel<true, false><<<blocks, kBlock, 0, s>>>(A, L);
+    } else {
+        if (A.wq) lm_small_kernel<false, true><<<blocks, kBlock, 0, s>>>(A, L);
+        else lm_small_kernel<false, false><<<blocks, kBlock, 0, s>>>(A, L);
+    }
 }
 
 void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s)

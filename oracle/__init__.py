@@ -76,6 +76,8 @@ _sig("orc_times", None, C.c_void_p, _pdbl, _pdbl, _pdbl, _pi64)
 _sig("orc_threads", C.c_int, C.c_void_p)
 _sig("orc_fx_from_double_words", None, C.c_double, _pu64, _pi64, C.POINTER(C.c_int))
 _sig("orc_fx_words_to_double", C.c_double, C.c_uint64, C.c_int64)
+_sig("orc_coloring", _i64, _i64, _pi64, _pi64, _pi64, _pi64)
+_sig("orc_reorder", None, _i64, _pi64, _pi64, _pdbl, _pi64, _pi64, _pi64, _pi64, _pdbl)
 
 
 def fx_from_double(w: float) -> int:
@@ -261,3 +263,36 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+
+def coloring(off, col, r):
+    """Colour of every vertex by the reference's colouring loop run by one
+    thread (graph.cpp:573-605), for random weights r; returns (colors, K)."""
+    off = np.ascontiguousarray(off, np.int64)
+    col = np.ascontiguousarray(col, np.int64)
+    r = np.ascontiguousarray(r, np.int64)
+    nv = len(off) - 1
+    color = np.empty(nv, np.int64)
+    K = _lib.orc_coloring(nv, _p(off), _p(col), _p(r), _p(color))
+    return color, int(K)
+
+
+def reorder(off, col, w, color):
+    """(new_orders, off, col, w) of the graph renumbered in (colour, id)
+    order, graph.cpp:607-691."""
+    off = np.ascontiguousarray(off, np.int64)
+    col = np.ascontiguousarray(col, np.int64)
+    w = np.ascontiguousarray(w, np.float64)
+    color = np.ascontiguousarray(color, np.int64)
+    nv = len(off) - 1
+    no = np.empty(nv, np.int64)
+    noff = np.empty(nv + 1, np.int64)
+    ncol = np.empty(len(col), np.int64)
+    nw = np.empty(len(col), np.float64)
+    _lib.orc_reorder(nv, _p(off), _p(col), w.ctypes.data_as(_pdbl), _p(color), _p(no), _p(noff), _p(ncol),
+                     nw.ctypes.data_as(_pdbl))
+    return no, noff, ncol, nw
+
+
+def _p(a):
+    return a.ctypes.data_as(_pi64)

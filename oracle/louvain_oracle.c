@@ -806,3 +806,72 @@ void orc_times(const orc_ctx* c, double* total, double* loop, double* agg, int64
 }
 
 int orc_threads(const orc_ctx* c) { return c->nthreads; }
+
+/* ------------------------------------------------------------------ */
+/* Colouring + renumbering (the reference's -c), graph.cpp:547-694    */
+/* ------------------------------------------------------------------ */
+
+/* graph.cpp:573-605 as one thread runs it: vertices in id order; vertex i
+ * collects the colours of its neighbours j that are already coloured (so
+ * j < i) with r[j] >= r[i] (graph.cpp:583-592) and takes the smallest colour
+ * not among them (graph.cpp:594-601).  Returns the colour count. */
+int64_t orc_coloring(int64_t nv, const int64_t* off, const int64_t* col, const int64_t* r, int64_t* color)
+{
+    int64_t maxc = -1;
+    unsigned char* seen = (unsigned char*)calloc((size_t)nv + 1, 1);
+    for (int64_t i = 0; i < nv; ++i) color[i] = -1;
+    for (int64_t i = 0; i < nv; ++i) {
+        const int64_t d = off[i + 1] - off[i];
+        /* the smallest free colour is at most d: mark colours 0..d only */
+        for (int64_t k = off[i]; k < off[i + 1]; ++k) {
+            const int64_t j = col[k];
+            const int64_t jc = color[j];
+            if (jc != -1 && r[i] <= r[j] && jc <= d) seen[jc] = 1;
+        }
+        int64_t c = 0;
+        while (seen[c]) ++c;
+        color[i] = c;
+        if (c > maxc) maxc = c;
+        for (int64_t k = off[i]; k < off[i + 1]; ++k) {
+            const int64_t jc = color[col[k]];
+            if (jc >= 0 && jc <= d) seen[jc] = 0;
+        }
+        seen[c] = 0;
+    }
+    free(seen);
+    return maxc + 1;
+}
+
+/* graph.cpp:607-691: (colour, id) order; new_orders[pos] = old id,
+ * orders[old id] = pos; the CSR rebuilt with row pos = old row
+ * new_orders[pos] (arc order kept) and every column mapped through orders.
+ * noff[nv+1], ncol[ne], nw[ne] out. */
+void orc_reorder(int64_t nv, const int64_t* off, const int64_t* col, const double* w, const int64_t* color,
+                 int64_t* new_orders, int64_t* noff, int64_t* ncol, double* nw)
+{
+    int64_t* orders = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nv > 0 ? nv : 1));
+    int64_t pos = 0;
+    /* a stable pass per colour, ids ascending (the sort of graph.cpp:614) */
+    int64_t maxc = -1;
+    for (int64_t i = 0; i < nv; ++i)
+        if (color[i] > maxc) maxc = color[i];
+    for (int64_t c = 0; c <= maxc; ++c)
+        for (int64_t i = 0; i < nv; ++i)
+            if (color[i] == c) {
+                orders[i] = pos;
+                new_orders[pos++] = i;
+            }
+    noff[0] = 0;
+    for (int64_t p = 0; p < nv; ++p) {
+        const int64_t v = new_orders[p];
+        noff[p + 1] = noff[p] + (off[v + 1] - off[v]);
+    }
+    for (int64_t p = 0; p < nv; ++p) {
+        const int64_t v = new_orders[p];
+        for (int64_t k = 0; k < off[v + 1] - off[v]; ++k) {
+            ncol[noff[p] + k] = orders[col[off[v] + k]];
+            nw[noff[p] + k] = w[off[v] + k];
+        }
+    }
+    free(orders);
+}

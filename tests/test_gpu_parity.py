@@ -134,6 +134,7 @@ KNOBS = [
     {"big_slots": 1024, "big_key_limit": 512, "big_part_arcs": 384, "overlap_long": 1},
     {"full_modularity": 1},                                         # every iteration sweeps all arcs for Q
     {"full_modularity": 1, "use_graph": 2},
+    {"fixed_short": 0},                                             # short rows in Q64.64 instead of 64-bit fixed point
 ]
 
 
@@ -166,6 +167,30 @@ def test_weight_ranges(scale_w):
     got = run_gpu(off, col, w, 4)
     ref = run_oracle(off, col, w, 4)
     assert_runs_equal(ref, got)
+
+
+@pytest.mark.parametrize("fixed_short", [1, 0])
+def test_signed_and_large_integer_weights(fixed_short):
+    """Short-row sums in 64-bit fixed point with integer weights (F = 0) of
+    magnitude up to 97 * 2^38, a seventh of them negative (two's-complement
+    sums that change sign), against the oracle; and the same graph with the
+    short rows in Q64.64."""
+    na = _na()
+    off, col, w = na.gen_rmat(11, 16, 0.57, 0.19, 0.19, True, 8).arrays()
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    lo, hi = np.minimum(src, col), np.maximum(src, col)
+    h = (lo * 2654435761 + hi * 40503) % 97                       # one value per undirected edge
+    w = np.ldexp(1.0 + h, 38) * np.where(h % 7 == 0, -1.0, 1.0)    # exact integers up to 97 * 2^38
+    print(f"case: rmat s11 signed integer weights, fixed_short={fixed_short}", flush=True)
+    ref = run_oracle(off, col, w, 4)
+    g = na.GraphGPU((off, col, w), 4)
+    g.set_tuning(fixed_short=fixed_short)
+    st = na.LouvainGPU(500, 1e-6, 4).run(g, record=True)
+    assert st.phases == len(ref["phases"]) and st.q == ref["q"]
+    for i, p in enumerate(ref["phases"]):
+        assert np.array_equal(g.phase_communities(i), p["comm"]), f"phase {i}"
+    assert np.array_equal(g.clustering(), ref["clustering"])
+    g.close()
 
 
 def test_repeat_runs_are_bit_identical():
