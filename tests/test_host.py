@@ -36,7 +36,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version():
-    assert na.lib.nido_abi_version() == na.ABI_VERSION == 2
+    assert na.lib.nido_abi_version() == na.ABI_VERSION == 3
 
 
 def write_reference_file(path, off, col, w, base=0, ne_header=None):
