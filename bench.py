@@ -20,7 +20,8 @@ around every class), its algorithmic bytes (24 B/arc + 40 B/vertex for local
 moving, 24 B/arc + 24 B/vertex for modularity) over the summed HIP-event
 durations of its launches in the timed steps, recorded on the stream they run
 on (only that class is bracketed while timing), against the 8 TB/s HBM peak;
-roofline_iter applies 48 B/arc + 64 B/vertex to the whole iteration.
+roofline_iter applies 48 B/arc + 64 B/vertex to the whole iteration (and,
+beside it, the local-moving bytes alone, 24 B/arc + 40 B/vertex).
 cpu_baseline: the oracle (C + OpenMP) timed on phase-0 iterations of the same
 graph on this host, bounded to ~20 s.
 
@@ -436,6 +437,7 @@ def main():
     d = ks[dom]
     ach = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
     b_iter = 48.0 * ne + 64.0 * nv
+    b_lm = 24.0 * ne + 40.0 * nv
     ach_iter = b_iter * args.steps / dt / 1e9
 
     out = {
@@ -468,6 +470,10 @@ def main():
         "roofline_iter": {
             "bytes_per_iter": b_iter, "achieved": round(ach_iter, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach_iter / HBM_PEAK_GBS, 4),
+            # B_iter charges a full modularity sweep (24 B/arc + 24 B/vertex)
+            # that the exact incremental update skips after a phase's first
+            # iteration; on the local-moving bytes alone:
+            "local_moving_bytes_per_iter": b_lm, "local_moving_frac": round(b_lm * args.steps / dt / 1e9 / HBM_PEAK_GBS, 4),
         },
         "kernel_ms_per_step_warmup_probe": {k: round(v["ms"] / max(1, min(args.warmup, 2)), 4)
                                             for k, v in probe.items()},

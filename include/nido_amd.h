@@ -188,6 +188,19 @@ int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* 
 int nido_graph_create(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,
                       const double* weights, int nbatches, void* stream, nido_graph** out);
 /* GraphGPU::~GraphGPU, graph_gpu.cpp:187-221. */
+/* GraphGPU with out-of-core edge windows (graph_gpu.cpp:341-417, 967-1027):
+ * the device holds at most window_arcs arcs of the graph at a time; a phase
+ * with more arcs keeps them in pinned host memory, cuts every batch into
+ * units of consecutive rows whose arcs fit the window, copies each unit's
+ * arcs in before its local-moving launches and sweeps the modularity window
+ * by window, and its coarse graph is built on the host; a phase whose arcs
+ * fit runs resident.  Results are those of nido_graph_create.  A row longer
+ * than the window is refused (NIDO_ERR_CAPACITY) when its phase starts.
+ * nido_graph_create picks such a window by itself, sized from the free
+ * device memory, when a graph would not fit resident. */
+int nido_graph_create_windowed(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,
+                               const double* weights, int nbatches, int64_t window_arcs, void* stream,
+                               nido_graph** out);
 int nido_graph_destroy(nido_graph* g);
 /* NV_, NE_, maxOrder_, mass_ of the current phase graph. */
 int nido_graph_info(const nido_graph* g, int64_t* nv, int64_t* ne, int64_t* max_order,

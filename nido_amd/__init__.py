@@ -100,6 +100,8 @@ _sig("nido_hgraph_coloring", C.c_int, C.c_void_p, C.c_int, _pi64, _pi64, _pi64)
 _sig("nido_dump_partition_orders", C.c_int, C.c_void_p, C.c_char_p, _pi64)
 _sig("nido_graph_create", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.c_int, C.c_void_p,
      C.POINTER(C.c_void_p))
+_sig("nido_graph_create_windowed", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.c_int, _i64, C.c_void_p,
+     C.POINTER(C.c_void_p))
 _sig("nido_graph_destroy", C.c_int, C.c_void_p)
 _sig("nido_graph_info", C.c_int, C.c_void_p, _pi64, _pi64, _pi64, _pdbl)
 _sig("nido_graph_batch_bounds", C.c_int, C.c_void_p, _pi64)
@@ -227,6 +229,16 @@ def batch_bounds_host(off: np.ndarray, nbatches: int) -> np.ndarray:
     return np.asarray(parts[: nbatches + 1], dtype=np.int64)
 
 
+class _View:
+    """A numpy array interface over memory `owner` holds; the array made from
+    it keeps `owner` alive."""
+
+    def __init__(self, owner, ptr, n: int, typestr: str):
+        self._owner = owner
+        addr = C.cast(ptr, C.c_void_p).value
+        self.__array_interface__ = {"shape": (n,), "typestr": typestr, "data": (addr, False), "version": 3}
+
+
 class Graph:
     """Host CSR graph (the reference's Graph, graph.hpp:14-102)."""
 
@@ -265,14 +277,15 @@ class Graph:
         return ne.value
 
     def arrays(self, copy: bool = True):
-        """(offsets, cols, weights) as numpy arrays."""
+        """(offsets, cols, weights) as numpy arrays; with copy=False, views of
+        the graph's own memory that keep the graph alive."""
         nv, ne = self.num_vertices, self.num_edges
         po, pc, pw = _pi64(), _pi64(), _pdbl()
         _check(lib.nido_hgraph_arrays(self._h, C.byref(po), C.byref(pc), C.byref(pw)),
                "nido_hgraph_arrays")
-        off = np.ctypeslib.as_array(po, shape=(nv + 1,))
-        col = np.ctypeslib.as_array(pc, shape=(ne,)) if ne else np.zeros(0, np.int64)
-        w = np.ctypeslib.as_array(pw, shape=(ne,)) if ne else np.zeros(0, np.float64)
+        off = np.asarray(_View(self, po, nv + 1, "<i8"))
+        col = np.asarray(_View(self, pc, ne, "<i8")) if ne else np.zeros(0, np.int64)
+        w = np.asarray(_View(self, pw, ne, "<f8")) if ne else np.zeros(0, np.float64)
         if copy:
             return off.copy(), col.copy(), w.copy()
         return off, col, w
@@ -335,7 +348,9 @@ class GraphGPU:
     """Device-resident Louvain state (the reference's GraphGPU,
     graph_gpu.hpp:168-247), one graph in HBM."""
 
-    def __init__(self, graph, nbatches: int = DEFAULT_BATCHES, stream: int | None = None):
+    def __init__(self, graph, nbatches: int = DEFAULT_BATCHES, stream: int | None = None, window: int = 0):
+        """window > 0: the device holds at most `window` arcs at a time
+        (nido_graph_create_windowed); 0: nido_graph_create."""
         if isinstance(graph, Graph):
             off, col, w = graph.arrays(copy=False)
         else:
@@ -345,9 +360,13 @@ class GraphGPU:
             w = np.ascontiguousarray(w, dtype=np.float64)
         self._keep = (graph,)
         h = C.c_void_p()
-        _check(lib.nido_graph_create(len(off) - 1, len(col), _p64(off), _p64(col), _pd(w), nbatches,
-                                     C.c_void_p(stream) if stream else None, C.byref(h)),
-               "nido_graph_create")
+        st = C.c_void_p(stream) if stream else None
+        if window:
+            _check(lib.nido_graph_create_windowed(len(off) - 1, len(col), _p64(off), _p64(col), _pd(w), nbatches,
+                                                  window, st, C.byref(h)), "nido_graph_create_windowed")
+        else:
+            _check(lib.nido_graph_create(len(off) - 1, len(col), _p64(off), _p64(col), _pd(w), nbatches, st,
+                                         C.byref(h)), "nido_graph_create")
         self._h = h
         self.nbatches = nbatches
         self.n0 = len(off) - 1

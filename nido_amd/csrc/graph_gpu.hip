@@ -343,9 +343,29 @@ struct nido_graph {
     int64_t iters_since_singleton = 0;   // the first iteration moves most vertices: full sweep there
     ModDelta delta() const { return ModDelta{d_dcounts, d_dchunks, d_dcomms, d_touched, d_term, d_iacc}; }
     std::vector<BatchResult> bres;       // per graph, from the last batched run
+    // ---- arcs streamed through a device window (graph_gpu.cpp:341-417, 967-1027) ----
+    int64_t window = 0;                  // arcs the device holds at once; 0: every phase resident
+    bool windowed = false;               // this phase has more arcs than the window
+    int* h_col32 = nullptr;              // the phase's CSR on the host (pinned) while windowed
+    double* h_w = nullptr;
+    int64_t h_cap = 0;
+    std::vector<int64_t> units;          // window units: consecutive rows of one batch
+    std::vector<int> first_unit;         // batch b: units first_unit[b] .. first_unit[b + 1]
+    int64_t win_e0 = -1, win_e1 = -1;    // arcs [win_e0, win_e1) sit in the window buffers
 };
 
 namespace {
+
+template <typename T>
+int d2h(nido_graph* g, T* dst, const T* src, int64_t n);
+
+// Device bytes of a resident graph: per vertex (state, bins, scratch) and per
+// arc at the peak of a phase (CSR, 32-bit mirrors, fixed-point weights, the
+// coarsening's hash tables and the coarse CSR being built); per arc of a
+// window (its CSR slices and long-row buckets).
+constexpr double kDeviceBytesPerVertex = 256.0;
+constexpr double kDeviceBytesPerArc = 96.0;
+constexpr double kWindowBytesPerArc = 48.0;
 
 void default_tuning(nido_tuning& t)
 {
@@ -396,9 +416,19 @@ int setup_phase(nido_graph* g)
     const int G = g->G;
     if (G == 1) batch_bounds(off, nv, g->nb, g->parts);
     else batch_bounds_all(off.data(), G, g->vb.data(), g->nb, g->gparts);
+    // A phase with more arcs than the device window streams them: every
+    // batch is cut into units of rows whose arcs fit the window, and each
+    // unit is one local-moving launch pair over the arcs copied in for it.
+    g->windowed = G == 1 && g->window > 0 && g->ne > g->window;
+    g->win_e0 = g->win_e1 = -1;
+    if (g->windowed) NIDO_TRY(window_units(off.data(), g->parts, g->nb, g->window, g->units, g->first_unit));
+    // launch units: the batches, or their window units (a handle of several
+    // graphs keeps its batch bounds per graph, in gparts)
+    const std::vector<int64_t>& lparts = g->windowed ? g->units : g->parts;
+    const int nl = g->windowed ? (int)g->units.size() - 1 : g->nb;
 
     PhaseBins pb;
-    build_phase_bins(off.data(), nv, g->nb, G, g->parts, g->gparts, g->tuning, pb);
+    build_phase_bins(off.data(), nv, nl, G, lparts, g->gparts, g->tuning, pb);
     const auto& binv = pb.binv;
     const auto& bitems = pb.bitems;
     const auto& chunks = pb.chunks;
@@ -455,7 +485,7 @@ int setup_phase(nido_graph* g)
         launch_fill_i32(g->d_cnt, 0, slot_need, g->stream);   // the kernels keep them zero afterwards
         g->slot_cap = slot_need;
     }
-    if (G == 1) {
+    if (G == 1 && !g->windowed) {
         // incremental modularity: room for every row chunk of the largest batch
         int64_t need = 0;
         for (int b = 0; b < g->nb; ++b) {
@@ -476,9 +506,9 @@ int setup_phase(nido_graph* g)
         HIP_TRY(hipStreamSynchronize(g->stream));
     }
     const size_t nbig_items = bitems.size();
-    g->small_bins.assign((size_t)g->nb, SmallBins{});
-    g->big_args.assign((size_t)g->nb, BigArgs{});
-    for (int b = 0; b < g->nb; ++b) {
+    g->small_bins.assign((size_t)nl, SmallBins{});
+    g->big_args.assign((size_t)nl, BigArgs{});
+    for (int b = 0; b < nl; ++b) {
         SmallBins& L = g->small_bins[(size_t)b];
         for (int k = 0; k < NUM_SMALL_BINS; ++k) {
             L.verts[k] = g->d_binv + small_start[(size_t)b * NUM_SMALL_BINS + k];
@@ -489,7 +519,7 @@ int setup_phase(nido_graph* g)
         L.bkey = g->d_bkey;
         L.bw = g->d_bw;
         L.cpo = g->d_cpo;
-        if (G == 1 && b > 0) {   // the previous batch's modularity delta rides in this launch
+        if (G == 1 && b > 0 && !g->windowed) {   // the previous batch's modularity delta rides in this launch
             L.dblocks = kDeltaBlocks;
             L.db = b - 1;
             L.dv0 = g->parts[(size_t)b - 1];
@@ -500,7 +530,7 @@ int setup_phase(nido_graph* g)
         small_bins_blocks(L, 4096);
         BigArgs& B = g->big_args[(size_t)b];
         const int64_t i0 = item_base[(size_t)b];
-        const int64_t i1 = b + 1 < g->nb ? item_base[(size_t)b + 1] : (int64_t)nbig_items;
+        const int64_t i1 = b + 1 < nl ? item_base[(size_t)b + 1] : (int64_t)nbig_items;
         B.items = g->d_big_items + i0;
         B.nitems = i1 - i0;
         B.bkey = g->d_bkey;
@@ -518,25 +548,40 @@ int setup_phase(nido_graph* g)
         B.tval = g->d_tval;
     }
 
-    std::vector<int> small;
-    std::vector<RowItem> items;
-    build_row_lists(off.data(), nv, small, items);
-    g->nsmall = (int64_t)small.size();
-    g->nitems = (int64_t)items.size();
-    NIDO_TRY(upload(&g->d_small, small, g->stream));
-    NIDO_TRY(upload(&g->d_items, items, g->stream));
-
-    launch_row_ids(g->d_off, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_src, g->stream);
-    launch_narrow_ids(g->d_col, g->d_col32, g->ne, g->stream);
-    unsigned long long* d_maxk = reinterpret_cast<unsigned long long*>(g->d_tot + 1);
-    launch_vertex_weights(g->d_off, g->d_w, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_k,
-                          g->d_kd, nv, g->d_acc + 2, d_maxk, g->d_part, g->stream);
-    HIP_TRY(hipGetLastError());
     fx_t twom = 0;
     unsigned long long maxk_bits = 0;
-    HIP_TRY(hipMemcpyAsync(&twom, g->d_acc + 2, sizeof(fx_t), hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipMemcpyAsync(&maxk_bits, d_maxk, sizeof(maxk_bits), hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (g->windowed) {
+        // the arcs are on the host: k_v, 2m and max |k_v| from there
+        std::vector<fx_t> hk((size_t)nv);
+        std::vector<double> hkd((size_t)nv);
+        double maxk_d = 0;
+        host_vertex_weights(nv, off.data(), g->h_w, hk.data(), hkd.data(), &twom, &maxk_d);
+        maxk_bits = nido_dbits(maxk_d);
+        if (nv > 0) {
+            HIP_TRY(hipMemcpyAsync(g->d_k, hk.data(), sizeof(fx_t) * (size_t)nv, hipMemcpyHostToDevice, g->stream));
+            HIP_TRY(hipMemcpyAsync(g->d_kd, hkd.data(), sizeof(double) * (size_t)nv, hipMemcpyHostToDevice,
+                                   g->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(g->stream));   // hk / hkd go out of scope
+    } else {
+        std::vector<int> small;
+        std::vector<RowItem> items;
+        build_row_lists(off.data(), nv, small, items);
+        g->nsmall = (int64_t)small.size();
+        g->nitems = (int64_t)items.size();
+        NIDO_TRY(upload(&g->d_small, small, g->stream));
+        NIDO_TRY(upload(&g->d_items, items, g->stream));
+
+        launch_row_ids(g->d_off, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_src, g->stream);
+        launch_narrow_ids(g->d_col, g->d_col32, g->ne, g->stream);
+        unsigned long long* d_maxk = reinterpret_cast<unsigned long long*>(g->d_tot + 1);
+        launch_vertex_weights(g->d_off, g->d_w, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_k,
+                              g->d_kd, nv, g->d_acc + 2, d_maxk, g->d_part, g->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&twom, g->d_acc + 2, sizeof(fx_t), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(&maxk_bits, d_maxk, sizeof(maxk_bits), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
     g->twom = twom;
     g->twom_d = fx_to_double(twom);
     g->m = g->twom_d * 0.5;                     // mass_ = 0.5 * sum(k), graph_cuda.cu:1855
@@ -552,7 +597,7 @@ int setup_phase(nido_graph* g)
     // units of 2^-F: the sums are then the same exact values as in Q64.64
     // (scaled by 2^(64 - F)), one word and no carry per LDS update.
     g->fixed = false;
-    if (g->tuning.fixed_short && g->ne > 0 && g->twom != 0) {
+    if (g->tuning.fixed_short && g->ne > 0 && g->twom != 0 && !g->windowed) {
         int* d_fneed = reinterpret_cast<int*>(g->d_tot);
         unsigned long long* d_wmax = reinterpret_cast<unsigned long long*>(g->d_tot + 1);
         launch_weight_grid(g->d_w, g->ne, d_fneed, d_wmax, g->stream);
@@ -632,6 +677,24 @@ int do_singleton(nido_graph* g)
     return NIDO_OK;
 }
 
+int window_load(nido_graph* g, int64_t v0, int64_t v1, int64_t* e0);
+
+// Q of a windowed phase into d_q: the internal weight window by window (the
+// arcs of each unit copied in), then the square terms.
+int windowed_modularity(nido_graph* g)
+{
+    launch_fill_fx(g->d_acc, 2, g->stream);
+    for (size_t u = 0; u + 1 < g->units.size(); ++u) {
+        int64_t e0 = 0;
+        NIDO_TRY(window_load(g, g->units[u], g->units[u + 1], &e0));
+        launch_modularity_rows(g->d_off, g->d_col32 - e0, g->d_w - e0, g->d_C32, g->units[u], g->units[u + 1],
+                               g->d_acc, g->stream);
+    }
+    launch_modularity_terms(g->d_a, g->nv, g->twom_d, g->d_acc, g->d_q, g->stream);
+    HIP_TRY(hipGetLastError());
+    return NIDO_OK;
+}
+
 int do_modularity(nido_graph* g, double* q)
 {
     if (g->twom == 0) {   // no weight at all: nothing is internal, nothing moves
@@ -640,11 +703,14 @@ int do_modularity(nido_graph* g, double* q)
     }
     {
         ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
-        launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d,
-                          g->d_part, g->d_q, g->stream, g->G == 1 ? g->d_term : nullptr,
-                          g->G == 1 ? g->d_iacc : nullptr, g->ident);
+        if (g->windowed)
+            NIDO_TRY(windowed_modularity(g));
+        else
+            launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d,
+                              g->d_part, g->d_q, g->stream, g->G == 1 ? g->d_term : nullptr,
+                              g->G == 1 ? g->d_iacc : nullptr, g->ident);
     }
-    g->inc_ok = g->G == 1;
+    g->inc_ok = g->G == 1 && !g->windowed;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(g->h_q, g->d_q, sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
@@ -654,6 +720,27 @@ int do_modularity(nido_graph* g, double* q)
 }
 
 bool replaying(const nido_graph* g);
+int lm_launch(nido_graph* g, const LmArgs& A, int u);
+
+// Copies the arcs of rows [v0, v1) of a windowed phase into the window
+// buffers (unless they are there already); *e0 = their first CSR index.
+int window_load(nido_graph* g, int64_t v0, int64_t v1, int64_t* e0)
+{
+    const int64_t a = g->h_off[(size_t)v0], z = g->h_off[(size_t)v1];
+    *e0 = a;
+    if (a >= g->win_e0 && z <= g->win_e1) return NIDO_OK;
+    if (z - a > g->window) return set_error(NIDO_ERR_CAPACITY, "window of %lld arcs too small for %lld",
+                                            (long long)g->window, (long long)(z - a));
+    if (z > a) {
+        HIP_TRY(hipMemcpyAsync(g->d_col32, g->h_col32 + a, sizeof(int) * (size_t)(z - a), hipMemcpyHostToDevice,
+                               g->stream));
+        HIP_TRY(hipMemcpyAsync(g->d_w, g->h_w + a, sizeof(double) * (size_t)(z - a), hipMemcpyHostToDevice,
+                               g->stream));
+    }
+    g->win_e0 = a;
+    g->win_e1 = z;
+    return NIDO_OK;
+}
 
 int do_lm(nido_graph* g, int b)
 {
@@ -678,6 +765,28 @@ int do_lm(nido_graph* g, int b)
         A.mg = g->d_mg;
         A.act = g->d_act;
     }
+    if (g->windowed) {
+        // each unit of the batch: its arcs copied into the window, then its
+        // launches, with the arc arrays based so that CSR indices address
+        // the window (the unit's rows use arcs [E0, E1) only)
+        for (int u = g->first_unit[(size_t)b]; u < g->first_unit[(size_t)b + 1]; ++u) {
+            int64_t e0 = 0;
+            NIDO_TRY(window_load(g, g->units[(size_t)u], g->units[(size_t)u + 1], &e0));
+            LmArgs Aw = A;
+            Aw.col = g->d_col32 - e0;
+            Aw.w = g->d_w - e0;
+            NIDO_TRY(lm_launch(g, Aw, u));
+        }
+        return NIDO_OK;
+    }
+    return lm_launch(g, A, b);
+}
+
+// The local-moving launches of launch unit u (a batch, or one window unit
+// of a batch).
+int lm_launch(nido_graph* g, const LmArgs& A, int u)
+{
+    const int b = u;
     const SmallBins& L = g->small_bins[(size_t)b];
     const BigArgs& B = g->big_args[(size_t)b];
     const bool split = g->tuning.split_small != 0;
@@ -774,6 +883,8 @@ int iteration_launches(nido_graph* g, bool inc)
                               g->G, g->d_gacc, g->d_gq, g->stream, g->d_act);
     else if (g->twom != 0 && inc)
         launch_mod_finish(D, g->nb, g->twom_d, g->d_q, g->stream);
+    else if (g->twom != 0 && g->windowed)
+        NIDO_TRY(windowed_modularity(g));
     else if (g->twom != 0)
         launch_modularity(g->d_src, g->d_col32, g->d_w, g->d_C32, g->ne, g->d_a, g->nv, g->twom_d, g->d_part,
                           g->d_q, g->stream, g->d_term, g->d_iacc);
@@ -787,7 +898,8 @@ constexpr int64_t kGraphMinArcs = int64_t(1) << 28;
 bool replaying(const nido_graph* g)
 {
     const int ug = g->tuning.use_graph;
-    return !g->prof.on && (g->G > 1 || g->twom != 0) && (ug == 2 || (ug == 1 && g->ne >= kGraphMinArcs));
+    return !g->prof.on && !g->windowed && (g->G > 1 || g->twom != 0) &&
+           (ug == 2 || (ug == 1 && g->ne >= kGraphMinArcs));
 }
 
 void drop_iteration_graph(nido_graph* g)
@@ -816,10 +928,10 @@ int run_iteration_launches(nido_graph* g)
     // arc is cheaper than listing the moved rows; later iterations move a
     // few percent, and the exact incremental update takes over.
     const bool inc = g->G == 1 && g->twom != 0 && g->inc_ok && !g->tuning.full_modularity &&
-                     g->iters_since_singleton > 0;
+                     g->iters_since_singleton > 0 && !g->windowed;
     ++g->iters_since_singleton;
     NIDO_TRY(launch_or_replay(g, inc));
-    g->inc_ok = g->G == 1 && g->twom != 0;   // both forms leave term/acc describing the new partition
+    g->inc_ok = g->G == 1 && g->twom != 0 && !g->windowed;   // both forms leave term/acc describing the new partition
     return NIDO_OK;
 }
 
@@ -923,6 +1035,39 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
     return rc;
 }
 
+// The coarse graph of a windowed phase, built on the host from its host
+// CSR (as the reference builds every coarse graph, graph_gpu.cpp:1088-1304):
+// it stays on the host while it has more arcs than the window, otherwise it
+// moves into the device arrays.
+int coarsen_host(nido_graph* g, const int64_t* newid, int64_t K)
+{
+    const int64_t nv = g->nv;
+    std::vector<int64_t> hC((size_t)nv), hn((size_t)nv);
+    NIDO_TRY(d2h(g, hC.data(), g->d_C, nv));
+    NIDO_TRY(d2h(g, hn.data(), newid, nv));
+    std::vector<int64_t> noff;
+    std::vector<int> ncol;
+    std::vector<double> nw;
+    host_coarsen(nv, g->h_off.data(), g->h_col32, g->h_w, hC.data(), hn.data(), K, noff, ncol, nw);
+    const int64_t nne = noff[(size_t)K];
+    hipStream_t s = g->stream;
+    HIP_TRY(hipMemcpyAsync(g->d_off, noff.data(), sizeof(int64_t) * (size_t)(K + 1), hipMemcpyHostToDevice, s));
+    if (nne > g->window) {
+        std::copy(ncol.begin(), ncol.end(), g->h_col32);   // fits: a coarse graph has at most the arcs of the fine one
+        std::copy(nw.begin(), nw.end(), g->h_w);
+    } else if (nne > 0) {
+        HIP_TRY(hipMemcpyAsync(g->d_col32, ncol.data(), sizeof(int) * (size_t)nne, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(g->d_w, nw.data(), sizeof(double) * (size_t)nne, hipMemcpyHostToDevice, s));
+        launch_widen_ids(g->d_col32, g->d_col, nne, s);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(s));   // the host vectors go out of scope
+    g->nv = K;
+    g->ne = nne;
+    g->h_off.swap(noff);
+    return NIDO_OK;
+}
+
 // aggregation, graph_gpu.cpp:1382-1545, on the device.
 int do_aggregation(nido_graph* g, int* done)
 {
@@ -943,7 +1088,8 @@ int do_aggregation(nido_graph* g, int* done)
         return NIDO_OK;
     }
     *done = 0;
-    NIDO_TRY(coarsen(g, newid, K, nullptr, nullptr));
+    if (g->windowed) NIDO_TRY(coarsen_host(g, newid, K));
+    else NIDO_TRY(coarsen(g, newid, K, nullptr, nullptr));
     return setup_phase(g);
 }
 
@@ -1319,7 +1465,7 @@ int do_autotune(nido_graph* g, int iters, nido_tuning* out)
 }
 
 int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int nb,
-                void* stream, nido_graph** out, int G = 1, const int64_t* vptr = nullptr)
+                void* stream, nido_graph** out, int G = 1, const int64_t* vptr = nullptr, int64_t window = 0)
 {
     if (!out) return set_error(NIDO_ERR_INVALID, "null output handle");
     *out = nullptr;
@@ -1329,9 +1475,10 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     const int64_t base = off[0];
     const int64_t ne = off[nv] - base;
     bool nonneg = true;
+    std::vector<fx_t> twom_g(G > 1 ? (size_t)G : 0);
     {
         const int64_t whole[2] = {0, nv};
-        NIDO_TRY(check_csr(nv, off, col, w, G, G > 1 ? vptr : whole, &nonneg));
+        NIDO_TRY(check_csr(nv, off, col, w, G, G > 1 ? vptr : whole, &nonneg, G > 1 ? twom_g.data() : nullptr));
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -1364,15 +1511,32 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     g->ne = ne;
     g->h_off.resize((size_t)nv + 1);
     rebase_offsets(off, nv, g->h_off.data());
+    // The device arc arrays hold the whole graph, or (window > 0 and more
+    // arcs than that) one window; the arcs of a windowed graph stay in
+    // pinned host memory.
+    g->window = window;
+    const bool windowed0 = window > 0 && ne > window;
+    const int64_t cap = windowed0 ? window : ne;
     int rc;
-    if ((rc = dev_alloc(&g->d_off, nv + 1)) || (rc = dev_alloc(&g->d_col, ne)) ||
-        (rc = dev_alloc(&g->d_w, ne)) || (rc = dev_alloc(&g->d_C, nv)) ||
+    if (windowed0) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&g->h_col32), sizeof(int) * (size_t)ne) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&g->h_w), sizeof(double) * (size_t)ne) != hipSuccess)
+            return fail(set_error(NIDO_ERR_NOMEM, "pinned host allocation of %lld arcs failed", (long long)ne));
+        g->h_cap = ne;
+#pragma omp parallel for schedule(static) if (ne >= (int64_t(1) << 22))
+        for (int64_t e = 0; e < ne; ++e) {
+            g->h_col32[e] = (int)col[base + e];
+            g->h_w[e] = w[base + e];
+        }
+    }
+    if ((rc = dev_alloc(&g->d_off, nv + 1)) || (rc = dev_alloc(&g->d_col, cap)) ||
+        (rc = dev_alloc(&g->d_w, cap)) || (rc = dev_alloc(&g->d_C, nv)) ||
         (rc = dev_alloc(&g->d_Cn, nv)) || (rc = dev_alloc(&g->d_k, nv)) ||
         (rc = dev_alloc(&g->d_a, nv)) || (rc = dev_alloc(&g->d_kd, nv)) ||
         (rc = dev_alloc(&g->d_clus, nv)) || (rc = dev_alloc(&g->d_work, nv + 1)) ||
         (rc = dev_alloc(&g->d_acc, 4)) || (rc = dev_alloc(&g->d_q, 1)) ||
-        (rc = dev_alloc(&g->d_tot, 2)) || (rc = dev_alloc(&g->d_src, ne)) ||
-        (rc = dev_alloc(&g->d_col32, ne)) || (rc = dev_alloc(&g->d_C32, nv)) ||
+        (rc = dev_alloc(&g->d_tot, 2)) || (rc = dev_alloc(&g->d_src, cap)) ||
+        (rc = dev_alloc(&g->d_col32, cap)) || (rc = dev_alloc(&g->d_C32, nv)) ||
         (rc = dev_alloc(&g->d_dcomms, nv)) || (rc = dev_alloc(&g->d_touched, nv)) ||
         (rc = dev_alloc(&g->d_term, nv)) || (rc = dev_alloc(&g->d_iacc, 2)) ||
         (rc = dev_alloc(&g->d_part, 2 * (int64_t)kPartBlocks)))
@@ -1385,19 +1549,18 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     // Clustering::commIds_ starts as the identity (clustering.cpp:4-10)
     launch_iota_i64(g->d_clus, nv, s);
     if (staged_h2d(g->d_off, g->h_off.data(), sizeof(int64_t) * (size_t)(nv + 1), s) != hipSuccess ||
-        (ne > 0 && staged_h2d_ids(g->d_col32, col + base, ne, s) != hipSuccess) ||
-        (ne > 0 && staged_h2d(g->d_w, w + base, sizeof(double) * (size_t)ne, s) != hipSuccess))
+        (ne > 0 && !windowed0 && staged_h2d_ids(g->d_col32, col + base, ne, s) != hipSuccess) ||
+        (ne > 0 && !windowed0 && staged_h2d(g->d_w, w + base, sizeof(double) * (size_t)ne, s) != hipSuccess))
         return fail(set_error(NIDO_ERR_HIP, "upload failed"));
     // ids crossed the host link as int32 (all below nv < 2^31 - 1, checked)
-    launch_widen_ids(g->d_col32, g->d_col, ne, s);
+    if (!windowed0) launch_widen_ids(g->d_col32, g->d_col, ne, s);
     if (G > 1) {
         g->G = G;
         g->vb.assign(vptr, vptr + G + 1);
         g->vb0 = g->vb;
-        g->twom_g.assign((size_t)G, 0);
+        g->twom_g.swap(twom_g);                           // 2m per graph, exactly (graph_cuda.cu:1855)
         g->twom_dg.assign((size_t)G, 0.0);
         std::vector<double> mg(2 * (size_t)G), tw((size_t)G);
-        graph_twom(off, w, G, vptr, g->twom_g.data());   // 2m per graph, exactly (graph_cuda.cu:1855)
         for (int i = 0; i < G; ++i) {
             tw[(size_t)i] = g->twom_dg[(size_t)i] = fx_to_double(g->twom_g[(size_t)i]);
             mg[2 * (size_t)i] = tw[(size_t)i] * 0.5;
@@ -1440,6 +1603,8 @@ int d2h(nido_graph* g, T* dst, const T* src, int64_t n)
         body                                                                     \
     } catch (const std::bad_alloc&) {                                            \
         return set_error(NIDO_ERR_NOMEM, "host allocation failed");              \
+    } catch (const std::exception& e_) {                                         \
+        return set_error(NIDO_ERR_STATE, "internal error: %s", e_.what());       \
     }
 
 extern "C" {
@@ -1462,7 +1627,29 @@ int nido_graph_create(int64_t nv, int64_t ne, const int64_t* off, const int64_t*
                       const double* w, int nbatches, void* stream, nido_graph** out)
 {
     (void)ne;
-    API_GUARD(return create_impl(nv, off, col, w, nbatches, stream, out);)
+    // determine_optimal_edges_per_partition (graph_gpu.cpp:362-391): arcs
+    // that do not fit the free device memory stream through a window
+    int64_t window = 0;
+    if (off && nv >= 0) {
+        const int64_t arcs = off[nv] - off[0];
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const double avail = 0.85 * (double)free_b - kDeviceBytesPerVertex * (double)nv;
+            if (kDeviceBytesPerArc * (double)arcs > avail && avail > 0)
+                window = std::max<int64_t>((int64_t)(avail / kWindowBytesPerArc), 1);
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    API_GUARD(return create_impl(nv, off, col, w, nbatches, stream, out, 1, nullptr, window);)
+}
+
+int nido_graph_create_windowed(int64_t nv, int64_t ne, const int64_t* off, const int64_t* col, const double* w,
+                               int nbatches, int64_t window_arcs, void* stream, nido_graph** out)
+{
+    (void)ne;
+    if (window_arcs < 1) return set_error(NIDO_ERR_INVALID, "window_arcs must be >= 1");
+    API_GUARD(return create_impl(nv, off, col, w, nbatches, stream, out, 1, nullptr, window_arcs);)
 }
 
 int nido_graph_destroy(nido_graph* g)
@@ -1487,6 +1674,8 @@ int nido_graph_destroy(nido_graph* g)
     dev_free(g->d_small); dev_free(g->d_items);
     raw_free(g->d_scan_tmp);
     if (g->h_q) (void)hipHostFree(g->h_q);
+    if (g->h_col32) (void)hipHostFree(g->h_col32);
+    if (g->h_w) (void)hipHostFree(g->h_w);
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
     if (g->side) (void)hipStreamDestroy(g->side);
@@ -1627,6 +1816,12 @@ int nido_get_community_weights(nido_graph* g, double* a)
 int nido_get_graph(nido_graph* g, int64_t* off, int64_t* col, double* w)
 {
     if (!g) return set_error(NIDO_ERR_INVALID, "null graph");
+    if (g->windowed) {   // the arcs are on the host
+        if (off) std::copy(g->h_off.begin(), g->h_off.end(), off);
+        if (col) std::copy(g->h_col32, g->h_col32 + g->ne, col);
+        if (w) std::copy(g->h_w, g->h_w + g->ne, w);
+        return NIDO_OK;
+    }
     if (off) NIDO_TRY(d2h(g, off, g->d_off, g->nv + 1));
     if (col) NIDO_TRY(d2h(g, col, g->d_col, g->ne));
     if (w) NIDO_TRY(d2h(g, w, g->d_w, g->ne));

@@ -45,6 +45,22 @@ struct PhaseBins {
 void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::vector<int64_t>& parts,
                       const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb);
 
+// Arcs streamed through a device window of `window` arcs: every batch b
+// is cut into units of consecutive rows holding at most `window` arcs
+// (units[first_unit[b]] .. units[first_unit[b + 1]] are its bounds, nb + 1
+// entries in first_unit).  A row longer than the window is refused.
+int window_units(const int64_t* off, const std::vector<int64_t>& parts, int nb, int64_t window,
+                 std::vector<int64_t>& units, std::vector<int>& first_unit);
+// Exact vertex weights on the host: k (Q64.64), kd = RN(k), 2m and max |kd|.
+void host_vertex_weights(int64_t nv, const int64_t* off, const double* w, fx_t* k, double* kd, fx_t* twom,
+                         double* maxk);
+// The coarse graph of a phase on the host (graph_gpu.cpp:1088-1304): row
+// newid[C[v]] gathers the arcs of v, merged by newid[C[target]] with exact
+// sums, rows sorted by target.
+void host_coarsen(int64_t nv, const int64_t* off, const int* col, const double* w, const int64_t* C,
+                  const int64_t* newid, int64_t K, std::vector<int64_t>& noff, std::vector<int>& ncol,
+                  std::vector<double>& nw);
+
 // Rows of at most kSmallRow arcs (one lane each) and kItemArcs-arc slices
 // of the longer rows, both in ascending vertex order.
 void build_row_lists(const int64_t* off, int64_t nv, std::vector<int>& small, std::vector<RowItem>& items);

@@ -6,12 +6,11 @@
 
 namespace nido {
 // Offsets non-decreasing; every arc of graph g (vertices [vptr[g],
-// vptr[g+1])) points inside graph g; weights finite with |w| < 2^62.
-// off/col/w are indexed from off[0].  *nonneg: every weight >= 0.
+// vptr[g+1])) points inside graph g; weights finite with |w| < 2^62 and the
+// bounds on their totals.  off/col/w are indexed from off[0].  *nonneg:
+// every weight >= 0; twom (if not null, G entries): every graph's exact 2m.
 int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int G,
-              const int64_t* vptr, bool* nonneg);
-// Exact 2m of every graph: the Q64.64 sum of its arc weights.
-void graph_twom(const int64_t* off, const double* w, int G, const int64_t* vptr, fx_t* out);
+              const int64_t* vptr, bool* nonneg, fx_t* twom = nullptr);
 // The greedy colouring of Graph::coloring run by one thread (coloring.hip):
 // colors[i] = smallest colour not held by a neighbour j < i with
 // r[j] >= r[i]; off[nv + 1], col[off[nv]] (ids in [0, nv)), host arrays.

@@ -141,8 +141,8 @@ int nido_hgraph_from_csr(int64_t nv, int64_t ne, const int64_t* off, const int64
         g->col.assign(col + base, col + base + arcs);
         g->w.assign(w + base, w + base + arcs);
         *out = g;
-    } catch (const std::bad_alloc&) {
-        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {   // bad_alloc, or a size no vector can hold (length_error)
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed: %s", e.what());
     }
     return NIDO_OK;
 }
@@ -194,8 +194,8 @@ int nido_hgraph_read(const char* path, nido_hgraph** out)
             }
         }
         *out = g;
-    } catch (const std::bad_alloc&) {
-        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {   // bad_alloc, or a size no vector can hold (length_error)
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed: %s", e.what());
     }
     return NIDO_OK;
 }
@@ -288,8 +288,8 @@ int nido_gen_sbm(int64_t nv, int64_t block, double p_in, double p_out, int weigh
             std::vector<uint64_t>().swap(r);
         }
         *out = csr_from_sorted_pairs(nv, e, weighted, seed ^ 0xA5A5A5A5DEADBEEFull);
-    } catch (const std::bad_alloc&) {
-        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {   // bad_alloc, or a size no vector can hold (length_error)
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed: %s", e.what());
     }
     return NIDO_OK;
 }
@@ -326,8 +326,8 @@ int nido_gen_rmat(int scale, int64_t edge_factor, double a, double b, double c, 
         sort_unique(e);
         if (!e.empty() && e.back() == ~0ull) e.pop_back();
         *out = csr_from_sorted_pairs(nv, e, weighted, seed ^ 0xA5A5A5A5DEADBEEFull);
-    } catch (const std::bad_alloc&) {
-        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {   // bad_alloc, or a size no vector can hold (length_error)
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed: %s", e.what());
     }
     return NIDO_OK;
 }
@@ -406,8 +406,8 @@ int nido_gen_chung_lu(int64_t nv, double alpha, double dmin, double dmax, double
         sort_unique(e);
         if (!e.empty() && e.back() == ~0ull) e.pop_back();
         *out = csr_from_sorted_pairs(nv, e, weighted, seed ^ 0xA5A5A5A5DEADBEEFull);
-    } catch (const std::bad_alloc&) {
-        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {   // bad_alloc, or a size no vector can hold (length_error)
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed: %s", e.what());
     }
     return NIDO_OK;
 }
@@ -493,8 +493,8 @@ int nido_gen_ba(int64_t nv, int64_t m0, nido_hgraph** out)
             }
         }
         *out = g;
-    } catch (const std::bad_alloc&) {
-        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {   // bad_alloc, or a size no vector can hold (length_error)
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed: %s", e.what());
     }
     return NIDO_OK;
 }
@@ -585,8 +585,8 @@ int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* 
         g->col.swap(ncol);
         g->w.swap(nw);
         if (num_colors) *num_colors = K;
-    } catch (const std::bad_alloc&) {
-        return set_error(NIDO_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {   // bad_alloc, or a size no vector can hold (length_error)
+        return set_error(NIDO_ERR_NOMEM, "host allocation failed: %s", e.what());
     }
     return NIDO_OK;
 }
@@ -597,7 +597,7 @@ int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* 
 namespace nido {
 
 int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int G,
-              const int64_t* vptr, bool* nonneg)
+              const int64_t* vptr, bool* nonneg, fx_t* twom)
 {
     const int64_t base = off[0];
     int64_t bad_off = INT64_MAX;
@@ -687,40 +687,8 @@ int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* 
                              "graph %d: the total of |w| is 2^31 or more times |2m| (negative weights)", g);
     }
     if (nonneg) *nonneg = !neg;
+    if (twom) std::copy(gsum.begin(), gsum.end(), twom);   // the exact sum of a graph's weights is its 2m
     return NIDO_OK;
-}
-
-void graph_twom(const int64_t* off, const double* w, int G, const int64_t* vptr, fx_t* out)
-{
-    const int64_t base = off[0];
-    if (G >= 64) {   // many graphs: one thread per graph
-#pragma omp parallel for schedule(dynamic, 16)
-        for (int g = 0; g < G; ++g) {
-            fx_t s = 0;
-            for (int64_t e = off[vptr[g]] - base; e < off[vptr[g + 1]] - base; ++e) s += fx_from_double(w[base + e]);
-            out[g] = s;
-        }
-        return;
-    }
-    for (int g = 0; g < G; ++g) {
-        const int64_t e0 = off[vptr[g]] - base, e1 = off[vptr[g + 1]] - base;
-        uint64_t lo = 0;
-        int64_t hi = 0;
-        // exact: the 128-bit sum is accumulated as two 64-bit halves with carry
-#pragma omp parallel
-        {
-            fx_t s = 0;
-#pragma omp for schedule(static) nowait
-            for (int64_t e = e0; e < e1; ++e) s += fx_from_double(w[base + e]);
-#pragma omp critical
-            {
-                const fx_t t = fx_make(lo, hi) + s;
-                lo = fx_lo(t);
-                hi = fx_hi(t);
-            }
-        }
-        out[g] = fx_make(lo, hi);
-    }
 }
 
 }  // namespace nido

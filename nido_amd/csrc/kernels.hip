@@ -260,6 +260,46 @@ mod_finalize_kernel(const fx_t* __restrict__ part, int nblocks, double twom_d, d
     }
 }
 
+// Modularity of a phase whose arcs stream through a window: the internal
+// weight of rows [v0, v1) (col / w may be based so that the window's arcs
+// keep their CSR indices), one wavefront per row, added exactly into
+// acc[0]; then every community's square term into acc[1]; then Q.
+__global__ void __launch_bounds__(kBlock)
+mod_rows_kernel(const int64_t* __restrict__ off, const int* __restrict__ col, const double* __restrict__ w,
+                const int* __restrict__ C, int64_t v0, int64_t v1, fx_t* __restrict__ acc)
+{
+    __shared__ fx_t red[kBlock / 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+    fx_t I = 0;
+    for (int64_t v = v0 + wid; v < v1; v += nw) {
+        const int cv = C[v];
+        for (int64_t e = off[v] + lane; e < off[v + 1]; e += 64)
+            if (C[col[e]] == cv) I += fx_from_double(w[e]);   // self arcs are internal
+    }
+    I = block_sum_fx(I, red);
+    if (threadIdx.x == 0 && I != 0) acc_add(acc, I);
+}
+
+__global__ void __launch_bounds__(kBlock)
+mod_terms_kernel(const fx_t* __restrict__ a, int64_t nv, double twom_d, fx_t* __restrict__ acc)
+{
+    __shared__ fx_t red[kBlock / 64];
+    fx_t T = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nv; i += (int64_t)gridDim.x * kBlock) {
+        const double x = fx_to_double(a[i]) / twom_d;
+        T += fx_from_double(x * x);
+    }
+    T = block_sum_fx(T, red);
+    if (threadIdx.x == 0 && T != 0) acc_add(acc + 1, T);
+}
+
+__global__ void mod_acc_finish_kernel(const fx_t* __restrict__ acc, double twom_d, double* q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *q = fx_to_double(acc[0]) / twom_d - fx_to_double(acc[1]);
+}
+
 __global__ void __launch_bounds__(kBlock)
 mod_delta_kernel(int b, int64_t v0, int64_t v1, const int64_t* __restrict__ Cn, const int* __restrict__ C32,
                  const int* __restrict__ col, const double* __restrict__ w, const fx_t* __restrict__ a,
@@ -656,6 +696,18 @@ void launch_modularity(const int* src, const int* col, const double* w, const in
     if (ident) mod_kernel<true><<<g, kBlock, 0, s>>>(src, col, w, C, ne, a, nv, twom_d, part, term);
     else mod_kernel<false><<<g, kBlock, 0, s>>>(src, col, w, C, ne, a, nv, twom_d, part, term);
     mod_finalize_kernel<<<1, kBlock, 0, s>>>(part, (int)g, twom_d, q, acc);
+}
+
+void launch_modularity_rows(const int64_t* off, const int* col, const double* w, const int* C, int64_t v0,
+                            int64_t v1, fx_t* acc, hipStream_t s)
+{
+    if (v1 > v0) mod_rows_kernel<<<grid_for(v1 - v0, kBlock / 64, 8192), kBlock, 0, s>>>(off, col, w, C, v0, v1, acc);
+}
+
+void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc, double* q, hipStream_t s)
+{
+    if (nv > 0) mod_terms_kernel<<<grid_for(nv, kBlock, kPartBlocks), kBlock, 0, s>>>(a, nv, twom_d, acc);
+    mod_acc_finish_kernel<<<1, 64, 0, s>>>(acc, twom_d, q);
 }
 
 void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const int* C32, const int* col,

@@ -202,12 +202,13 @@ void nido_server::worker(int slot)
         try {
             if (grp.size() == 1) run_one(grp[0], streams[(size_t)slot]);
             else run_group(grp, streams[(size_t)slot]);
-        } catch (const std::bad_alloc&) {
-            // a host copy of the group or a clustering did not fit: the
+        } catch (const std::exception& e) {
+            // a host copy of the group or a clustering did not fit (or any
+            // other failure a worker thread must not let escape): the
             // requests fail, and their slot and budget share are freed below
             for (Request* r : grp) {
                 r->rc = NIDO_ERR_NOMEM;
-                r->err = "out of host memory";
+                r->err = std::string("out of host memory: ") + e.what();
                 std::vector<int64_t>().swap(r->clustering);
             }
         }

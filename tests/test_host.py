@@ -87,6 +87,14 @@ def test_reader_errors(tmp_path):
         na.Graph.read(str(p))
 
 
+def test_reader_refuses_a_header_no_vector_can_hold(tmp_path):
+    # a corrupted header asking for 2^62 vertices: an error, not an abort
+    p = tmp_path / "huge.bin"
+    p.write_bytes(np.array([1 << 62, 5], "<i8").tobytes())
+    with pytest.raises(na.NidoError):
+        na.Graph.read(str(p))
+
+
 def check_simple_symmetric(off, col, w):
     nv = len(off) - 1
     assert off[0] == 0 and np.all(np.diff(off) >= 0)
