@@ -146,31 +146,29 @@ __device__ __forceinline__ double slot_sum(const LmArgs& A, const unsigned long 
     else return fx_to_double(fx_make(lo[s], (int64_t)hi[s]));
 }
 
-// Scores the listed slots of one row's table with L lanes: a row of a
-// short-row bin has at most 2L arcs, hence at most 2L keys, so each lane
-// takes at most two, with both a_c gathers in flight.  The lane holding the
-// row's own community stores e_{v,own} (rounded) in *self_d, so only the
+// The keys a lane created in a short row's table: their slots and
+// communities, and a_c, loaded as soon as the key was new -- the lane that
+// creates a key scores it, so the load travels with the row's inserts
+// instead of after them.  A short-row lane handles at most two arcs.
+struct OwnKeys {
+    int s[2], c[2];
+    fx_t a[2];
+};
+
+// Scores a lane's own keys (after the table is complete); the lane holding
+// the row's own community stores e_{v,own} (rounded) in *self_d, so only the
 // argmax needs a cross-lane reduction.
-template <int L, bool Fixed>
-__device__ __forceinline__ void sweep_list(const LmArgs& A, Mass m, double ki, int own, const int* key,
-                                           const unsigned long long* lo, const unsigned long long* hi,
-                                           const int* klist, int nk, int lane, double* self_d, double& bg, int& bc)
+template <bool Fixed>
+__device__ __forceinline__ void sweep_own(const LmArgs& A, Mass m, double ki, int own, const OwnKeys& k,
+                                          const unsigned long long* lo, const unsigned long long* hi,
+                                          double* self_d, double& bg, int& bc)
 {
-    const int s1 = lane < nk ? klist[lane] : -1;
-    const int s2 = lane + L < nk ? klist[lane + L] : -1;
-    const int c1 = s1 >= 0 ? key[s1] : -1;
-    const int c2 = s2 >= 0 ? key[s2] : -1;
-    const fx_t a1 = A.a[c1 >= 0 ? c1 : 0];
-    const fx_t a2 = A.a[c2 >= 0 ? c2 : 0];
-    if (c1 >= 0) {
-        const double e = slot_sum<Fixed>(A, lo, hi, s1);
-        if (c1 == own) *self_d = e;
-        else consider_e(m, ki, c1, e, a1, bg, bc);
-    }
-    if (c2 >= 0) {
-        const double e = slot_sum<Fixed>(A, lo, hi, s2);
-        if (c2 == own) *self_d = e;
-        else consider_e(m, ki, c2, e, a2, bg, bc);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (k.s[j] < 0) continue;
+        const double e = slot_sum<Fixed>(A, lo, hi, k.s[j]);
+        if (k.c[j] == own) *self_d = e;
+        else consider_e(m, ki, k.c[j], e, k.a[j], bg, bc);
     }
 }
 
@@ -189,77 +187,74 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask)
                                           __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-// Insert (c, w) into an LDS open-addressing table of S slots (S a power of
-// 2, never more than half full by construction of the bins).
+// The slot of key c in an LDS open-addressing table of S slots (S a power
+// of 2, never more than half full by construction of the bins), creating
+// it if absent; *fresh tells whether this lane created it.
 template <int S>
-__device__ __forceinline__ void lds_insert(int* key, unsigned long long* lo, unsigned long long* hi,
-                                           int* nkeys, int* klist, int c, fx_t wf)
+__device__ __forceinline__ unsigned lds_claim(int* key, int c, bool* fresh)
 {
     unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(S));
+    *fresh = false;
     for (;;) {
         const int kk = key[h];
         if (kk == c) break;
         if (kk < 0) {
             const int prev = atomicCAS(&key[h], -1, c);
             if (prev < 0) {
-                klist[atomicAdd(nkeys, 1)] = (int)h;   // a new key: list its slot
+                *fresh = true;
                 break;
             }
             if (prev == c) break;
         }
         h = (h + 1) & (S - 1);
     }
-    fx_atomic_add(&lo[h], &hi[h], wf);
+    return h;
 }
 
-// The same with a 64-bit fixed-point value (no carry word).
-template <int S>
-__device__ __forceinline__ void lds_insert64(int* key, unsigned long long* lo, int* nkeys, int* klist, int c,
-                                             int64_t x)
+// Inserts arc (c, weight) of a short row: the weight joins slot c's sum
+// (Q64.64 in {lo, hi}, or Fixed: one 64-bit word in units of 2^-F); a new
+// key goes to the lane's own list j with its a_c load issued at once.
+template <int S, bool Fixed>
+__device__ __forceinline__ void insert_own(const LmArgs& A, int* key, unsigned long long* lo, unsigned long long* hi,
+                                           int c, double w, int64_t wq, OwnKeys& k, int j)
 {
-    unsigned h = ((unsigned)c * kHashMul) >> (32 - ilog2c(S));
-    for (;;) {
-        const int kk = key[h];
-        if (kk == c) break;
-        if (kk < 0) {
-            const int prev = atomicCAS(&key[h], -1, c);
-            if (prev < 0) {
-                klist[atomicAdd(nkeys, 1)] = (int)h;
-                break;
-            }
-            if (prev == c) break;
-        }
-        h = (h + 1) & (S - 1);
+    bool fresh;
+    const unsigned h = lds_claim<S>(key, c, &fresh);
+    if constexpr (Fixed) atomicAdd(&lo[h], (unsigned long long)wq);
+    else fx_atomic_add(&lo[h], &hi[h], fx_from_double(w));
+    if (fresh) {
+        k.s[j] = (int)h;
+        k.c[j] = c;
+        k.a[j] = A.a[c];
     }
-    atomicAdd(&lo[h], (unsigned long long)x);
 }
 
-// Scans row v with `lanes` lanes (lane index `lane`), two arcs per lane per
-// step, inserting every non-self arc into the S-slot table; the slots of new
-// keys are appended to klist (nkeys counts them).
+// Scans short row v with `lanes` lanes (lane index `lane`): the row has at
+// most 2 * lanes arcs, so each lane takes arcs e0 + lane and e0 + lane +
+// lanes, inserting every non-self arc into the S-slot table and keeping the
+// keys it creates in k.
 template <int S, bool Fixed>
 __device__ __forceinline__ void scan_row(const LmArgs& A, int v, int64_t e0, int64_t e1, int lane, int lanes, int* key,
-                                         unsigned long long* lo, unsigned long long* hi, int* nkeys, int* klist)
+                                         unsigned long long* lo, unsigned long long* hi, OwnKeys& k)
 {
-    for (int64_t e = e0 + lane; e < e1; e += 2 * (int64_t)lanes) {
-        const int64_t f = e + lanes;
-        const bool two = f < e1;
-        const int u1 = (int)A.col[e];
-        const int u2 = two ? (int)A.col[f] : v;
-        const int c1 = u1 != v ? (int)A.C[u1] : -1;     // self arcs weigh 0 (graph_cuda.cu:2312-2315)
-        const int c2 = u2 != v ? (int)A.C[u2] : -1;
-        if constexpr (Fixed) {
-            const int64_t w1 = A.wq[e];
-            const int64_t w2 = two ? A.wq[f] : 0;
-            if (c1 >= 0) lds_insert64<S>(key, lo, nkeys, klist, c1, w1);
-            if (c2 >= 0) lds_insert64<S>(key, lo, nkeys, klist, c2, w2);
-        } else {
-            const double w1 = A.w[e];
-            const double w2 = two ? A.w[f] : 0.0;
-            if (c1 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c1, fx_from_double(w1));
-            if (c2 >= 0) lds_insert<S>(key, lo, hi, nkeys, klist, c2, fx_from_double(w2));
-        }
+    k.s[0] = k.s[1] = -1;
+    const int64_t e = e0 + lane, f = e + lanes;
+    const bool one = e < e1, two = f < e1;
+    const int u1 = one ? (int)A.col[e] : v;
+    const int u2 = two ? (int)A.col[f] : v;
+    double w1 = 0.0, w2 = 0.0;
+    int64_t q1 = 0, q2 = 0;
+    if constexpr (Fixed) {
+        q1 = one ? A.wq[e] : 0;
+        q2 = two ? A.wq[f] : 0;
+    } else {
+        w1 = one ? A.w[e] : 0.0;
+        w2 = two ? A.w[f] : 0.0;
     }
+    const int c1 = u1 != v ? (int)A.C[u1] : -1;     // self arcs weigh 0 (graph_cuda.cu:2312-2315)
+    const int c2 = u2 != v ? (int)A.C[u2] : -1;
+    if (c1 >= 0) insert_own<S, Fixed>(A, key, lo, hi, c1, w1, q1, k, 0);
+    if (c2 >= 0) insert_own<S, Fixed>(A, key, lo, hi, c2, w2, q2, k, 1);
 }
 
 // ---- launch 1 bodies (blocks [bi, bi+nbk) of the fused launch) -------------
@@ -277,7 +272,6 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
         const fx_t kv = A.k[v];
         const int64_t e0 = r.e0;
         const int d = r.d;
-        const fx_t rest = A.a[own] - kv;
         using W = typename std::conditional<Fixed, int64_t, double>::type;
         int uu[kThreadMaxDeg];
         W wd[kThreadMaxDeg];
@@ -290,6 +284,9 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
         int cc[kThreadMaxDeg];
 #pragma unroll
         for (int j = 0; j < kThreadMaxDeg; ++j) cc[j] = uu[j] != v ? (int)A.C[uu[j]] : -1;
+        // a_own is loaded once the row's gathers are on their way: it waits
+        // for C[v], which would otherwise hold up the row's loads
+        const fx_t rest = A.a[own] - kv;
         double self_d = 0.0;
         double bg = -kMaxFloat;
         int bc = kNone;
@@ -327,9 +324,9 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
 // per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
 template <int G, int S, bool Multi, bool Fixed>
 __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
-                           char* lds, int* kcnt, int* klist_all, double* sself)
+                           char* lds, double* sself)
 {
-    static_assert(S == 4 * G, "a group's key list holds 2G slots");
+    static_assert(S == 4 * G, "a row of at most 2G arcs fills at most half the table");
     constexpr int NG = kBlock / G;
     int* skey = reinterpret_cast<int*>(lds);
     unsigned long long* slo = reinterpret_cast<unsigned long long*>(lds + NG * S * 4);
@@ -339,8 +336,6 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
     int* key = skey + grp * S;
     unsigned long long* lo = slo + grp * S;
     unsigned long long* hi = shi + grp * S;
-    int* const klist = klist_all + grp * 2 * G;
-    int* const nkeys = kcnt + grp;
     double* const self_d = sself + grp;
 
     for (int64_t base = (int64_t)bi * NG; base < n; base += (int64_t)nbk * NG) {
@@ -349,10 +344,7 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
             lo[s] = 0;
             if constexpr (!Fixed) hi[s] = 0;
         }
-        if (lane == 0) {
-            *nkeys = 0;
-            *self_d = 0.0;
-        }
+        if (lane == 0) *self_d = 0.0;
         const int64_t idx = base + grp;
         bool act = idx < n;
         const BinRow r = act ? verts[idx] : BinRow{0, 0, 0};
@@ -361,14 +353,17 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
         act = act && graph_mass<Multi>(A, v, m);
         const int own = act ? A.C[v] : 0;
         const double ki = act ? A.kd[v] : 0.0;
-        const fx_t rest = act ? A.a[own] - A.k[v] : fx_t(0);
         wave_lds_sync();
-        if (act) scan_row<S, Fixed>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi, nkeys, klist);
+        OwnKeys k;
+        k.s[0] = k.s[1] = -1;
+        if (act) scan_row<S, Fixed>(A, v, r.e0, r.e0 + r.d, lane, G, key, lo, hi, k);
+        // a_own after the row's loads (it waits for C[v])
+        const fx_t rest = act ? A.a[own] - A.k[v] : fx_t(0);
         wave_lds_sync();
         if (act) {   // uniform within the group
             double bg = -kMaxFloat;
             int bc = kNone;
-            sweep_list<G, Fixed>(A, m, ki, own, key, lo, hi, klist, *nkeys, lane, self_d, bg, bc);
+            sweep_own<Fixed>(A, m, ki, own, k, lo, hi, self_d, bg, bc);
 #pragma unroll
             for (int msk = G / 2; msk > 0; msk >>= 1) {
                 const double og = __shfl_xor(bg, msk, G);
@@ -388,9 +383,9 @@ __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __rest
 // One workgroup per vertex, S-slot LDS table.
 template <int S, bool Multi, bool Fixed>
 __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
-                           char* lds, BlockRed& R, int* kcnt, int* klist, double* self_d)
+                           char* lds, BlockRed& R, double* self_d)
 {
-    static_assert(S == 4 * kBlock, "the key list holds 2 * kBlock slots");
+    static_assert(S == 4 * kBlock, "a row of at most 2 * kBlock arcs fills at most half the table");
     int* key = reinterpret_cast<int*>(lds);
     unsigned long long* lo = reinterpret_cast<unsigned long long*>(lds + S * 4);
     unsigned long long* hi = lo + S;
@@ -400,23 +395,21 @@ __device__ __forceinline__ void block_body(const LmArgs& A, const BinRow* __rest
             lo[s] = 0;
             if constexpr (!Fixed) hi[s] = 0;
         }
-        if (threadIdx.x == 0) {
-            *kcnt = 0;
-            *self_d = 0.0;
-        }
+        if (threadIdx.x == 0) *self_d = 0.0;
         const BinRow r = verts[idx];
         const int v = r.v;
         Mass m;
         if (!graph_mass<Multi>(A, v, m)) continue;   // uniform: the workgroup holds one vertex
         const int own = A.C[v];
         const double ki = A.kd[v];
-        const fx_t rest = A.a[own] - A.k[v];
         __syncthreads();
-        scan_row<S, Fixed>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi, kcnt, klist);
+        OwnKeys k;
+        scan_row<S, Fixed>(A, v, r.e0, r.e0 + r.d, threadIdx.x, kBlock, key, lo, hi, k);
+        const fx_t rest = A.a[own] - A.k[v];   // after the row's loads (it waits for C[v])
         __syncthreads();
         double bg = -kMaxFloat;
         int bc = kNone;
-        sweep_list<kBlock, Fixed>(A, m, ki, own, key, lo, hi, klist, *kcnt, threadIdx.x, self_d, bg, bc);
+        sweep_own<Fixed>(A, m, ki, own, k, lo, hi, self_d, bg, bc);
         block_argmax(R, bg, bc);         // its barriers also publish *self_d
         if (threadIdx.x == 0) A.Cn[v] = decide_d(m, ki, rest, own, *self_d, bg, bc);
     }
@@ -505,8 +498,6 @@ lm_small_kernel(LmArgs A, SmallBins L)
 {
     __shared__ __align__(16) char lds[kSmallLds];
     __shared__ BlockRed R;
-    __shared__ int klist[2 * kBlock];          // slots of the keys of the rows in flight
-    __shared__ int kcnt[kBlock / 8];           // keys per row (one counter per group)
     __shared__ double sself[kBlock / 8];       // e_{v,own} per row, rounded
     // Every index into L is a constant (a dynamically indexed kernel
     // argument would be copied to scratch memory).
@@ -519,11 +510,11 @@ lm_small_kernel(LmArgs A, SmallBins L)
 #define NIDO_BIN_ARGS(K) L.verts[K], L.n[K], b - L.blk0[K], L.blk0[K + 1] - L.blk0[K]
     switch (bin) {
     case BIN_THREAD: thread_body<Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_THREAD)); break;
-    case BIN_G8: group_body<8, 32, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G8), lds, kcnt, klist, sself); break;
-    case BIN_G16: group_body<16, 64, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G16), lds, kcnt, klist, sself); break;
-    case BIN_G32: group_body<32, 128, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G32), lds, kcnt, klist, sself); break;
-    case BIN_G64: group_body<64, 256, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G64), lds, kcnt, klist, sself); break;
-    case BIN_B256: block_body<1024, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_B256), lds, R, kcnt, klist, sself); break;
+    case BIN_G8: group_body<8, 32, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G8), lds, sself); break;
+    case BIN_G16: group_body<16, 64, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G16), lds, sself); break;
+    case BIN_G32: group_body<32, 128, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G32), lds, sself); break;
+    case BIN_G64: group_body<64, 256, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_G64), lds, sself); break;
+    case BIN_B256: block_body<1024, Multi, Fixed>(A, NIDO_BIN_ARGS(BIN_B256), lds, R, sself); break;
     case NUM_SMALL_BINS:
         keyfill_body<Multi>(A, L, b - L.blk0[NUM_SMALL_BINS], L.blk0[NUM_SMALL_BINS + 1] - L.blk0[NUM_SMALL_BINS],
                             lds);
