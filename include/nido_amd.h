@@ -82,7 +82,11 @@ typedef struct {
     int32_t split_small;      /* launch every short-row bin as a kernel of its own (0/1; profiling) */
     int32_t overlap_long;     /* run the long rows of a batch on a second stream, beside the short rows
                                  (0/1, default 1; only while iterations are replayed from a hipGraph) */
-    int32_t full_modularity;  /* 1: every iteration sweeps all arcs for Q; 0: exact incremental update (0/1) */
+    int32_t full_modularity;  /* how Q is evaluated after an iteration (the same Q either way): 0 (default)
+                                 the exact incremental update over the moved rows after each phase's first
+                                 iteration, for a handle of several graphs only when the iteration before
+                                 moved at most 1/32 of the arcs (else a sweep of every arc); 1 always the
+                                 sweep; 2 the update after each phase's first iteration (0/1/2) */
     int32_t fixed_short;      /* 1 (default): rows up to big_min_deg arcs sum in 64-bit fixed point when every
                                  weight is a multiple of one power of two 2^-F and those rows' sums stay below
                                  2^63 in units of 2^-F (the same exact sums); 0: always Q64.64 (0/1) */

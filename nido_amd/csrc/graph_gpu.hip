@@ -325,12 +325,15 @@ struct nido_graph {
     unsigned char* d_act = nullptr;      // per graph: moves in this iteration
     unsigned char* d_keep = nullptr;     // per graph: selected for a restore / a coarsening
     fx_t* d_gacc = nullptr;              // 2 accumulators per graph
-    double* d_gq = nullptr;              // modularity per graph
+    double* d_gq = nullptr;              // modularity per graph, then the iteration's moved arcs (bits)
+    unsigned long long* d_moved = nullptr;   // kMovedSlots counters of moved arcs
     SegItem* d_seg = nullptr;
     int64_t nseg = 0;
     std::vector<int64_t> blist0, blistn; // per batch: its slice of d_binv (non-isolated vertices)
     hipGraphExec_t iter_exec[2] = {};    // one iteration's launches (full / incremental Q), per phase
-    // incremental modularity (one graph)
+    // incremental modularity (per graph with G > 1)
+    int64_t* d_gp = nullptr;             // G * (nb+1): gparts on the device
+    int64_t gp_cap = 0;
     int* d_dcounts = nullptr;            // 2 per batch
     RowItem* d_dchunks = nullptr;
     int64_t dchunk_cap = 0;
@@ -338,10 +341,22 @@ struct nido_graph {
     int* d_touched = nullptr;            // capacity n0, kept zero between batches
     fx_t* d_term = nullptr;              // capacity n0
     fx_t* d_iacc = nullptr;              // 2
-    bool inc_ok = false;                 // d_term / d_iacc describe the current partition
+    bool inc_ok = false;                 // d_term and d_iacc (d_gacc) describe the current partition
     bool ident = false;                  // C is the singleton partition: set only by do_singleton
+    int64_t last_moved = -1;             // several graphs: arcs the last iteration moved; -1: unknown
+    bool ride_delta = false;             // short-row launches carry the previous batch's delta
     int64_t iters_since_singleton = 0;   // the first iteration moves most vertices: full sweep there
-    ModDelta delta() const { return ModDelta{d_dcounts, d_dchunks, d_dcomms, d_touched, d_term, d_iacc}; }
+    // several graphs: the iteration's moved-arc counters (their sum is read
+    // back with Q, after d_gq[G - 1]); a single graph does not count them
+    unsigned long long* moved() const { return G == 1 ? nullptr : d_moved; }
+    ModDelta delta() const
+    {
+        if (G == 1)
+            return ModDelta{d_dcounts, d_dchunks, d_dcomms, d_touched, d_term,   d_iacc,
+                            nullptr,   nullptr,   nullptr,  0,         moved()};
+        return ModDelta{d_dcounts, d_dchunks, d_dcomms, d_touched, d_term, d_gacc,
+                        d_gid,     d_gp,      d_twomg,  nb + 1,    moved()};
+    }
     std::vector<BatchResult> bres;       // per graph, from the last batched run
     // ---- arcs streamed through a device window (graph_gpu.cpp:341-417, 967-1027) ----
     int64_t window = 0;                  // arcs the device holds at once; 0: every phase resident
@@ -485,14 +500,36 @@ int setup_phase(nido_graph* g)
         launch_fill_i32(g->d_cnt, 0, slot_need, g->stream);   // the kernels keep them zero afterwards
         g->slot_cap = slot_need;
     }
-    if (G == 1 && !g->windowed) {
+    if (!g->windowed) {
         // incremental modularity: room for every row chunk of the largest batch
         int64_t need = 0;
-        for (int b = 0; b < g->nb; ++b) {
-            int64_t n = 0;
-            for (int64_t v = g->parts[(size_t)b]; v < g->parts[(size_t)b + 1]; ++v)
-                n += (off[v + 1] - off[v] + kDeltaArcs - 1) / kDeltaArcs;
-            need = std::max(need, n);
+        if (G == 1) {
+            for (int b = 0; b < g->nb; ++b) {
+                int64_t n = 0;
+                for (int64_t v = g->parts[(size_t)b]; v < g->parts[(size_t)b + 1]; ++v)
+                    n += (off[v + 1] - off[v] + kDeltaArcs - 1) / kDeltaArcs;
+                need = std::max(need, n);
+            }
+        } else {
+            // ceil(d / K) <= 1 + d / K: rows + arcs / K of the batch bound it
+            const size_t nb1 = (size_t)g->nb + 1;
+            for (int b = 0; b < g->nb; ++b) {
+                int64_t rows = 0, arcs = 0;
+                for (int i = 0; i < G; ++i) {
+                    const int64_t lo = g->gparts[(size_t)i * nb1 + (size_t)b];
+                    const int64_t hi = g->gparts[(size_t)i * nb1 + (size_t)b + 1];
+                    rows += hi - lo;
+                    arcs += off[hi] - off[lo];
+                }
+                need = std::max(need, rows + arcs / kDeltaArcs);
+            }
+            if ((int64_t)g->gparts.size() > g->gp_cap) {
+                dev_free(g->d_gp);
+                NIDO_TRY(dev_alloc(&g->d_gp, (int64_t)g->gparts.size()));
+                g->gp_cap = (int64_t)g->gparts.size();
+            }
+            HIP_TRY(hipMemcpyAsync(g->d_gp, g->gparts.data(), sizeof(int64_t) * g->gparts.size(),
+                                   hipMemcpyHostToDevice, g->stream));
         }
         if (need > g->dchunk_cap) {
             dev_free(g->d_dchunks);
@@ -519,11 +556,11 @@ int setup_phase(nido_graph* g)
         L.bkey = g->d_bkey;
         L.bw = g->d_bw;
         L.cpo = g->d_cpo;
-        if (G == 1 && b > 0 && !g->windowed) {   // the previous batch's modularity delta rides in this launch
+        if (b > 0 && !g->windowed) {   // the previous batch's modularity delta rides in this launch
             L.dblocks = kDeltaBlocks;
             L.db = b - 1;
-            L.dv0 = g->parts[(size_t)b - 1];
-            L.dv1 = g->parts[(size_t)b];
+            L.dv0 = G == 1 ? g->parts[(size_t)b - 1] : 0;
+            L.dv1 = G == 1 ? g->parts[(size_t)b] : 0;
             L.D = g->delta();
             L.Cn64 = g->d_Cn;
         }
@@ -670,6 +707,7 @@ int do_singleton(nido_graph* g)
     g->inc_ok = false;
     g->ident = false;
     g->iters_since_singleton = 0;
+    g->last_moved = -1;
     launch_singleton(g->d_C, g->d_Cn, g->d_C32, g->d_a, g->d_k, g->nv, g->stream);
     HIP_TRY(hipGetLastError());
     g->ident = true;
@@ -759,8 +797,9 @@ int do_lm(nido_graph* g, int b)
             return NIDO_OK;
         }
     } else {
-        // graphs without weight never take part (their act flag stays 0)
-        if (g->blistn[(size_t)b] == 0) return NIDO_OK;
+        // graphs without weight never take part (their act flag stays 0);
+        // an empty batch still carries the previous batch's modularity delta
+        if (g->blistn[(size_t)b] == 0 && g->small_bins[(size_t)b].dblocks == 0) return NIDO_OK;
         A.gid = g->d_gid;
         A.mg = g->d_mg;
         A.act = g->d_act;
@@ -806,14 +845,14 @@ int lm_launch(nido_graph* g, const LmArgs& A, int u)
         HIP_TRY(hipEventRecord(g->ev_join, g->side));
         {
             ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
-            launch_lm_small(A, L, true, false, split, g->stream);
+            launch_lm_small(A, L, true, false, split, g->stream, g->ride_delta);
         }
         HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_join, 0));
     } else {
         // launch 1 fills the long rows' buckets that launch 2 reads
         {
             ProfScope ps(g->prof, g->stream, NIDO_K_LM_SHORT, g->short_bytes[(size_t)b]);
-            launch_lm_small(A, L, true, true, split, g->stream);
+            launch_lm_small(A, L, true, true, split, g->stream, g->ride_delta);
         }
         if (B.nitems > 0) {
             ProfScope ps(g->prof, g->stream, NIDO_K_LM_LONG, g->long_bytes[(size_t)b]);
@@ -832,10 +871,10 @@ int do_commit(nido_graph* g, int b)
         ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
         if (g->G == 1)
             launch_commit(g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_C, g->d_Cn, g->d_C32, g->d_k,
-                          g->d_a, g->stream);
+                          g->d_a, g->stream, g->d_off, g->moved());
         else
             launch_commit_filtered(g->d_binv + g->blist0[(size_t)b], 0, g->blistn[(size_t)b], g->d_C, g->d_Cn,
-                                   g->d_C32, g->d_k, g->d_a, g->d_gid, g->d_act, g->stream);
+                                   g->d_C32, g->d_k, g->d_a, g->d_gid, g->d_act, g->stream, g->moved());
     }
     HIP_TRY(hipGetLastError());
     g->committed[(size_t)b] = 1;
@@ -861,26 +900,40 @@ int iteration_launches(nido_graph* g, bool inc)
 {
     g->ident = false;
     const ModDelta D = g->delta();
+    if (g->G > 1) launch_fill_i64(reinterpret_cast<int64_t*>(g->d_moved), 0, kMovedSlots, g->stream);
+    g->ride_delta = inc;
     for (int b = 0; b < g->nb; ++b) {
-        NIDO_TRY(do_lm(g, b));
+        const int rc = do_lm(g, b);
+        if (rc != NIDO_OK) {
+            g->ride_delta = false;
+            return rc;
+        }
         if (!inc) {
             NIDO_TRY(do_commit(g, b));
             continue;
         }
-        const int64_t v0 = g->parts[(size_t)b], v1 = g->parts[(size_t)b + 1];
         {
             ProfScope ps(g->prof, g->stream, NIDO_K_COMMIT, 0.0);
-            launch_commit_tracked(v0, v1, b, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->d_off, D, g->stream);
+            if (g->G == 1)
+                launch_commit_tracked(g->parts[(size_t)b], g->parts[(size_t)b + 1], b, g->d_C, g->d_Cn, g->d_C32,
+                                      g->d_k, g->d_a, g->d_off, D, g->stream);
+            else
+                launch_commit_tracked_list(g->d_binv + g->blist0[(size_t)b], g->blistn[(size_t)b], b, g->d_C,
+                                           g->d_Cn, g->d_C32, g->d_k, g->d_a, g->d_off, D, g->d_act, g->stream);
         }
         g->committed[(size_t)b] = 1;
     }
+    g->ride_delta = false;
     ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
     if (inc && g->nb > 0)   // the earlier batches' deltas rode in the next batch's short-row launch
-        launch_mod_delta(g->nb - 1, g->parts[(size_t)g->nb - 1], g->parts[(size_t)g->nb], g->d_Cn, g->d_C32,
-                         g->d_col32, g->d_w, g->d_a, g->twom_d, D, g->stream);
-    if (g->G > 1)   // the graphs that sat out this iteration keep their modularity
+        launch_mod_delta(g->nb - 1, g->G == 1 ? g->parts[(size_t)g->nb - 1] : 0,
+                         g->G == 1 ? g->parts[(size_t)g->nb] : 0, g->d_Cn, g->d_C32, g->d_col32, g->d_w, g->d_a,
+                         g->twom_d, D, g->stream);
+    if (g->G > 1 && inc)    // the graphs that sat out this iteration keep their modularity
+        launch_mod_finish_graphs(D, g->nb, g->G, g->d_gq, g->d_act, g->stream);
+    else if (g->G > 1)
         launch_modularity_seg(g->d_seg, g->nseg, g->d_src, g->d_col32, g->d_w, g->d_C32, g->d_a, g->d_twomg,
-                              g->G, g->d_gacc, g->d_gq, g->stream, g->d_act);
+                              g->G, g->d_gacc, g->d_gq, g->stream, g->d_act, g->d_term, g->d_moved);
     else if (g->twom != 0 && inc)
         launch_mod_finish(D, g->nb, g->twom_d, g->d_q, g->stream);
     else if (g->twom != 0 && g->windowed)
@@ -893,6 +946,7 @@ int iteration_launches(nido_graph* g, bool inc)
 }
 
 constexpr int64_t kGraphMinArcs = int64_t(1) << 28;
+constexpr int64_t kIncShare = 32;
 
 // Whether this phase's iterations are replayed from a hipGraph.
 bool replaying(const nido_graph* g)
@@ -923,15 +977,23 @@ int launch_or_replay(nido_graph* g, bool inc);
 // 0.9% faster; 2 always.
 int run_iteration_launches(nido_graph* g)
 {
-    // The first iteration from the singleton partition moves most vertices
-    // (their rows hold ~70% of the arcs on R-MAT), where one sweep of every
-    // arc is cheaper than listing the moved rows; later iterations move a
-    // few percent, and the exact incremental update takes over.
-    const bool inc = g->G == 1 && g->twom != 0 && g->inc_ok && !g->tuning.full_modularity &&
-                     g->iters_since_singleton > 0 && !g->windowed;
+    // Q by a sweep of every arc or by the exact incremental update over the
+    // rows that moved (the same Q).  The first iteration from the singleton
+    // partition moves most vertices (their rows hold ~70% of the arcs on
+    // R-MAT) and sweeps; with full_modularity 0 a single graph takes the
+    // update in every later iteration (its deltas ride beside the next
+    // batch's rows), and a handle of many graphs when the iteration before
+    // moved at most 1/kIncShare of the arcs: its batches are all short rows,
+    // so the deltas' dependent gathers set the length of those launches and
+    // cost several times the sweep's streaming per arc.  2: the update in
+    // every later iteration; 1: never.
+    const bool any_w = g->G > 1 || g->twom != 0;
+    const int fm = g->tuning.full_modularity;
+    bool inc = any_w && g->inc_ok && fm != 1 && g->iters_since_singleton > 0 && !g->windowed;
+    if (inc && fm == 0 && g->G > 1) inc = g->last_moved >= 0 && g->last_moved * kIncShare <= g->ne;
     ++g->iters_since_singleton;
     NIDO_TRY(launch_or_replay(g, inc));
-    g->inc_ok = g->G == 1 && g->twom != 0 && !g->windowed;   // both forms leave term/acc describing the new partition
+    g->inc_ok = any_w && !g->windowed;   // both forms leave term/acc describing the new partition
     return NIDO_OK;
 }
 
@@ -1248,9 +1310,10 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
             loops[(size_t)i] = 0;
             act[(size_t)i] = alive[(size_t)i] && g->twom_g[(size_t)i] != 0;
             if (alive[(size_t)i] && !act[(size_t)i]) {
-                // no weight: one pass with no moves and Q = 0 (the GraphGPU
-                // path short-circuits the same way)
-                loops[(size_t)i] = 1;
+                // no weight: passes with no moves and Q = 0, so dQ = 0 after
+                // the first; the loop (louvain_gpu.cpp:53) goes on to
+                // max_loops only when tol < 0
+                loops[(size_t)i] = max_loops <= 0 ? 0 : tol < 0.0 ? max_loops : 1;
                 Q[(size_t)i] = 0.0;
                 dQ[(size_t)i] = 0.0;
             }
@@ -1260,10 +1323,12 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
         while (any(act)) {
             NIDO_TRY(upload_flags(g, g->d_act, act));
             NIDO_TRY(run_iteration_launches(g));
-            qt.resize((size_t)G);
-            HIP_TRY(hipMemcpyAsync(qt.data(), g->d_gq, sizeof(double) * (size_t)G, hipMemcpyDeviceToHost, g->stream));
+            qt.resize((size_t)G + 1);
+            HIP_TRY(hipMemcpyAsync(qt.data(), g->d_gq, sizeof(double) * ((size_t)G + 1), hipMemcpyDeviceToHost,
+                                   g->stream));
             HIP_TRY(hipStreamSynchronize(g->stream));
             g->prof.harvest();
+            std::memcpy(&g->last_moved, &qt[(size_t)G], sizeof(int64_t));
             bool rollback = false;
             for (int i = 0; i < G; ++i) {
                 flag[(size_t)i] = 0;
@@ -1283,6 +1348,10 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
                 launch_commit_filtered(nullptr, 0, g->nv, g->d_C, g->d_Cn, g->d_C32, g->d_k, g->d_a, g->d_gid,
                                        g->d_keep, g->stream);
                 HIP_TRY(hipGetLastError());
+                // a graph that rolled back and goes on (tol < 0) needs its
+                // terms and sums again: the next iteration sweeps
+                for (int i = 0; i < G; ++i)
+                    if (flag[(size_t)i] && act[(size_t)i]) g->inc_ok = false;
             }
             ++iters;
         }
@@ -1393,6 +1462,16 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
 // value only when it is at least 2% faster.  Results never depend on the
 // knobs, so any choice is safe; the handle is left in the singleton
 // partition of its current phase.
+// Waits for the iteration and reads its moved-arc count (which picks the
+// next iteration's form of Q).
+bool sync_moved(nido_graph* g)
+{
+    if (g->G > 1 && hipMemcpyAsync(&g->last_moved, g->d_gq + g->G, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                   g->stream) != hipSuccess)
+        return false;
+    return hipStreamSynchronize(g->stream) == hipSuccess;
+}
+
 double tune_trial(nido_graph* g, const nido_tuning& t, int iters)
 {
     g->tuning = t;
@@ -1407,10 +1486,10 @@ double tune_trial(nido_graph* g, const nido_tuning& t, int iters)
     // launches; the capture of both graph forms when replaying), then the
     // timed ones continue the same trajectory.
     for (int i = 0; i < 2; ++i)
-        if (run_iteration_launches(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
+        if (run_iteration_launches(g) != NIDO_OK || !sync_moved(g)) return -1.0;
     const double t0 = now_s();
     for (int i = 0; i < iters; ++i)
-        if (run_iteration_launches(g) != NIDO_OK || hipStreamSynchronize(g->stream) != hipSuccess) return -1.0;
+        if (run_iteration_launches(g) != NIDO_OK || !sync_moved(g)) return -1.0;
     return now_s() - t0;
 }
 
@@ -1570,7 +1649,8 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
             (rc = dev_alloc(&g->d_vb, G + 1)) || (rc = dev_alloc(&g->d_vbn, G + 1)) ||
             (rc = dev_alloc(&g->d_mg, 2 * (int64_t)G)) || (rc = dev_alloc(&g->d_twomg, G)) || (rc = dev_alloc(&g->d_act, G)) ||
             (rc = dev_alloc(&g->d_keep, G)) || (rc = dev_alloc(&g->d_gacc, 2 * (int64_t)G)) ||
-            (rc = dev_alloc(&g->d_gq, G)))
+            (rc = dev_alloc(&g->d_gq, (int64_t)G + 1)) ||
+            (rc = dev_alloc(&g->d_moved, kMovedSlots)))
             return fail(rc);
         if (hipMemcpyAsync(g->d_vb, g->vb.data(), sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
             hipMemcpyAsync(g->d_mg, mg.data(), sizeof(double) * 2 * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1664,9 +1744,10 @@ int nido_graph_destroy(nido_graph* g)
     dev_free(g->d_src); dev_free(g->d_part); dev_free(g->d_col32); dev_free(g->d_C32); dev_free(g->d_wq);
     dev_free(g->d_gid); dev_free(g->d_gid0); dev_free(g->d_vb); dev_free(g->d_vbn); dev_free(g->d_mg);
     dev_free(g->d_twomg); dev_free(g->d_act); dev_free(g->d_keep); dev_free(g->d_gacc); dev_free(g->d_gq);
+    dev_free(g->d_moved);
     dev_free(g->d_seg);
     dev_free(g->d_dcounts); dev_free(g->d_dchunks); dev_free(g->d_dcomms); dev_free(g->d_touched);
-    dev_free(g->d_term); dev_free(g->d_iacc);
+    dev_free(g->d_term); dev_free(g->d_iacc); dev_free(g->d_gp);
     dev_free(g->d_binv); dev_free(g->d_big_items); dev_free(g->d_big_poff); dev_free(g->d_big_toff);
     dev_free(g->d_key_chunks); dev_free(g->d_bkey); dev_free(g->d_bw); dev_free(g->d_cpo);
     dev_free(g->d_pg); dev_free(g->d_pc); dev_free(g->d_pself); dev_free(g->d_cnt);
@@ -1717,7 +1798,8 @@ int nido_graph_set_tuning(nido_graph* g, const nido_tuning* t)
     if (t->use_graph < 0 || t->use_graph > 2) return set_error(NIDO_ERR_INVALID, "use_graph must be 0, 1 or 2");
     if ((t->split_small | 1) != 1) return set_error(NIDO_ERR_INVALID, "split_small must be 0 or 1");
     if ((t->overlap_long | 1) != 1) return set_error(NIDO_ERR_INVALID, "overlap_long must be 0 or 1");
-    if ((t->full_modularity | 1) != 1) return set_error(NIDO_ERR_INVALID, "full_modularity must be 0 or 1");
+    if (t->full_modularity < 0 || t->full_modularity > 2)
+        return set_error(NIDO_ERR_INVALID, "full_modularity must be 0, 1 or 2");
     if ((t->fixed_short | 1) != 1) return set_error(NIDO_ERR_INVALID, "fixed_short must be 0 or 1");
     if (!g) return set_error(NIDO_ERR_INVALID, "null handle");
     AllocScope scope(g->stream);

@@ -52,19 +52,26 @@ struct RowItem {
     int64_t e0, e1;
 };
 
-// Incremental modularity (one graph).  The commit of batch b lists the rows
-// of its moved vertices (in chunks of kDeltaArcs arcs) and the communities
-// whose total changed; the delta kernel then adds the exact change of the
-// internal weight and of the sum of square terms to acc, so Q after the last
-// batch equals a full sweep bit for bit.
+// Incremental modularity.  The commit of batch b lists the rows of its
+// moved vertices (in chunks of kDeltaArcs arcs) and the communities whose
+// total changed; the delta kernel then adds the exact change of the internal
+// weight and of the sum of square terms to acc, so Q after the last batch
+// equals a full sweep bit for bit.  A handle of several graphs (gid set)
+// keeps one accumulator pair per graph, acc[2g] and acc[2g + 1].
 constexpr int64_t kDeltaArcs = 1024;
+constexpr int kMovedSlots = 256;           // counters of moved arcs (== kBlock: one thread each)
 struct ModDelta {
     int* counts;             // per batch: [2b] row chunks, [2b+1] communities
     RowItem* chunks;         // row chunks of moved vertices (current batch)
     int* comms;              // communities touched (current batch)
     int* touched;            // per community: listed in the current batch
-    fx_t* term;              // per community: fx((a_c / 2m)^2)
-    fx_t* acc;               // [0] internal weight, [1] sum of square terms
+    fx_t* term;              // per community: fx((a_c / 2m)^2), 2m of its graph
+    fx_t* acc;               // [0] internal weight, [1] sum of square terms (per graph with gid)
+    const int* gid;          // several graphs: graph of every vertex; null for one graph
+    const int64_t* gp;       // several graphs: batch b of graph g is [gp[g nb1 + b], gp[g nb1 + b + 1])
+    const double* twomg;     // several graphs: 2m per graph
+    int nb1;                 // batches + 1
+    unsigned long long* moved;   // several graphs: kMovedSlots counters of the arcs of moved vertices
 };
 constexpr int kDeltaBlocks = 256;          // workgroups of a delta riding in a short-row launch
 
@@ -188,16 +195,21 @@ void launch_widen_ids(const int* col32, int64_t* col, int64_t n, hipStream_t s);
 // fused launch.
 int small_bins_blocks(SmallBins& L, unsigned cap_per_bin);
 // bins / fill: launch the short-row bins / the long-row key fill (both: one
-// fused launch); split: one launch per bin instead of one fused launch.
-void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s);
+// fused launch); split: one launch per bin instead of one fused launch;
+// delta: the previous batch's modularity delta rides with the bins.
+void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s,
+                     bool delta = true);
 void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s);
 
+// With moved set, the commits add the arcs of the vertices that moved to
+// the kMovedSlots counters at moved (off: the CSR offsets, for a null list).
 void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, const fx_t* k, fx_t* a,
-                   hipStream_t s);
+                   hipStream_t s, const int64_t* off = nullptr, unsigned long long* moved = nullptr);
 // The commit over list[0, n) (or [v0, v0 + n) with a null list), restricted
 // to the vertices v whose graph has flag[gid[v]] set.
 void launch_commit_filtered(const BinRow* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
-                            const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s);
+                            const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s,
+                            unsigned long long* moved = nullptr);
 
 // Modularity over the arc-parallel CSR (src = row ids): writes Q to *q;
 // part holds 2 * kPartBlocks partials.  With term/acc set it also leaves
@@ -218,10 +230,20 @@ void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc
 
 void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* Cn, int* C32, const fx_t* k,
                            fx_t* a, const int64_t* off, const ModDelta& D, hipStream_t s);
+// The same over the listed vertices list[0, n) of batch b of a handle of
+// several graphs, those of graphs whose flag is set (gid from D).
+void launch_commit_tracked_list(const BinRow* list, int64_t n, int b, int64_t* C, int64_t* Cn, int* C32,
+                                const fx_t* k, fx_t* a, const int64_t* off, const ModDelta& D,
+                                const unsigned char* flag, hipStream_t s);
+// Batch b's delta (v0, v1 are its bounds for one graph; unused with D.gid).
 void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const int* C32, const int* col,
                       const double* w, const fx_t* a, double twom_d, const ModDelta& D, hipStream_t s);
 // q = I / 2m - T from acc; zeroes the per-batch counts for the next iteration.
 void launch_mod_finish(const ModDelta& D, int nb, double twom_d, double* q, hipStream_t s);
+// The same per graph (D.gid set): q[g] for the G graphs whose act flag is
+// set, and the sum of the D.moved counters in the bits of q[G].
+void launch_mod_finish_graphs(const ModDelta& D, int nb, int G, double* q, const unsigned char* act,
+                              hipStream_t s);
 
 // ---- several graphs in one CSR ----------------------------------------------
 // Arcs [e0, e1) and vertices [v0, v1) of graph g.
@@ -232,9 +254,13 @@ struct SegItem {
 // Per-graph modularity: acc holds 2 accumulators per graph, q[g] receives Q_g;
 // with act set, only for the graphs g with act[g] set (the others' q[g] is
 // left as it was).
+// With term set, also term[c] = fx((a_c / 2m_g)^2) for every community of
+// the graphs it sums (the state the incremental update starts from); with
+// moved set, the sum of its kMovedSlots counters in the bits of q[G].
 void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src, const int* col,
                            const double* w, const int* C, const fx_t* a, const double* twom_g, int G,
-                           fx_t* acc, double* q, hipStream_t s, const unsigned char* act = nullptr);
+                           fx_t* acc, double* q, hipStream_t s, const unsigned char* act = nullptr,
+                           fx_t* term = nullptr, const unsigned long long* moved = nullptr);
 // out[g] = newid[vb[g]] for g <= G (the total where vb[g] == nv).
 void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64_t* vb, int G, int64_t nv,
                          int64_t* out, hipStream_t s);

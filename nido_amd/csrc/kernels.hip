@@ -317,6 +317,26 @@ mod_finish_kernel(ModDelta D, int nb, double twom_d, double* q)
     for (int i = threadIdx.x; i < 2 * nb; i += kBlock) D.counts[i] = 0;
 }
 
+// Per graph: the graphs that sat out keep their last q[g] (and their
+// accumulators describe no partition once they have been rolled back).
+__global__ void __launch_bounds__(kBlock)
+mod_finish_graphs_kernel(ModDelta D, int nb, int G, double* __restrict__ q, const unsigned char* __restrict__ act)
+{
+    __shared__ unsigned long long red[kBlock / 64];
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock) {
+        if (!act[g]) continue;
+        const double tw = D.twomg[g];
+        q[g] = tw == 0 ? 0.0 : fx_to_double(D.acc[2 * g]) / tw - fx_to_double(D.acc[2 * g + 1]);
+    }
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < 2 * nb; i += kBlock) D.counts[i] = 0;
+        if (D.moved) {
+            const unsigned long long t = sum_moved_slots(D.moved, red);
+            if (threadIdx.x == 0) reinterpret_cast<unsigned long long*>(q)[G] = t;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive scan (int64), tiles of 2048 values per 256-thread workgroup.
 // ---------------------------------------------------------------------------
@@ -549,7 +569,7 @@ __global__ void __launch_bounds__(kBlock)
 mod_seg_kernel(const SegItem* __restrict__ items, int64_t nitems, const int* __restrict__ src,
                const int* __restrict__ col, const double* __restrict__ w, const int* __restrict__ C,
                const fx_t* __restrict__ a, const double* __restrict__ twom_g, fx_t* __restrict__ acc,
-               const unsigned char* __restrict__ act)
+               const unsigned char* __restrict__ act, fx_t* __restrict__ term)
 {
     __shared__ fx_t red[kBlock / 64];
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -561,7 +581,9 @@ mod_seg_kernel(const SegItem* __restrict__ items, int64_t nitems, const int* __r
         const double tw = twom_g[s.g];
         for (int64_t i = s.v0 + threadIdx.x; i < s.v1; i += kBlock) {
             const double x = fx_to_double(a[i]) / tw;
-            T += fx_from_double(x * x);
+            const fx_t t = fx_from_double(x * x);
+            if (term) term[i] = t;
+            T += t;
         }
         I = block_sum_fx(I, red);
         T = block_sum_fx(T, red);
@@ -574,12 +596,18 @@ mod_seg_kernel(const SegItem* __restrict__ items, int64_t nitems, const int* __r
 
 __global__ void __launch_bounds__(kBlock)
 mod_seg_finalize_kernel(const fx_t* __restrict__ acc, const double* __restrict__ twom_g, int G,
-                        double* __restrict__ q, const unsigned char* __restrict__ act)
+                        double* __restrict__ q, const unsigned char* __restrict__ act,
+                        const unsigned long long* __restrict__ moved)
 {
+    __shared__ unsigned long long red[kBlock / 64];
     for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock) {
         if (act && !act[g]) continue;     // q[g] keeps the value of its last evaluation
         const double tw = twom_g[g];
         q[g] = tw == 0 ? 0.0 : fx_to_double(acc[2 * g]) / tw - fx_to_double(acc[2 * g + 1]);
+    }
+    if (moved && blockIdx.x == 0) {   // uniform
+        const unsigned long long t = sum_moved_slots(moved, red);
+        if (threadIdx.x == 0) reinterpret_cast<unsigned long long*>(q)[G] = t;
     }
 }
 
@@ -716,6 +744,12 @@ void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const in
     mod_delta_kernel<<<512, kBlock, 0, s>>>(b, v0, v1, Cn, C32, col, w, a, twom_d, D);
 }
 
+void launch_mod_finish_graphs(const ModDelta& D, int nb, int G, double* q, const unsigned char* act,
+                              hipStream_t s)
+{
+    mod_finish_graphs_kernel<<<grid_for(G, kBlock), kBlock, 0, s>>>(D, nb, G, q, act);
+}
+
 void launch_mod_finish(const ModDelta& D, int nb, double twom_d, double* q, hipStream_t s)
 {
     mod_finish_kernel<<<1, kBlock, 0, s>>>(D, nb, twom_d, q);
@@ -789,14 +823,15 @@ void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* 
 
 void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src, const int* col,
                            const double* w, const int* C, const fx_t* a, const double* twom_g, int G,
-                           fx_t* acc, double* q, hipStream_t s, const unsigned char* act)
+                           fx_t* acc, double* q, hipStream_t s, const unsigned char* act, fx_t* term,
+                           const unsigned long long* moved)
 {
     if (G <= 0) return;
     launch_fill_fx(acc, 2 * (int64_t)G, s);
     if (nitems > 0)
         mod_seg_kernel<<<grid_for(nitems, 1, 8192), kBlock, 0, s>>>(items, nitems, src, col, w, C, a,
-                                                                    twom_g, acc, act);
-    mod_seg_finalize_kernel<<<grid_for(G, kBlock), kBlock, 0, s>>>(acc, twom_g, G, q, act);
+                                                                    twom_g, acc, act, term);
+    mod_seg_finalize_kernel<<<grid_for(G, kBlock), kBlock, 0, s>>>(acc, twom_g, G, q, act, moved);
 }
 
 void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64_t* vb, int G, int64_t nv,

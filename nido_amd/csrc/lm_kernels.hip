@@ -981,10 +981,12 @@ __device__ __forceinline__ int ctab_take(CommitTable& T, int i, fx_t& x)
 __global__ void __launch_bounds__(kBlock)
 commit_kernel(const BinRow* __restrict__ list, int64_t v0, int64_t n, int64_t* __restrict__ C,
               int64_t* __restrict__ Cn, int* __restrict__ C32, const fx_t* __restrict__ k,
-              fx_t* __restrict__ a, const int* __restrict__ gid, const unsigned char* __restrict__ flag)
+              fx_t* __restrict__ a, const int* __restrict__ gid, const unsigned char* __restrict__ flag,
+              const int64_t* __restrict__ off, unsigned long long* __restrict__ moved_arcs)
 {
     __shared__ CommitTable T;
     ctab_init(T);
+    unsigned long long marcs = 0;
     for (int64_t base = blockIdx.x * (int64_t)kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + threadIdx.x;
         int64_t v = 0, src = 0, dst = 0;
@@ -1005,6 +1007,7 @@ commit_kernel(const BinRow* __restrict__ list, int64_t v0, int64_t n, int64_t* _
             C32[v] = (int)dst;
             C[v] = dst;
             Cn[v] = src;
+            if (moved_arcs) marcs += list ? (unsigned long long)list[i].d : (unsigned long long)(off[v + 1] - off[v]);
         }
         __syncthreads();
         const int nk = T.n;
@@ -1016,32 +1019,43 @@ commit_kernel(const BinRow* __restrict__ list, int64_t v0, int64_t n, int64_t* _
         __syncthreads();                 // every thread has read T.n
         if (threadIdx.x == 0) T.n = 0;
     }
+    if (moved_arcs) {   // uniform
+        __shared__ unsigned long long red[kBlock / 64];
+        add_moved_arcs(moved_arcs, marcs, red);
+    }
 }
 
 // The commit plus the bookkeeping of the incremental modularity: a moved
 // vertex lists its row (in chunks) and its old and new community (each
-// community once per batch, through its touched flag).
+// community once per batch, through its touched flag).  The vertices are
+// list[0, n) or, with a null list, [v0, v0 + n); with D.gid set, only
+// vertices of graphs whose flag is set take part.
 __global__ void __launch_bounds__(kBlock)
-commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, int64_t* __restrict__ Cn,
-                      int* __restrict__ C32, const fx_t* __restrict__ k, fx_t* __restrict__ a,
-                      const int64_t* __restrict__ off, ModDelta D)
+commit_tracked_kernel(const BinRow* __restrict__ list, int64_t v0, int64_t n, int b, int64_t* __restrict__ C,
+                      int64_t* __restrict__ Cn, int* __restrict__ C32, const fx_t* __restrict__ k,
+                      fx_t* __restrict__ a, const int64_t* __restrict__ off, ModDelta D,
+                      const unsigned char* __restrict__ flag)
 {
     __shared__ CommitTable T;
     __shared__ int wtot[kBlock / 64];
     __shared__ int s_chunk0, s_comm0;
     ctab_init(T);
+    unsigned long long marcs = 0;
     // workgroup-uniform loop: list space is reserved once per workgroup
     // and chunk, both lists with one 64-bit atomic on the batch's counter
     // pair {chunks, communities} (no carry crosses: each count stays below
     // 2^31)
-    for (int64_t base = v0 + blockIdx.x * (int64_t)kBlock; base < v1; base += (int64_t)gridDim.x * kBlock) {
-        const int64_t v = base + threadIdx.x;
-        int64_t src = 0, dst = 0;
+    for (int64_t base = blockIdx.x * (int64_t)kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = base + threadIdx.x;
+        int64_t v = 0, src = 0, dst = 0;
         bool moved = false;
-        if (v < v1) {
-            src = C[v];
-            dst = Cn[v];
-            moved = src != dst;
+        if (i < n) {
+            v = list ? (int64_t)list[i].v : v0 + i;
+            if (!D.gid || flag[D.gid[v]]) {
+                src = C[v];
+                dst = Cn[v];
+                moved = src != dst;
+            }
         }
         if (!__syncthreads_or(moved)) continue;   // uniform; also orders the previous chunk's clears
         int nchunk = 0;
@@ -1056,6 +1070,7 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
             e0 = off[v];
             e1 = off[v + 1];
             nchunk = (int)((e1 - e0 + kDeltaArcs - 1) / kDeltaArcs);
+            marcs += (unsigned long long)(e1 - e0);
         }
         __syncthreads();                  // the table is complete
         // every community a moved vertex left or joined: its total, and a
@@ -1103,6 +1118,10 @@ commit_tracked_kernel(int64_t v0, int64_t v1, int b, int64_t* __restrict__ C, in
         for (int j = 0; j < 2; ++j)
             if (cs[j] >= 0) D.comms[at++] = cs[j];
     }
+    if (D.moved) {   // uniform
+        __shared__ unsigned long long red[kBlock / 64];
+        add_moved_arcs(D.moved, marcs, red);
+    }
 }
 
 }  // namespace
@@ -1137,11 +1156,13 @@ static void small_launch(const LmArgs& A, const SmallBins& L, int blocks, hipStr
     }
 }
 
-void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s)
+void launch_lm_small(const LmArgs& A, const SmallBins& L, bool bins, bool fill, bool split, hipStream_t s,
+                     bool delta)
 {
     // parts: 0..5 short-row bins, 6 key fill, 7 delta (goes with the bins)
     bool want[NUM_SMALL_BINS + 2];
     for (int k = 0; k < NUM_SMALL_BINS + 2; ++k) want[k] = k == NUM_SMALL_BINS ? fill : bins;
+    want[NUM_SMALL_BINS + 1] = bins && delta && L.dblocks > 0;
     auto launch_parts = [&](const bool* on) {
         SmallBins one = L;
         for (int k = 0; k < NUM_SMALL_BINS; ++k) one.n[k] = on[k] ? L.n[k] : 0;
@@ -1189,25 +1210,36 @@ void launch_lm_big(const LmArgs& A, const BigArgs& B, hipStream_t s)
 }
 
 void launch_commit(int64_t v0, int64_t v1, int64_t* C, int64_t* Cn, int* C32, const fx_t* k, fx_t* a,
-                   hipStream_t s)
+                   hipStream_t s, const int64_t* off, unsigned long long* moved)
 {
     if (v1 <= v0) return;
     commit_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(nullptr, v0, v1 - v0, C, Cn, C32, k, a,
-                                                               nullptr, nullptr);
+                                                               nullptr, nullptr, off, moved);
 }
 
 void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* Cn, int* C32, const fx_t* k,
                            fx_t* a, const int64_t* off, const ModDelta& D, hipStream_t s)
 {
     if (v1 <= v0) return;
-    commit_tracked_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(v0, v1, b, C, Cn, C32, k, a, off, D);
+    commit_tracked_kernel<<<grid_for(v1 - v0, kBlock), kBlock, 0, s>>>(nullptr, v0, v1 - v0, b, C, Cn, C32, k, a,
+                                                                       off, D, nullptr);
+}
+
+void launch_commit_tracked_list(const BinRow* list, int64_t n, int b, int64_t* C, int64_t* Cn, int* C32,
+                                const fx_t* k, fx_t* a, const int64_t* off, const ModDelta& D,
+                                const unsigned char* flag, hipStream_t s)
+{
+    if (n <= 0) return;
+    commit_tracked_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(list, 0, n, b, C, Cn, C32, k, a, off, D, flag);
 }
 
 void launch_commit_filtered(const BinRow* list, int64_t v0, int64_t n, int64_t* C, int64_t* Cn, int* C32,
-                            const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s)
+                            const fx_t* k, fx_t* a, const int* gid, const unsigned char* flag, hipStream_t s,
+                            unsigned long long* moved)
 {
     if (n <= 0) return;
-    commit_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(list, v0, n, C, Cn, C32, k, a, gid, flag);
+    commit_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(list, v0, n, C, Cn, C32, k, a, gid, flag, nullptr,
+                                                         moved);
 }
 
 }  // namespace nido

@@ -9,9 +9,9 @@ import oracle
 Q_TOL = 1e-12   # |dQ| bound; the canonical arithmetic makes it 0 in practice
 
 
-def run_oracle(off, col, w, nb, tol=1e-6, max_loops=500, nthreads=0):
+def run_oracle(off, col, w, nb, tol=1e-6, max_loops=500, nthreads=0, tol_phase=None):
     o = oracle.Oracle(off, col, w, nb, nthreads)
-    r = o.run(max_loops, tol, record=True)
+    r = o.run(max_loops, tol, tol_phase, record=True)
     r["clustering"] = o.clustering()
     o.close()
     return r

@@ -31,12 +31,12 @@ def _mixed_graphs(na):
     return gs
 
 
-def _check(na, graphs, nb, **tuning):
-    ref = [run_oracle(off, col, w, nb) for off, col, w in graphs]
+def _check(na, graphs, nb, max_loops=500, tol=1e-6, tol_phase=None, **tuning):
+    ref = [run_oracle(off, col, w, nb, tol, max_loops, tol_phase=tol_phase) for off, col, w in graphs]
     bg = na.BatchGPU(graphs, nb)
     if tuning:
         bg.set_tuning(**tuning)
-    st = bg.run(500, 1e-6)
+    st = bg.run(max_loops, tol, tol_phase)
     assert st.phases == max(len(r["phases"]) for r in ref)
     for i, r in enumerate(ref):
         gs = bg.graph_stats(i)
@@ -110,3 +110,40 @@ def test_batch_with_partitioned_long_rows():
     _check(na, graphs, 8)
     print("case: batch with long rows nb=8, small partitions", flush=True)
     _check(na, graphs, 8, big_part_arcs=64, big_min_deg=128)
+
+
+def _history(bg, n):
+    out = []
+    for i in range(n):
+        gs = bg.graph_stats(i)
+        ph = [bg.graph_phase(i, p) for p in range(gs.phases)]
+        out.append(([(p.nv, p.ne, p.loops, p.q0, p.q) for p in ph], gs.q, bg.clustering(i).tolist()))
+    return out
+
+
+def test_batch_incremental_modularity_equals_the_sweep():
+    """Per-graph Q from the tracked commits and per-batch deltas (always
+    after a phase's first iteration, or when few arcs moved) must equal a
+    sweep over every arc bit for bit: same loops, same Q in every phase of
+    every graph."""
+    na = _na()
+    graphs = _mixed_graphs(na) + [na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 31).arrays()]
+    runs = []
+    for full in (1, 0, 2):
+        print(f"case: mixed batch nb=8 full_modularity={full}", flush=True)
+        bg = na.BatchGPU(graphs, 8)
+        bg.set_tuning(full_modularity=full)
+        bg.run(500, 1e-6)
+        runs.append(_history(bg, len(graphs)))
+        bg.close()
+    for i, (a, b, c) in enumerate(zip(*runs)):
+        assert a == b == c, f"graph {i}: the incremental Q differs from the sweep"
+
+
+def test_batch_rollbacks_that_go_on():
+    """tol < 0: a graph whose iteration lowers Q rolls back and keeps
+    iterating (the loop runs while dQ > tol, louvain_gpu.cpp:53), so its Q
+    must be evaluated afresh from the restored partition."""
+    na = _na()
+    print("case: mixed batch nb=4 tol=-1 max_loops=5", flush=True)
+    _check(na, _mixed_graphs(na), 4, max_loops=5, tol=-1.0, tol_phase=1e-6)

@@ -43,13 +43,13 @@ def test_pinned_staged_upload_with_many_chunks():
         na.set_upload_staging(4096, 1000)          # chunks are multiples of 4096
 
 
-def test_tuning_switches_reject_values_other_than_zero_or_one():
+def test_tuning_switches_reject_values_out_of_range():
     na = _na()
     off, col, w = na.gen_sbm(500, 50, 0.2, 0.01, True, 3).arrays()
     g = na.GraphGPU((off, col, w), 2)
-    for knob in ("split_small", "overlap_long", "full_modularity"):
+    for knob, bad in (("split_small", 2), ("overlap_long", 2), ("full_modularity", 3)):
         with pytest.raises(na.NidoError):
-            g.set_tuning(**{knob: 2})
-    g.set_tuning(split_small=1, overlap_long=0, full_modularity=1)
-    assert g.get_tuning()["split_small"] == 1
+            g.set_tuning(**{knob: bad})
+    g.set_tuning(split_small=1, overlap_long=0, full_modularity=2)
+    assert g.get_tuning()["split_small"] == 1 and g.get_tuning()["full_modularity"] == 2
     g.close()

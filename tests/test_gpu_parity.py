@@ -134,6 +134,8 @@ KNOBS = [
     {"big_slots": 1024, "big_key_limit": 512, "big_part_arcs": 384, "overlap_long": 1},
     {"full_modularity": 1},                                         # every iteration sweeps all arcs for Q
     {"full_modularity": 1, "use_graph": 2},
+    {"full_modularity": 2},                                         # incremental Q after every first iteration
+    {"full_modularity": 2, "use_graph": 2},
     {"fixed_short": 0},                                             # short rows in Q64.64 instead of 64-bit fixed point
 ]
 

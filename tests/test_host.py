@@ -210,10 +210,11 @@ def test_weight_totals_that_would_wrap_the_exact_sums_are_refused():
     assert "graph 1" in str(e.value)
 
 
-def test_tuning_flags_accept_only_zero_or_one():
-    """split_small, overlap_long, full_modularity and fixed_short are 0/1 switches; the
-    knob values are checked before the handle, so a null handle shows which
-    check refused the struct without a device."""
+def test_tuning_flags_accept_only_their_values():
+    """split_small, overlap_long and fixed_short are 0/1 switches,
+    full_modularity takes 0, 1 or 2; the knob values are checked before the
+    handle, so a null handle shows which check refused the struct without a
+    device."""
     import ctypes as C
     base = dict(thread_max_deg=2, big_min_deg=512, big_part_arcs=1024, big_key_limit=1024, use_graph=1,
                 big_slots=2048, split_small=0, overlap_long=1, full_modularity=0, fixed_short=1)
@@ -221,10 +222,13 @@ def test_tuning_flags_accept_only_zero_or_one():
     assert na.lib.nido_graph_set_tuning(None, C.byref(t)) == -1
     assert b"null handle" in na.lib.nido_last_error()
     for knob in ("split_small", "overlap_long", "full_modularity", "fixed_short"):
-        for bad in (2, -1, 7):
+        for bad in ((3, -1, 7) if knob == "full_modularity" else (2, -1, 7)):
             t = na.Tuning(**dict(base, **{knob: bad}))
             assert na.lib.nido_graph_set_tuning(None, C.byref(t)) == -1
             assert knob.encode() in na.lib.nido_last_error(), (knob, bad)
+    t = na.Tuning(**dict(base, full_modularity=2))
+    assert na.lib.nido_graph_set_tuning(None, C.byref(t)) == -1
+    assert b"null handle" in na.lib.nido_last_error()
 
 
 def test_batch_bounds_host_matches_oracle():
