@@ -114,6 +114,13 @@ int nido_device_count(int* count);
  * (its uploads are plain pageable copies, graph_gpu.cpp:967-1027); lowering
  * it lets small inputs take the pinned path. */
 int nido_set_upload_staging(int64_t min_bytes, int64_t chunk_bytes);
+/* Phases of at least min_vertices vertices whose arcs are resident build
+ * their degree bins and sweep row lists on the device, smaller ones on the
+ * host (the same lists); 0 restores the default, 2^22.  Process-wide.  No
+ * counterpart in the reference (its bins are built by Thrust calls each
+ * phase, graph_gpu.cpp:94-139); lowering it lets small inputs take the
+ * device builder. */
+int nido_set_device_lists(int64_t min_vertices);
 
 /* ---- host graph: Graph (graph.hpp:14-102) ------------------------------- */
 

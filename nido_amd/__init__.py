@@ -83,6 +83,7 @@ if lib.nido_abi_version() != ABI_VERSION:
                       f"rebuild it with `make -C {_HERE}`")
 _sig("nido_device_count", C.c_int, C.POINTER(C.c_int))
 _sig("nido_set_upload_staging", C.c_int, _i64, _i64)
+_sig("nido_set_device_lists", C.c_int, _i64)
 _sig("nido_hgraph_read", C.c_int, C.c_char_p, C.POINTER(C.c_void_p))
 _sig("nido_hgraph_write", C.c_int, C.c_void_p, C.c_char_p)
 _sig("nido_hgraph_from_csr", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.POINTER(C.c_void_p))
@@ -195,6 +196,12 @@ def set_upload_staging(min_bytes: int = 0, chunk_bytes: int = 0) -> None:
     """Pinned staging of graph uploads from min_bytes in chunk_bytes pieces
     (0: the defaults, 1 GiB and 64 MiB); process-wide."""
     _check(lib.nido_set_upload_staging(int(min_bytes), int(chunk_bytes)), "nido_set_upload_staging")
+
+
+def set_device_lists(min_vertices: int = 0) -> None:
+    """Phases of at least min_vertices vertices build their degree bins and
+    row lists on the device (0: the default, 2^22); process-wide."""
+    _check(lib.nido_set_device_lists(int(min_vertices)), "nido_set_device_lists")
 
 
 def modularity_noalloc(nv: int, ne: int, offsets: int, cols: int, weights: int, comm: int, q: int,

@@ -5,6 +5,7 @@
 // stays in HBM, every step is a hand-written kernel on the handle's stream,
 // and the host only reads back one scalar per modularity evaluation.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <mutex>
@@ -344,6 +345,11 @@ struct nido_graph {
     bool inc_ok = false;                 // d_term and d_iacc (d_gacc) describe the current partition
     bool ident = false;                  // C is the singleton partition: set only by do_singleton
     int64_t last_moved = -1;             // several graphs: arcs the last iteration moved; -1: unknown
+    // Every arc (v, u, w) has a reverse arc (u, v, w): the incremental update
+    // counts an arc whose state changes twice, for itself and its reverse.
+    // Checked once at creation (coarse graphs of such a graph are such
+    // graphs too); otherwise every iteration sweeps.
+    bool symmetric = false;
     bool ride_delta = false;             // short-row launches carry the previous batch's delta
     int64_t iters_since_singleton = 0;   // the first iteration moves most vertices: full sweep there
     // several graphs: the iteration's moved-arc counters (their sum is read
@@ -414,6 +420,149 @@ int upload(T** d, const std::vector<T>& h, hipStream_t s)
 
 void drop_iteration_graph(nido_graph* g);
 
+// Phases of at least this many vertices (and arcs resident on the device)
+// build their degree bins and row lists on the device: for 4,096 graphs of
+// 48.5 M vertices in one handle the host built them in ~100 ms and uploaded
+// 1 GB of them.
+std::atomic<int64_t> g_device_lists_min{kParallelRows};
+
+// The degree bins of the phase built on the device into d_binv (bins.hip);
+// the host gets the group bounds and each batch's long rows, and builds the
+// long-row work from those.
+int device_bins(nido_graph* g, PhaseBins& pb)
+{
+    const int nb = g->nb, G = g->G;
+    const auto& off = g->h_off;
+    hipStream_t s = g->stream;
+    std::vector<BinTile> tiles;
+    std::vector<int64_t> tb0;
+    bin_tiles(nb, G, g->parts, g->gparts, tiles, tb0);
+    const int64_t T = (int64_t)tiles.size(), nc = NUM_BINS * T;
+    BinTile* d_tiles = nullptr;
+    int64_t *d_tb0 = nullptr, *d_cnt = nullptr, *d_starts = nullptr, *d_total = nullptr;
+    void* tmp = nullptr;
+    auto cleanup = [&]() {
+        dev_free(d_tiles); dev_free(d_tb0); dev_free(d_cnt); dev_free(d_starts); dev_free(d_total);
+        raw_free(tmp);
+        tmp = nullptr;
+    };
+    std::vector<int64_t> starts((size_t)nb * NUM_BINS, -1);
+    int64_t rows = 0;
+    int rc = NIDO_OK;
+    do {
+        if ((rc = dev_alloc(&d_tiles, std::max<int64_t>(T, 1))) || (rc = dev_alloc(&d_tb0, (int64_t)nb + 1)) ||
+            (rc = dev_alloc(&d_cnt, std::max<int64_t>(nc, 1))) ||
+            (rc = dev_alloc(&d_starts, std::max<int64_t>((int64_t)nb * NUM_BINS, 1))) ||
+            (rc = dev_alloc(&d_total, 1)) || (rc = raw_alloc(&tmp, scan_temp_bytes(std::max<int64_t>(nc, 1)))))
+            break;
+        if (T > 0 && (hipMemcpyAsync(d_tiles, tiles.data(), sizeof(BinTile) * (size_t)T, hipMemcpyHostToDevice,
+                                     s) != hipSuccess ||
+                      hipMemcpyAsync(d_tb0, tb0.data(), sizeof(int64_t) * tb0.size(), hipMemcpyHostToDevice,
+                                     s) != hipSuccess)) {
+            rc = set_error(NIDO_ERR_HIP, "bin tile upload failed");
+            break;
+        }
+        if (T > 0) {
+            launch_bin_count(d_tiles, T, g->d_off, g->tuning.thread_max_deg, g->tuning.big_min_deg, d_cnt, s);
+            launch_exclusive_scan(d_cnt, d_cnt, nc, d_total, tmp, s);
+            if (hipMemcpyAsync(&rows, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = set_error(NIDO_ERR_HIP, "bin count failed");
+                break;
+            }
+        }
+        dev_free(g->d_binv);
+        if ((rc = dev_alloc(&g->d_binv, std::max<int64_t>(rows, 1)))) break;
+        if (T > 0) {
+            launch_bin_scatter(d_tiles, T, g->d_off, g->tuning.thread_max_deg, g->tuning.big_min_deg, d_cnt,
+                               g->d_binv, s);
+            launch_bin_starts(d_cnt, d_tb0, nb, d_starts, s);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(starts.data(), d_starts, sizeof(int64_t) * starts.size(), hipMemcpyDeviceToHost,
+                               s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = set_error(NIDO_ERR_HIP, "bin scatter failed");
+                break;
+            }
+        }
+    } while (false);
+    cleanup();
+    if (rc != NIDO_OK) return rc;
+    // a batch without tiles starts where the next one does
+    int64_t next = rows;
+    for (int b = nb - 1; b >= 0; --b) {
+        for (int k = NUM_BINS - 1; k >= 0; --k) {
+            int64_t& st = starts[(size_t)b * NUM_BINS + k];
+            if (st < 0) st = next;
+        }
+        next = starts[(size_t)b * NUM_BINS];
+    }
+    std::vector<int64_t> cnt((size_t)nb * NUM_BINS), bverts((size_t)nb, 0), barcs((size_t)nb, 0);
+    for (size_t i = 0; i < cnt.size(); ++i) cnt[i] = (i + 1 < cnt.size() ? starts[i + 1] : rows) - starts[i];
+    std::vector<std::vector<BinRow>> big((size_t)nb);
+    for (int b = 0; b < nb; ++b) {
+        const int64_t n = cnt[(size_t)b * NUM_BINS + BIN_BIG];
+        big[(size_t)b].resize((size_t)n);
+        if (n > 0)
+            HIP_TRY(hipMemcpyAsync(big[(size_t)b].data(), g->d_binv + starts[(size_t)b * NUM_BINS + BIN_BIG],
+                                   sizeof(BinRow) * (size_t)n, hipMemcpyDeviceToHost, s));
+        auto range = [&](int64_t v0, int64_t v1) {
+            bverts[(size_t)b] += v1 - v0;
+            barcs[(size_t)b] += off[(size_t)v1] - off[(size_t)v0];
+        };
+        if (G == 1) range(g->parts[(size_t)b], g->parts[(size_t)b + 1]);
+        else
+            for (int i = 0; i < G; ++i)
+                range(g->gparts[(size_t)i * (nb + 1) + b], g->gparts[(size_t)i * (nb + 1) + b + 1]);
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    build_phase_lists(off.data(), nb, cnt, big, bverts, barcs, g->tuning, pb);
+    return NIDO_OK;
+}
+
+// The row lists of the degree and modularity sweeps built on the device.
+int device_row_lists(nido_graph* g)
+{
+    const int64_t nv = g->nv, nt = (nv + kBinTileVerts - 1) / kBinTileVerts;
+    hipStream_t s = g->stream;
+    int64_t *cs = nullptr, *ci = nullptr, *tot = nullptr;
+    void* tmp = nullptr;
+    auto cleanup = [&]() {
+        dev_free(cs); dev_free(ci); dev_free(tot);
+        raw_free(tmp);
+        tmp = nullptr;
+    };
+    int64_t h_tot[2] = {0, 0};
+    int rc = NIDO_OK;
+    do {
+        if ((rc = dev_alloc(&cs, std::max<int64_t>(nt, 1))) || (rc = dev_alloc(&ci, std::max<int64_t>(nt, 1))) ||
+            (rc = dev_alloc(&tot, 2)) || (rc = raw_alloc(&tmp, scan_temp_bytes(std::max<int64_t>(nt, 1)))))
+            break;
+        if (nt > 0) {
+            launch_rowlist_count(g->d_off, nv, cs, ci, s);
+            launch_exclusive_scan(cs, cs, nt, tot, tmp, s);
+            launch_exclusive_scan(ci, ci, nt, tot + 1, tmp, s);
+            if (hipMemcpyAsync(h_tot, tot, sizeof(h_tot), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = set_error(NIDO_ERR_HIP, "row list count failed");
+                break;
+            }
+        }
+        dev_free(g->d_small);
+        dev_free(g->d_items);
+        if ((rc = dev_alloc(&g->d_small, std::max<int64_t>(h_tot[0], 1))) ||
+            (rc = dev_alloc(&g->d_items, std::max<int64_t>(h_tot[1], 1))))
+            break;
+        if (nt > 0) launch_rowlist_scatter(g->d_off, nv, cs, ci, g->d_small, g->d_items, s);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            rc = set_error(NIDO_ERR_HIP, "row list scatter failed");
+    } while (false);
+    cleanup();
+    g->nsmall = h_tot[0];
+    g->nitems = h_tot[1];
+    return rc;
+}
+
 // Per-phase set-up: maximum order, batches, degree bins, the row slices of
 // the degree/modularity sweeps, k_v and 2m (GraphGPU ctor steps
 // graph_gpu.cpp:94-139 and aggregation steps graph_gpu.cpp:1487-1516).
@@ -442,8 +591,10 @@ int setup_phase(nido_graph* g)
     const std::vector<int64_t>& lparts = g->windowed ? g->units : g->parts;
     const int nl = g->windowed ? (int)g->units.size() - 1 : g->nb;
 
+    const bool dev_lists = !g->windowed && nv >= g_device_lists_min.load();
     PhaseBins pb;
-    build_phase_bins(off.data(), nv, nl, G, lparts, g->gparts, g->tuning, pb);
+    if (dev_lists) NIDO_TRY(device_bins(g, pb));
+    else build_phase_bins(off.data(), nv, nl, G, lparts, g->gparts, g->tuning, pb);
     const auto& binv = pb.binv;
     const auto& bitems = pb.bitems;
     const auto& chunks = pb.chunks;
@@ -461,7 +612,7 @@ int setup_phase(nido_graph* g)
     g->long_bytes = pb.long_bytes;
     g->blist0 = pb.blist0;
     g->blistn = pb.blistn;
-    NIDO_TRY(upload(&g->d_binv, binv, g->stream));
+    if (!dev_lists) NIDO_TRY(upload(&g->d_binv, binv, g->stream));
     NIDO_TRY(upload(&g->d_big_items, bitems, g->stream));
     NIDO_TRY(upload(&g->d_key_chunks, chunks, g->stream));
     if (key_need > g->key_cap) {
@@ -601,13 +752,17 @@ int setup_phase(nido_graph* g)
         }
         HIP_TRY(hipStreamSynchronize(g->stream));   // hk / hkd go out of scope
     } else {
-        std::vector<int> small;
-        std::vector<RowItem> items;
-        build_row_lists(off.data(), nv, small, items);
-        g->nsmall = (int64_t)small.size();
-        g->nitems = (int64_t)items.size();
-        NIDO_TRY(upload(&g->d_small, small, g->stream));
-        NIDO_TRY(upload(&g->d_items, items, g->stream));
+        if (dev_lists) {
+            NIDO_TRY(device_row_lists(g));
+        } else {
+            std::vector<int> small;
+            std::vector<RowItem> items;
+            build_row_lists(off.data(), nv, small, items);
+            g->nsmall = (int64_t)small.size();
+            g->nitems = (int64_t)items.size();
+            NIDO_TRY(upload(&g->d_small, small, g->stream));
+            NIDO_TRY(upload(&g->d_items, items, g->stream));
+        }
 
         launch_row_ids(g->d_off, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_src, g->stream);
         launch_narrow_ids(g->d_col, g->d_col32, g->ne, g->stream);
@@ -989,7 +1144,7 @@ int run_iteration_launches(nido_graph* g)
     // every later iteration; 1: never.
     const bool any_w = g->G > 1 || g->twom != 0;
     const int fm = g->tuning.full_modularity;
-    bool inc = any_w && g->inc_ok && fm != 1 && g->iters_since_singleton > 0 && !g->windowed;
+    bool inc = any_w && g->symmetric && g->inc_ok && fm != 1 && g->iters_since_singleton > 0 && !g->windowed;
     if (inc && fm == 0 && g->G > 1) inc = g->last_moved >= 0 && g->last_moved * kIncShare <= g->ne;
     ++g->iters_since_singleton;
     NIDO_TRY(launch_or_replay(g, inc));
@@ -1662,6 +1817,19 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
             return fail(set_error(NIDO_ERR_HIP, "graph id set-up failed"));
     }
     if ((rc = setup_phase(g))) return fail(rc);
+    if (!g->windowed && ne > 0) {
+        fx_t* d_h = nullptr;   // four 64-bit sums
+        unsigned long long h[4];
+        if ((rc = dev_alloc(&d_h, 2))) return fail(rc);
+        launch_fill_fx(d_h, 2, s);
+        launch_sym_hash(g->d_src, g->d_col32, g->d_w, ne, reinterpret_cast<unsigned long long*>(d_h), s);
+        const bool ok = hipGetLastError() == hipSuccess &&
+                        hipMemcpyAsync(h, d_h, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                        hipStreamSynchronize(s) == hipSuccess;
+        dev_free(d_h);
+        if (!ok) return fail(set_error(NIDO_ERR_HIP, "arc symmetry check failed"));
+        g->symmetric = h[0] == h[1] && h[2] == h[3];
+    }
     if ((rc = do_singleton(g))) return fail(rc);
     *out = g;
     return NIDO_OK;
@@ -1690,6 +1858,13 @@ int d2h(nido_graph* g, T* dst, const T* src, int64_t n)
 extern "C" {
 
 int nido_abi_version(void) { return NIDO_ABI_VERSION; }
+
+int nido_set_device_lists(int64_t min_vertices)
+{
+    if (min_vertices < 0) return set_error(NIDO_ERR_INVALID, "min_vertices must be >= 0");
+    g_device_lists_min.store(min_vertices ? min_vertices : kParallelRows);
+    return NIDO_OK;
+}
 
 int nido_device_count(int* count)
 {

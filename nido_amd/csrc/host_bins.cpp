@@ -92,6 +92,107 @@ int64_t max_degree(const int64_t* off, int64_t nv)
     return m;
 }
 
+namespace {
+
+// Long row v (d arcs) of a batch: P = 2^ceil(log2(d / part_arcs)) partition
+// items, their partial slots, an HBM fallback table, and for P >
+// kDirectParts the key-fill chunks and bucket space.
+void add_long_row(BatchWork& w, int64_t v, int64_t d, const int64_t* off, const nido_tuning& t)
+{
+    int P = 1;
+    while (P < kMaxParts && (int64_t)P * t.big_part_arcs < d) P <<= 1;
+    const bool direct = P <= kDirectParts;
+    for (int p = 0; p < P; ++p)
+        w.items.push_back(BigItem{(int)v, p, P, w.slot, w.ksum, direct ? -1 : w.csum, off[v], d});
+    if (!direct) {
+        for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs) {
+            w.chunks.push_back(KeyChunk{(int)v, P, e, std::min(off[v + 1], e + kKeyChunkArcs),
+                                        w.ksum + (e - off[v]), w.csum});
+            w.csum += P + 1;
+        }
+        w.ksum += d;
+    }
+    w.poff.push_back(w.psum);
+    w.psum += P;
+    w.tsum += pow2_at_least(2 * d);
+    w.toff.push_back(w.tsum);
+    ++w.slot;
+}
+
+// Longest rows first: their partition workgroups are the long poles.
+void order_items(BatchWork& w, const int64_t* off)
+{
+    std::stable_sort(w.items.begin(), w.items.end(), [&](const BigItem& x, const BigItem& y) {
+        return off[x.v + 1] - off[x.v] > off[y.v + 1] - off[y.v];
+    });
+}
+
+// Batch offsets of every list, in batch order; binv is filled from the
+// batches' rows when they were built on the host.
+void merge_batches(const std::vector<BatchWork>& W, int nb, bool rows_on_host, PhaseBins& pb)
+{
+    const size_t nb1 = (size_t)nb + 1;
+    pb.item_base.assign((size_t)nb, 0);
+    pb.poff_base.assign((size_t)nb, 0);
+    pb.toff_base.assign((size_t)nb, 0);
+    pb.chunk_base.assign(nb1, 0);
+    pb.small_start.assign((size_t)nb * NUM_SMALL_BINS, 0);
+    pb.small_n.assign((size_t)nb * NUM_SMALL_BINS, 0);
+    pb.blist0.assign((size_t)nb, 0);
+    pb.blistn.assign((size_t)nb, 0);
+    pb.short_bytes.assign((size_t)nb, 0.0);
+    pb.long_bytes.assign((size_t)nb, 0.0);
+    pb.tab_need = pb.part_need = pb.slot_need = pb.key_need = pb.cpo_need = 0;
+    int64_t rows = 0, items = 0, chunks = 0, poffs = 0, toffs = 0;
+    for (int b = 0; b < nb; ++b) {
+        const BatchWork& w = W[(size_t)b];
+        pb.blist0[(size_t)b] = rows;
+        int64_t p = rows, n = 0;
+        for (int k = 0; k < NUM_SMALL_BINS; ++k) {
+            pb.small_start[(size_t)b * NUM_SMALL_BINS + k] = p;
+            pb.small_n[(size_t)b * NUM_SMALL_BINS + k] = w.cnt[k];
+            p += w.cnt[k];
+        }
+        for (int k = 0; k < NUM_BINS; ++k) n += w.cnt[k];
+        pb.blistn[(size_t)b] = n;
+        pb.item_base[(size_t)b] = items;
+        pb.poff_base[(size_t)b] = poffs;
+        pb.toff_base[(size_t)b] = toffs;
+        pb.chunk_base[(size_t)b] = chunks;
+        pb.short_bytes[(size_t)b] = w.sbytes;
+        pb.long_bytes[(size_t)b] = w.lbytes;
+        rows += n;
+        items += (int64_t)w.items.size();
+        chunks += (int64_t)w.chunks.size();
+        poffs += (int64_t)w.poff.size();
+        toffs += (int64_t)w.toff.size();
+        pb.tab_need = std::max(pb.tab_need, w.tsum);
+        pb.part_need = std::max(pb.part_need, w.psum);
+        pb.slot_need = std::max<int64_t>(pb.slot_need, w.slot);
+        pb.key_need = std::max(pb.key_need, w.ksum);
+        pb.cpo_need = std::max(pb.cpo_need, w.csum);
+    }
+    pb.chunk_base[(size_t)nb] = chunks;
+    pb.rows = rows;
+    pb.binv.clear();
+    if (rows_on_host) pb.binv.resize((size_t)std::max<int64_t>(rows, 1));
+    pb.bitems.resize((size_t)items);
+    pb.chunks.resize((size_t)chunks);
+    pb.poff.resize((size_t)poffs);
+    pb.toff.resize((size_t)toffs);
+#pragma omp parallel for schedule(dynamic, 1) if (rows >= kParallelRows)
+    for (int b = 0; b < nb; ++b) {
+        const BatchWork& w = W[(size_t)b];
+        if (rows_on_host) std::copy(w.rows.begin(), w.rows.end(), pb.binv.begin() + pb.blist0[(size_t)b]);
+        std::copy(w.items.begin(), w.items.end(), pb.bitems.begin() + pb.item_base[(size_t)b]);
+        std::copy(w.chunks.begin(), w.chunks.end(), pb.chunks.begin() + pb.chunk_base[(size_t)b]);
+        std::copy(w.poff.begin(), w.poff.end(), pb.poff.begin() + pb.poff_base[(size_t)b]);
+        std::copy(w.toff.begin(), w.toff.end(), pb.toff.begin() + pb.toff_base[(size_t)b]);
+    }
+}
+
+}  // namespace
+
 void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::vector<int64_t>& parts,
                       const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb)
 {
@@ -107,10 +208,7 @@ void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::
     // Short rows: vertices grouped by (batch, bin), ascending id inside a
     // group, then the batch's long rows; isolated vertices are left out
     // (they never move, and C' = C holds for them throughout), so a batch's
-    // slice of binv lists every vertex its commit has to visit.  Long rows:
-    // per batch, P = 2^ceil(log2(d / part_arcs)) partition items, their
-    // partial slots, HBM fallback tables, and for P > kDirectParts the
-    // key-fill chunks and bucket space.
+    // slice of binv lists every vertex its commit has to visit.
 #pragma omp parallel for schedule(dynamic, 1) if (nv >= kParallelRows)
     for (int b = 0; b < nb; ++b) {
         BatchWork& w = W[(size_t)b];
@@ -135,87 +233,61 @@ void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::
                 (k != BIN_BIG ? w.sbytes : w.lbytes) += 24.0 * (double)d + 40.0;
                 if (d == 0) continue;
                 w.rows[(size_t)cur[k]++] = BinRow{(int)v, (int)std::min<int64_t>(d, INT32_MAX), off[v]};
-                if (k != BIN_BIG) continue;
-                int P = 1;
-                while (P < kMaxParts && (int64_t)P * t.big_part_arcs < d) P <<= 1;
-                const bool direct = P <= kDirectParts;
-                for (int p = 0; p < P; ++p)
-                    w.items.push_back(BigItem{(int)v, p, P, w.slot, w.ksum, direct ? -1 : w.csum, off[v], d});
-                if (!direct) {
-                    for (int64_t e = off[v]; e < off[v + 1]; e += kKeyChunkArcs) {
-                        w.chunks.push_back(KeyChunk{(int)v, P, e, std::min(off[v + 1], e + kKeyChunkArcs),
-                                                    w.ksum + (e - off[v]), w.csum});
-                        w.csum += P + 1;
-                    }
-                    w.ksum += d;
-                }
-                w.poff.push_back(w.psum);
-                w.psum += P;
-                w.tsum += pow2_at_least(2 * d);
-                w.toff.push_back(w.tsum);
-                ++w.slot;
+                if (k == BIN_BIG) add_long_row(w, v, d, off, t);
             }
         });
-        // longest rows first: their partition workgroups are the long poles
-        std::stable_sort(w.items.begin(), w.items.end(), [&](const BigItem& x, const BigItem& y) {
-            return off[x.v + 1] - off[x.v] > off[y.v + 1] - off[y.v];
-        });
+        order_items(w, off);
     }
+    merge_batches(W, nb, true, pb);
+}
 
-    pb.item_base.assign((size_t)nb, 0);
-    pb.poff_base.assign((size_t)nb, 0);
-    pb.toff_base.assign((size_t)nb, 0);
-    pb.chunk_base.assign(nb1, 0);
-    pb.small_start.assign((size_t)nb * NUM_SMALL_BINS, 0);
-    pb.small_n.assign((size_t)nb * NUM_SMALL_BINS, 0);
-    pb.blist0.assign((size_t)nb, 0);
-    pb.blistn.assign((size_t)nb, 0);
-    pb.short_bytes.assign((size_t)nb, 0.0);
-    pb.long_bytes.assign((size_t)nb, 0.0);
-    pb.tab_need = pb.part_need = pb.slot_need = pb.key_need = pb.cpo_need = 0;
-    int64_t rows = 0, items = 0, chunks = 0, poffs = 0, toffs = 0;
+void bin_tiles(int nb, int G, const std::vector<int64_t>& parts, const std::vector<int64_t>& gparts,
+               std::vector<BinTile>& tiles, std::vector<int64_t>& tb0)
+{
+    const size_t nb1 = (size_t)nb + 1;
+    tiles.clear();
+    tb0.assign(nb1, 0);
     for (int b = 0; b < nb; ++b) {
-        const BatchWork& w = W[(size_t)b];
-        pb.blist0[(size_t)b] = rows;
-        int64_t p = rows;
-        for (int k = 0; k < NUM_SMALL_BINS; ++k) {
-            pb.small_start[(size_t)b * NUM_SMALL_BINS + k] = p;
-            pb.small_n[(size_t)b * NUM_SMALL_BINS + k] = w.cnt[k];
-            p += w.cnt[k];
+        tb0[(size_t)b] = (int64_t)tiles.size();
+        auto add = [&](int64_t v0, int64_t v1) {
+            for (int64_t v = v0; v < v1; v += kBinTileVerts)
+                tiles.push_back(BinTile{v, std::min(v1, v + kBinTileVerts), 0, 0});
+        };
+        if (G == 1) add(parts[(size_t)b], parts[(size_t)b + 1]);
+        else
+            for (int gg = 0; gg < G; ++gg) add(gparts[(size_t)gg * nb1 + b], gparts[(size_t)gg * nb1 + b + 1]);
+        const int64_t t0 = tb0[(size_t)b], nt = (int64_t)tiles.size() - t0;
+        for (int64_t i = 0; i < nt; ++i) {
+            tiles[(size_t)(t0 + i)].at = NUM_BINS * t0 + i;
+            tiles[(size_t)(t0 + i)].stride = nt;
         }
-        pb.blistn[(size_t)b] = (int64_t)w.rows.size();
-        pb.item_base[(size_t)b] = items;
-        pb.poff_base[(size_t)b] = poffs;
-        pb.toff_base[(size_t)b] = toffs;
-        pb.chunk_base[(size_t)b] = chunks;
-        pb.short_bytes[(size_t)b] = w.sbytes;
-        pb.long_bytes[(size_t)b] = w.lbytes;
-        rows += (int64_t)w.rows.size();
-        items += (int64_t)w.items.size();
-        chunks += (int64_t)w.chunks.size();
-        poffs += (int64_t)w.poff.size();
-        toffs += (int64_t)w.toff.size();
-        pb.tab_need = std::max(pb.tab_need, w.tsum);
-        pb.part_need = std::max(pb.part_need, w.psum);
-        pb.slot_need = std::max<int64_t>(pb.slot_need, w.slot);
-        pb.key_need = std::max(pb.key_need, w.ksum);
-        pb.cpo_need = std::max(pb.cpo_need, w.csum);
     }
-    pb.chunk_base[(size_t)nb] = chunks;
-    pb.binv.resize((size_t)std::max<int64_t>(rows, 1));
-    pb.bitems.resize((size_t)items);
-    pb.chunks.resize((size_t)chunks);
-    pb.poff.resize((size_t)poffs);
-    pb.toff.resize((size_t)toffs);
-#pragma omp parallel for schedule(dynamic, 1) if (nv >= kParallelRows)
+    tb0[(size_t)nb] = (int64_t)tiles.size();
+}
+
+void build_phase_lists(const int64_t* off, int nb, const std::vector<int64_t>& cnt,
+                       const std::vector<std::vector<BinRow>>& big, const std::vector<int64_t>& bverts,
+                       const std::vector<int64_t>& barcs, const nido_tuning& t, PhaseBins& pb)
+{
+    std::vector<BatchWork> W((size_t)nb);
+#pragma omp parallel for schedule(dynamic, 1) if (nb >= 8)
     for (int b = 0; b < nb; ++b) {
-        const BatchWork& w = W[(size_t)b];
-        std::copy(w.rows.begin(), w.rows.end(), pb.binv.begin() + pb.blist0[(size_t)b]);
-        std::copy(w.items.begin(), w.items.end(), pb.bitems.begin() + pb.item_base[(size_t)b]);
-        std::copy(w.chunks.begin(), w.chunks.end(), pb.chunks.begin() + pb.chunk_base[(size_t)b]);
-        std::copy(w.poff.begin(), w.poff.end(), pb.poff.begin() + pb.poff_base[(size_t)b]);
-        std::copy(w.toff.begin(), w.toff.end(), pb.toff.begin() + pb.toff_base[(size_t)b]);
+        BatchWork& w = W[(size_t)b];
+        for (int k = 0; k < NUM_BINS; ++k) w.cnt[k] = cnt[(size_t)b * NUM_BINS + k];
+        w.toff.push_back(0);
+        int64_t la = 0;
+        for (const BinRow& r : big[(size_t)b]) {
+            const int64_t d = off[r.v + 1] - off[r.v];
+            add_long_row(w, r.v, d, off, t);
+            la += d;
+        }
+        const int64_t lv = (int64_t)big[(size_t)b].size();
+        // the same integer-valued sums as the host builder's, exact in FP64
+        w.lbytes = 24.0 * (double)la + 40.0 * (double)lv;
+        w.sbytes = 24.0 * (double)(barcs[(size_t)b] - la) + 40.0 * (double)(bverts[(size_t)b] - lv);
+        order_items(w, off);
     }
+    merge_batches(W, nb, false, pb);
 }
 
 void build_row_lists(const int64_t* off, int64_t nv, std::vector<int>& small, std::vector<RowItem>& items)

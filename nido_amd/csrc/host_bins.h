@@ -11,8 +11,8 @@
 
 namespace nido {
 
-constexpr int64_t kSmallRow = 32;      // rows handled by one lane in the degree/modularity sweeps
-constexpr int64_t kItemArcs = 1024;    // longer rows are cut into slices of this many arcs
+constexpr int64_t kSmallRow = kSmallRowArcs;
+constexpr int64_t kItemArcs = kItemRowArcs;
 // Host work lists are built with threads only from this many vertices up:
 // smaller graphs are the serving loop's, whose requests already run side by
 // side, one host thread each.
@@ -40,10 +40,23 @@ struct PhaseBins {
     std::vector<int64_t> small_start, small_n;                          // [b * NUM_SMALL_BINS + k]
     std::vector<int64_t> blist0, blistn;                                // batch slices of binv
     std::vector<double> short_bytes, long_bytes;                        // algorithmic bytes per batch
+    int64_t rows = 0;                                                   // entries of binv
     int64_t tab_need = 0, part_need = 0, slot_need = 0, key_need = 0, cpo_need = 0;
 };
 void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::vector<int64_t>& parts,
                       const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb);
+
+// Tiles of the device bin builder (BinTile): batch by batch, each batch's
+// vertex ranges (graph by graph) cut into tiles of kBinTileVerts; tb0 gets
+// nb + 1 tile bounds.
+void bin_tiles(int nb, int G, const std::vector<int64_t>& parts, const std::vector<int64_t>& gparts,
+               std::vector<BinTile>& tiles, std::vector<int64_t>& tb0);
+// The lists of a phase whose binv the device built: cnt[b * NUM_BINS + k]
+// rows of batch b in bin k, big[b] the long rows of batch b (ascending id),
+// bverts / barcs the vertices and arcs of batch b.  Leaves pb.binv empty.
+void build_phase_lists(const int64_t* off, int nb, const std::vector<int64_t>& cnt,
+                       const std::vector<std::vector<BinRow>>& big, const std::vector<int64_t>& bverts,
+                       const std::vector<int64_t>& barcs, const nido_tuning& t, PhaseBins& pb);
 
 // Arcs streamed through a device window of `window` arcs: every batch b
 // is cut into units of consecutive rows holding at most `window` arcs

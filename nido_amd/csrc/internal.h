@@ -82,6 +82,33 @@ struct BinRow {
     int64_t e0;
 };
 
+// Work lists built on the device (bins.hip) for phases of many vertices.
+// A tile is up to kBinTileVerts consecutive vertices of one batch (of one
+// graph); the tiles of batch b are [tb0[b], tb0[b + 1]), in vertex order,
+// and tile t's count of bin k sits at cnt[at + k * stride] (at =
+// NUM_BINS * tb0[b] + t - tb0[b], stride = the batch's tile count), so one
+// exclusive scan of cnt orders the rows by (batch, bin, id).
+constexpr int64_t kBinTileVerts = 2048;
+constexpr int64_t kSmallRowArcs = 32;      // rows handled by one lane in the degree/modularity sweeps
+constexpr int64_t kItemRowArcs = 1024;     // longer rows are cut into slices of this many arcs
+struct BinTile {
+    int64_t v0, v1, at, stride;
+};
+void launch_bin_count(const BinTile* tiles, int64_t ntiles, const int64_t* off, int thread_max, int64_t big_min,
+                      int64_t* cnt, hipStream_t s);
+void launch_bin_scatter(const BinTile* tiles, int64_t ntiles, const int64_t* off, int thread_max, int64_t big_min,
+                        const int64_t* base, BinRow* binv, hipStream_t s);
+// out[b * NUM_BINS + k]: the first position of group (b, k); -1 for a batch
+// without tiles.
+void launch_bin_starts(const int64_t* base, const int64_t* tb0, int nb, int64_t* out, hipStream_t s);
+// Row lists over tiles of kBinTileVerts vertices: cs / ci receive each
+// tile's rows of at most kSmallRowArcs arcs and slices of the longer rows;
+// after their exclusive scans, the scatter writes small[] and items[] in
+// ascending vertex order.
+void launch_rowlist_count(const int64_t* off, int64_t nv, int64_t* cs, int64_t* ci, hipStream_t s);
+void launch_rowlist_scatter(const int64_t* off, int64_t nv, const int64_t* cs, const int64_t* ci, int* small,
+                            RowItem* items, hipStream_t s);
+
 // The first launch of a batch: the small bins plus the key fill of the
 // batch's partitioned long rows.
 struct SmallBins {
@@ -235,6 +262,11 @@ void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* C
 void launch_commit_tracked_list(const BinRow* list, int64_t n, int b, int64_t* C, int64_t* Cn, int* C32,
                                 const fx_t* k, fx_t* a, const int64_t* off, const ModDelta& D,
                                 const unsigned char* flag, hipStream_t s);
+// out[0..3] (zeroed by the caller) += hash sums of the arcs in both
+// directions: out[0] == out[1] and out[2] == out[3] when every arc (v, u, w)
+// has a reverse arc (u, v, w), the condition the incremental update needs.
+void launch_sym_hash(const int* src, const int* col, const double* w, int64_t ne, unsigned long long* out,
+                     hipStream_t s);
 // Batch b's delta (v0, v1 are its bounds for one graph; unused with D.gid).
 void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const int* C32, const int* col,
                       const double* w, const fx_t* a, double twom_d, const ModDelta& D, hipStream_t s);

@@ -300,6 +300,48 @@ __global__ void mod_acc_finish_kernel(const fx_t* __restrict__ acc, double twom_
     if (threadIdx.x == 0 && blockIdx.x == 0) *q = fx_to_double(acc[0]) / twom_d - fx_to_double(acc[1]);
 }
 
+// The multisets {(src, dst, w)} and {(dst, src, w)} of the arcs, compared
+// through sums (mod 2^64) of two keyed hashes of each form: equal when every
+// arc has a reverse arc of the same weight, and unequal otherwise except
+// with probability about 2^-128.  The mixer is the output function of
+// splitmix64 (Vigna's public-domain splitmix64.c; Stafford's "Mix13"
+// constants); the two keys are the first 64 fraction bits of sqrt(2) and
+// sqrt(3).
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ unsigned long long arc_hash(unsigned a, unsigned b, unsigned long long wb,
+                                                       unsigned long long key)
+{
+    return mix64(mix64((((unsigned long long)a << 32) | b) ^ key) ^ wb);
+}
+
+__global__ void __launch_bounds__(kBlock)
+sym_hash_kernel(const int* __restrict__ src, const int* __restrict__ col, const double* __restrict__ w, int64_t ne,
+                unsigned long long* __restrict__ out)
+{
+    constexpr unsigned long long k1 = 0x6a09e667f3bcc909ull, k2 = 0xbb67ae8584caa73bull;
+    unsigned long long h[4] = {0, 0, 0, 0};
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+        const unsigned a = (unsigned)src[e], b = (unsigned)col[e];
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(w[e]);
+        h[0] += arc_hash(a, b, wb, k1);
+        h[1] += arc_hash(b, a, wb, k1);
+        h[2] += arc_hash(a, b, wb, k2);
+        h[3] += arc_hash(b, a, wb, k2);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int msk = 32; msk > 0; msk >>= 1) h[i] += __shfl_xor(h[i], msk, 64);
+        if ((threadIdx.x & 63) == 0 && h[i] != 0) atomicAdd(out + i, h[i]);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
 mod_delta_kernel(int b, int64_t v0, int64_t v1, const int64_t* __restrict__ Cn, const int* __restrict__ C32,
                  const int* __restrict__ col, const double* __restrict__ w, const fx_t* __restrict__ a,
@@ -736,6 +778,12 @@ void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc
 {
     if (nv > 0) mod_terms_kernel<<<grid_for(nv, kBlock, kPartBlocks), kBlock, 0, s>>>(a, nv, twom_d, acc);
     mod_acc_finish_kernel<<<1, 64, 0, s>>>(acc, twom_d, q);
+}
+
+void launch_sym_hash(const int* src, const int* col, const double* w, int64_t ne, unsigned long long* out,
+                     hipStream_t s)
+{
+    if (ne > 0) sym_hash_kernel<<<grid_for(ne, kBlock, 2048), kBlock, 0, s>>>(src, col, w, ne, out);
 }
 
 void launch_mod_delta(int b, int64_t v0, int64_t v1, const int64_t* Cn, const int* C32, const int* col,

@@ -17,9 +17,11 @@ def run_oracle(off, col, w, nb, tol=1e-6, max_loops=500, nthreads=0, tol_phase=N
     return r
 
 
-def run_gpu(off, col, w, nb, tol=1e-6, max_loops=500):
+def run_gpu(off, col, w, nb, tol=1e-6, max_loops=500, **knobs):
     import nido_amd as na
     g = na.GraphGPU((off, col, w), nb)
+    if knobs:
+        g.set_tuning(**knobs)
     st = na.LouvainGPU(max_loops, tol, nb).run(g, record=True)
     phases = []
     for i in range(st.phases):

@@ -147,3 +147,13 @@ def test_batch_rollbacks_that_go_on():
     na = _na()
     print("case: mixed batch nb=4 tol=-1 max_loops=5", flush=True)
     _check(na, _mixed_graphs(na), 4, max_loops=5, tol=-1.0, tol_phase=1e-6)
+
+
+def test_batch_with_asymmetric_weights():
+    """One graph whose reverse arcs weigh something else: the handle sweeps
+    every iteration, and every graph still equals its oracle run."""
+    from test_gpu_parity import _asymmetric_graphs
+    na = _na()
+    graphs = _mixed_graphs(na)[:6] + _asymmetric_graphs(na)[:1]
+    print("case: mixed batch with an asymmetric graph nb=4", flush=True)
+    _check(na, graphs, 4)

@@ -405,3 +405,32 @@ def test_min_id_tie_break_closed_form_on_the_device():
         g.louvain_update(0)
         assert g.new_communities()[0] == want, (list(c), list(w))
         g.close()
+
+
+def _asymmetric_graphs(na):
+    """Graphs where an arc's reverse carries another weight, or is missing."""
+    rng = np.random.default_rng(11)
+    n = 5000
+    src = rng.integers(0, n // 4, 3000)
+    dst = rng.integers(0, n // 4, 3000)
+    keep = src != dst
+    e = np.unique(np.concatenate([np.stack([src[keep], dst[keep]], 1), np.stack([dst[keep], src[keep]], 1)]), axis=0)
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, e[:, 0] + 1, 1)
+    sparse = (np.cumsum(off), e[:, 1].astype(np.int64), rng.uniform(0.1, 1.0, len(e)))   # weights per arc
+    off, col, w = na.gen_rmat(11, 16, 0.57, 0.19, 0.19, True, 9).arrays()
+    rmat = (off, col, w * rng.uniform(0.5, 1.5, len(w)))                                   # weights per arc
+    one_way = (off, col, w.copy())
+    one_way[2][::7] = 0.0      # some arcs weigh nothing one way only
+    return [sparse, rmat, one_way]
+
+
+@pytest.mark.parametrize("nb", [1, 2, 4, 32])
+def test_asymmetric_weights(nb):
+    """The incremental modularity counts an arc whose state changes for
+    itself and for its reverse arc; where the reverse arc weighs something
+    else every iteration sweeps all arcs, and runs equal the oracle's."""
+    na = _na()
+    for i, (off, col, w) in enumerate(_asymmetric_graphs(na)):
+        print(f"case: asymmetric graph {i} nb={nb}", flush=True)
+        assert_runs_equal(run_oracle(off, col, w, nb), run_gpu(off, col, w, nb))
