@@ -1709,11 +1709,16 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     const int64_t base = off[0];
     const int64_t ne = off[nv] - base;
     bool nonneg = true;
-    std::vector<fx_t> twom_g(G > 1 ? (size_t)G : 0);
-    {
-        const int64_t whole[2] = {0, nv};
-        NIDO_TRY(check_csr(nv, off, col, w, G, G > 1 ? vptr : whole, &nonneg, G > 1 ? twom_g.data() : nullptr));
-    }
+    const int64_t whole[2] = {0, nv};
+    const int64_t* gv = G > 1 ? vptr : whole;
+    // Inputs large enough for the pinned staging buffers are checked while
+    // they pass through them (ids and weights are read there anyway); the
+    // full host check runs first for smaller ones, and after the upload when
+    // the quick checks saw anything it has to decide exactly (a bad arc, a
+    // negative weight, a graph whose total of |w| nears the bound).
+    const bool fused = !(window > 0 && ne > window) && ne > 0 && staged_path(sizeof(double) * (size_t)ne);
+    if (fused) NIDO_TRY(check_offsets(nv, off, col, w));
+    else NIDO_TRY(check_csr(nv, off, col, w, G, gv, &nonneg, nullptr));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         (void)hipGetLastError();
@@ -1721,7 +1726,6 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     }
     nido_graph* g = new nido_graph;
     g->nb = nb;
-    g->nonneg = nonneg;
     default_tuning(g->tuning);
     auto fail = [&](int rc) {
         nido_graph_destroy(g);
@@ -1782,24 +1786,30 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     launch_fill_fx(g->d_acc, 4, s);
     // Clustering::commIds_ starts as the identity (clustering.cpp:4-10)
     launch_iota_i64(g->d_clus, nv, s);
+    ArcCheck chk;
+    std::vector<int64_t> gstart((size_t)G + 1);
+    for (int i = 0; i <= G; ++i) gstart[(size_t)i] = off[gv[i]] - base;
+    chk.gstart = gstart.data();
+    chk.vptr = gv;
+    chk.G = G;
     if (staged_h2d(g->d_off, g->h_off.data(), sizeof(int64_t) * (size_t)(nv + 1), s) != hipSuccess ||
-        (ne > 0 && !windowed0 && staged_h2d_ids(g->d_col32, col + base, ne, s) != hipSuccess) ||
-        (ne > 0 && !windowed0 && staged_h2d(g->d_w, w + base, sizeof(double) * (size_t)ne, s) != hipSuccess))
+        (ne > 0 && !windowed0 && staged_h2d_ids(g->d_col32, col + base, ne, s, fused ? &chk : nullptr) != hipSuccess) ||
+        (ne > 0 && !windowed0 &&
+         (fused ? staged_h2d_weights(g->d_w, w + base, ne, s, chk)
+                : staged_h2d(g->d_w, w + base, sizeof(double) * (size_t)ne, s)) != hipSuccess))
         return fail(set_error(NIDO_ERR_HIP, "upload failed"));
+    if (fused) {
+        bool exact = chk.bad.load() != INT64_MAX || chk.neg.load();
+        for (double x : chk.wabs) exact |= !(x <= 0x1p61);   // FP64 sums: far from 2^62 below this
+        if (exact && (rc = check_csr(nv, off, col, w, G, gv, &nonneg, nullptr))) return fail(rc);
+    }
+    g->nonneg = nonneg;
     // ids crossed the host link as int32 (all below nv < 2^31 - 1, checked)
     if (!windowed0) launch_widen_ids(g->d_col32, g->d_col, ne, s);
     if (G > 1) {
         g->G = G;
         g->vb.assign(vptr, vptr + G + 1);
         g->vb0 = g->vb;
-        g->twom_g.swap(twom_g);                           // 2m per graph, exactly (graph_cuda.cu:1855)
-        g->twom_dg.assign((size_t)G, 0.0);
-        std::vector<double> mg(2 * (size_t)G), tw((size_t)G);
-        for (int i = 0; i < G; ++i) {
-            tw[(size_t)i] = g->twom_dg[(size_t)i] = fx_to_double(g->twom_g[(size_t)i]);
-            mg[2 * (size_t)i] = tw[(size_t)i] * 0.5;
-            mg[2 * (size_t)i + 1] = 1.0 / mg[2 * (size_t)i];   // unused while the graph has no weight
-        }
         if ((rc = dev_alloc(&g->d_gid, nv)) || (rc = dev_alloc(&g->d_gid0, nv)) ||
             (rc = dev_alloc(&g->d_vb, G + 1)) || (rc = dev_alloc(&g->d_vbn, G + 1)) ||
             (rc = dev_alloc(&g->d_mg, 2 * (int64_t)G)) || (rc = dev_alloc(&g->d_twomg, G)) || (rc = dev_alloc(&g->d_act, G)) ||
@@ -1807,9 +1817,8 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
             (rc = dev_alloc(&g->d_gq, (int64_t)G + 1)) ||
             (rc = dev_alloc(&g->d_moved, kMovedSlots)))
             return fail(rc);
-        if (hipMemcpyAsync(g->d_vb, g->vb.data(), sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipMemcpyAsync(g->d_mg, mg.data(), sizeof(double) * 2 * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipMemcpyAsync(g->d_twomg, tw.data(), sizeof(double) * (size_t)G, hipMemcpyHostToDevice, s) != hipSuccess)
+        if (hipMemcpyAsync(g->d_vb, g->vb.data(), sizeof(int64_t) * (size_t)(G + 1), hipMemcpyHostToDevice, s) !=
+            hipSuccess)
             return fail(set_error(NIDO_ERR_HIP, "upload failed"));
         launch_fill_gid(g->d_vb, G, g->d_gid, s);
         launch_fill_gid(g->d_vb, G, g->d_gid0, s);
@@ -1817,6 +1826,29 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
             return fail(set_error(NIDO_ERR_HIP, "graph id set-up failed"));
     }
     if ((rc = setup_phase(g))) return fail(rc);
+    if (G > 1) {
+        // 2m per graph, exactly (graph_cuda.cu:1855): the sum of its k_v
+        g->twom_g.assign((size_t)G, 0);
+        launch_graph_sums(g->d_k, g->d_vb, G, g->d_gacc, s);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(g->twom_g.data(), g->d_gacc, sizeof(fx_t) * (size_t)G, hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail(set_error(NIDO_ERR_HIP, "per-graph 2m failed"));
+        g->twom_dg.assign((size_t)G, 0.0);
+        std::vector<double> mg(2 * (size_t)G), tw((size_t)G);
+        for (int i = 0; i < G; ++i) {
+            tw[(size_t)i] = g->twom_dg[(size_t)i] = fx_to_double(g->twom_g[(size_t)i]);
+            mg[2 * (size_t)i] = tw[(size_t)i] * 0.5;
+            mg[2 * (size_t)i + 1] = 1.0 / mg[2 * (size_t)i];   // unused while the graph has no weight
+        }
+        if (hipMemcpyAsync(g->d_mg, mg.data(), sizeof(double) * 2 * (size_t)G, hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(g->d_twomg, tw.data(), sizeof(double) * (size_t)G, hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail(set_error(NIDO_ERR_HIP, "upload failed"));
+    }
     if (!g->windowed && ne > 0) {
         fx_t* d_h = nullptr;   // four 64-bit sums
         unsigned long long h[4];

@@ -11,6 +11,9 @@ namespace nido {
 // every weight >= 0; twom (if not null, G entries): every graph's exact 2m.
 int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int G,
               const int64_t* vptr, bool* nonneg, fx_t* twom = nullptr);
+// Its first part: offsets non-decreasing, col and w present when there are
+// arcs (the rest is checked while large inputs are uploaded).
+int check_offsets(int64_t nv, const int64_t* off, const int64_t* col, const double* w);
 // The greedy colouring of Graph::coloring run by one thread (coloring.hip):
 // colors[i] = smallest colour not held by a neighbour j < i with
 // r[j] >= r[i]; off[nv + 1], col[off[nv]] (ids in [0, nv)), host arrays.

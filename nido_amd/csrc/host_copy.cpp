@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -21,7 +22,87 @@ constexpr int64_t kDefaultMinBytes = int64_t(1) << 30;
 constexpr int64_t kDefaultChunkBytes = int64_t(64) << 20;
 std::atomic<int64_t> g_min_bytes{kDefaultMinBytes};
 std::atomic<int64_t> g_chunk_bytes{kDefaultChunkBytes};
+
+void atomic_min(std::atomic<int64_t>& a, int64_t x)
+{
+    int64_t cur = a.load();
+    while (x < cur && !a.compare_exchange_weak(cur, x)) {
+    }
+}
+
+// Calls fn(g, j0, j1) for the pieces of arcs [j0, j1) inside graph g, in
+// order (graphs without arcs skipped).
+template <typename Fn>
+void by_graph(const ArcCheck& chk, int64_t j0, int64_t j1, Fn&& fn)
+{
+    if (j0 >= j1) return;
+    int g = (int)(std::upper_bound(chk.gstart, chk.gstart + chk.G + 1, j0) - chk.gstart) - 1;
+    for (int64_t j = j0; j < j1; ++g) {
+        const int64_t e = std::min(j1, chk.gstart[g + 1]);
+        if (e > j) fn(g, j, e);
+        j = std::max(j, e);
+    }
+}
+
+// Arcs [at, at + m) of src narrowed into b, their columns checked.
+void narrow_checked(int* b, const int64_t* src, int64_t at, int64_t m, ArcCheck& chk)
+{
+#pragma omp parallel
+    {
+        const int nt = omp_get_num_threads(), tid = omp_get_thread_num();
+        by_graph(chk, at + m * tid / nt, at + m * (tid + 1) / nt, [&](int g, int64_t j0, int64_t j1) {
+            const int64_t lo = chk.vptr[g], hi = chk.vptr[g + 1];
+            bool out = false;
+            for (int64_t j = j0; j < j1; ++j) {
+                const int64_t c = src[j];
+                b[j - at] = (int)c;
+                out |= c < lo || c >= hi;
+            }
+            if (out)
+                for (int64_t j = j0; j < j1; ++j)
+                    if (src[j] < lo || src[j] >= hi) {
+                        atomic_min(chk.bad, j);
+                        break;
+                    }
+        });
+    }
+}
+
+// Weights [at, at + m) of src copied into b, checked, |w| summed per graph.
+void weights_checked(double* b, const double* src, int64_t at, int64_t m, ArcCheck& chk)
+{
+#pragma omp parallel
+    {
+        const int nt = omp_get_num_threads(), tid = omp_get_thread_num();
+        by_graph(chk, at + m * tid / nt, at + m * (tid + 1) / nt, [&](int g, int64_t j0, int64_t j1) {
+            bool bad = false, neg = false;
+            double sa = 0.0;
+            for (int64_t j = j0; j < j1; ++j) {
+                const double x = src[j];
+                b[j - at] = x;
+                const double ax = std::fabs(x);
+                bad |= !(ax < 0x1p62);   // NaN, infinite or too large
+                neg |= x < 0.0;
+                sa += ax < 0x1p62 ? ax : 0.0;
+            }
+            if (bad)
+                for (int64_t j = j0; j < j1; ++j)
+                    if (!(std::fabs(src[j]) < 0x1p62)) {
+                        atomic_min(chk.bad, j);
+                        break;
+                    }
+            if (neg) chk.neg.store(true);
+            std::lock_guard<std::mutex> lk(chk.mu);
+            chk.wabs[(size_t)g] += sa;
+        });
+    }
+}
 }  // namespace
+
+bool staged_path(size_t bytes)
+{
+    return bytes >= (size_t)g_min_bytes.load();
+}
 
 hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
 {
@@ -62,15 +143,19 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
     return e;
 }
 
-hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s)
+hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s, ArcCheck* chk)
 {
     const int64_t kChunk = g_chunk_bytes.load() / (int64_t)sizeof(int);   // ids per pinned buffer
     if (n * (int64_t)sizeof(int) < g_min_bytes.load()) {   // one pageable copy, no pinned allocation
         std::vector<int> t((size_t)n);
-        // threads only for graphs larger than a serving request, whose
-        // requests already run side by side
+        if (chk) {
+            narrow_checked(t.data(), src, 0, n, *chk);
+        } else {
+            // threads only for graphs larger than a serving request, whose
+            // requests already run side by side
 #pragma omp parallel for schedule(static) if (n >= (int64_t(1) << 26))
-        for (int64_t j = 0; j < n; ++j) t[(size_t)j] = (int)src[j];
+            for (int64_t j = 0; j < n; ++j) t[(size_t)j] = (int)src[j];
+        }
         hipError_t e = hipMemcpyAsync(dst, t.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s);
         return e != hipSuccess ? e : hipStreamSynchronize(s);
     }
@@ -87,9 +172,41 @@ hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s
         if (e != hipSuccess) break;
         int* b = buf[i];
         const int64_t* in = src + at;
+        if (chk) {
+            narrow_checked(b, src, at, m, *chk);
+        } else {
 #pragma omp parallel for schedule(static) if (m >= (int64_t(1) << 20))
-        for (int64_t j = 0; j < m; ++j) b[j] = (int)in[j];
+            for (int64_t j = 0; j < m; ++j) b[j] = (int)in[j];
+        }
         e = hipMemcpyAsync(dst + at, b, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(done[i], s);
+    }
+    const hipError_t w = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = w;
+    for (int i = 0; i < 2; ++i) {
+        if (done[i]) (void)hipEventDestroy(done[i]);
+        if (buf[i]) (void)hipHostFree(buf[i]);
+    }
+    return e;
+}
+
+hipError_t staged_h2d_weights(double* dst, const double* src, int64_t n, hipStream_t s, ArcCheck& chk)
+{
+    const int64_t kChunk = g_chunk_bytes.load() / (int64_t)sizeof(double);
+    chk.wabs.assign((size_t)chk.G, 0.0);
+    double* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), sizeof(double) * (size_t)kChunk);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+    }
+    for (int64_t at = 0, i = 0; e == hipSuccess && at < n; at += kChunk, i ^= 1) {
+        const int64_t m = std::min(kChunk, n - at);
+        e = hipEventSynchronize(done[i]);      // the buffer's previous copy has landed
+        if (e != hipSuccess) break;
+        weights_checked(buf[i], src, at, m, chk);
+        e = hipMemcpyAsync(dst + at, buf[i], sizeof(double) * (size_t)m, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(done[i], s);
     }
     const hipError_t w = hipStreamSynchronize(s);

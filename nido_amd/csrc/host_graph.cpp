@@ -596,17 +596,25 @@ int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* 
 // ---- checks and sums used by the device handle's constructors ------------
 namespace nido {
 
-int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int G,
-              const int64_t* vptr, bool* nonneg, fx_t* twom)
+int check_offsets(int64_t nv, const int64_t* off, const int64_t* col, const double* w)
 {
-    const int64_t base = off[0];
     int64_t bad_off = INT64_MAX;
 #pragma omp parallel for reduction(min : bad_off) schedule(static)
     for (int64_t i = 0; i < nv; ++i)
         if (off[i + 1] < off[i] && i < bad_off) bad_off = i;
     if (bad_off != INT64_MAX) return set_error(NIDO_ERR_INVALID, "offsets decrease at %lld", (long long)bad_off);
-    const int64_t ne = off[nv] - base;
-    if (ne > 0 && (!col || !w)) return set_error(NIDO_ERR_INVALID, "null cols/weights");
+    if (off[nv] > off[0] && (!col || !w)) return set_error(NIDO_ERR_INVALID, "null cols/weights");
+    return NIDO_OK;
+}
+
+int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* w, int G,
+              const int64_t* vptr, bool* nonneg, fx_t* twom)
+{
+    const int64_t base = off[0];
+    {
+        const int rc = check_offsets(nv, off, col, w);
+        if (rc != NIDO_OK) return rc;
+    }
     // first bad arc of graph g: column outside the graph, weight not finite
     // or too large; and the graph's exact total of |fx(w)| (saturating at
     // 2^126 units) and of fx(w).  One parallel loop over the graphs when

@@ -298,6 +298,8 @@ void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64
                          int64_t* out, hipStream_t s);
 // gid[i] = g for i in [vb[g], vb[g+1]).
 void launch_fill_gid(const int64_t* vb, int G, int* gid, hipStream_t s);
+// out[g] = sum of k over [vb[g], vb[g+1]), exactly (one workgroup per graph).
+void launch_graph_sums(const fx_t* k, const int64_t* vb, int G, fx_t* out, hipStream_t s);
 
 // Exclusive scan of n int64 values (out may alias in); writes the total to
 // *total (device) when total is non-null.

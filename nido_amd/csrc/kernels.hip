@@ -662,6 +662,19 @@ gather_bases_kernel(const int64_t* __restrict__ newid, const int64_t* __restrict
         out[g] = vb[g] < nv ? newid[vb[g]] : *total;
 }
 
+// out[g] = the exact sum of k over graph g's vertices; one workgroup per graph.
+__global__ void __launch_bounds__(kBlock)
+graph_sums_kernel(const fx_t* __restrict__ k, const int64_t* __restrict__ vb, int G, fx_t* __restrict__ out)
+{
+    __shared__ fx_t red[kBlock / 64];
+    for (int g = blockIdx.x; g < G; g += gridDim.x) {
+        fx_t t = 0;
+        for (int64_t v = vb[g] + threadIdx.x; v < vb[g + 1]; v += kBlock) t += k[v];
+        t = block_sum_fx(t, red);
+        if (threadIdx.x == 0) out[g] = t;
+    }
+}
+
 // gid[i] = g for i in [vb[g], vb[g+1]); one workgroup per graph.
 __global__ void __launch_bounds__(kBlock)
 fill_gid_kernel(const int64_t* __restrict__ vb, int G, int* __restrict__ gid)
@@ -880,6 +893,11 @@ void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src,
         mod_seg_kernel<<<grid_for(nitems, 1, 8192), kBlock, 0, s>>>(items, nitems, src, col, w, C, a,
                                                                     twom_g, acc, act, term);
     mod_seg_finalize_kernel<<<grid_for(G, kBlock), kBlock, 0, s>>>(acc, twom_g, G, q, act, moved);
+}
+
+void launch_graph_sums(const fx_t* k, const int64_t* vb, int G, fx_t* out, hipStream_t s)
+{
+    if (G > 0) graph_sums_kernel<<<grid_for(G, 1, 8192), kBlock, 0, s>>>(k, vb, G, out);
 }
 
 void launch_gather_bases(const int64_t* newid, const int64_t* total, const int64_t* vb, int G, int64_t nv,

@@ -43,6 +43,66 @@ def test_pinned_staged_upload_with_many_chunks():
         na.set_upload_staging(4096, 1000)          # chunks are multiples of 4096
 
 
+def test_checks_made_while_staging_match_the_host_checks():
+    """Inputs that take the pinned staging path are checked while they pass
+    through it (arc columns against their graph, weights finite and below
+    2^62, a bound on each graph's total); forced at small sizes, every input
+    the host check refuses is refused with the same message, and inputs
+    that need its exact totals (negative weights, totals near 2^62) still
+    get them."""
+    na = _na()
+    off = np.array([0, 1, 3, 4])
+    col = np.array([1, 0, 2, 1])
+    G = 100
+    vptr100 = np.arange(0, 2 * G + 1, 2)
+    off100 = np.arange(0, 2 * G + 1)
+    col100 = np.arange(2 * G) ^ 1
+    bad_col100 = col100.copy()
+    bad_col100[2 * 80] = 2 * 80 + 5
+    bad_w100 = np.ones(2 * G)
+    bad_w100[2 * 70 + 1] = np.inf
+    vptr2 = np.array([0, 3, 6])
+    off2 = np.array([0, 1, 3, 4, 5, 7, 8])
+    col2 = np.array([1, 0, 2, 1, 4, 3, 5, 4])
+    cases = {
+        "column out of range": lambda: na.GraphGPU((off, np.array([1, 0, 7, 1]), np.ones(4)), 2),
+        "NaN weight": lambda: na.GraphGPU((off, col, np.array([1.0, np.nan, 1.0, 1.0])), 2),
+        "weight 2^62": lambda: na.GraphGPU((off, col, np.array([1.0, 2.0 ** 62, 1.0, 1.0])), 2),
+        "total 2^62": lambda: na.GraphGPU((off, col, np.full(4, 3e18)), 2),
+        "cancelling negatives": lambda: na.GraphGPU(
+            (off, col, np.array([1.0, 1.0, -1.0 + 2.0 ** -40, -1.0 + 2.0 ** -40])), 2),
+        "graph total 2^62": lambda: na.BatchGPU(
+            csr=(vptr2, off2, col2, np.concatenate([np.ones(4), np.full(4, 3e18)])), nbatches=2),
+        "arc leaves its graph": lambda: na.BatchGPU(
+            csr=(vptr2, off2, np.array([1, 0, 2, 1, 4, 3, 5, 0]), np.ones(8)), nbatches=2),
+        "first bad graph weight": lambda: na.BatchGPU(csr=(vptr100, off100, bad_col100, bad_w100), nbatches=2),
+        "first bad graph column": lambda: na.BatchGPU(
+            csr=(vptr100, off100, bad_col100, np.ones(2 * G)), nbatches=2),
+    }
+
+    def refusal(fn):
+        try:
+            h = fn()
+        except na.NidoError as e:
+            return str(e)
+        h.close()
+        return None
+
+    expected = {k: refusal(f) for k, f in cases.items()}
+    assert all(expected.values()), expected
+    na.set_upload_staging(16, 4096)   # weights of two or more arcs take the staging path
+    try:
+        for k, f in cases.items():
+            print(f"case: staged checks, {k}", flush=True)
+            assert refusal(f) == expected[k], k
+        # negative weights that pass: the exact check runs after the upload
+        mild = np.array([1.0, 1.0, -0.5, -0.5])
+        print("case: staged checks, mild negative weights", flush=True)
+        assert_runs_equal(run_oracle(off, col, mild, 2), run_gpu(off, col, mild, 2))
+    finally:
+        na.set_upload_staging(0, 0)
+
+
 def test_tuning_switches_reject_values_out_of_range():
     na = _na()
     off, col, w = na.gen_sbm(500, 50, 0.2, 0.01, True, 3).arrays()
