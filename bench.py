@@ -217,6 +217,8 @@ def run_batched(args):
     bg.close()
     return {
         "value": round(batch_gps, 3), "unit": "graphs/s", "ms_per_step": round(t_batch * 1e3, 2),
+        "split_s": {"create": round(t1 - t0, 3), "iterations": round(st.t_loop, 3),
+                    "aggregation": round(st.t_agg, 3)},
         "config": {"workload": f"{G} planted-partition graphs, n log-uniform [256, 65536], blocks of 64, "
                                f"intra-degree ~12, inter-degree ~2, U(0,1] weights, seeds 1000+i, -b {nb} "
                                f"-t {tol:g}, full runs (all phases)",
