@@ -1202,34 +1202,56 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
     hipStream_t s = g->stream;
     const int64_t nv = g->nv;
     int64_t *arcs = nullptr, *toff = nullptr, *cnt = nullptr, *noff = nullptr, *ncol = nullptr;
-    double* nw = nullptr;
+    int64_t *cpos = nullptr, *mstart = nullptr;
+    unsigned long long* mfill = nullptr;
+    int *members = nullptr, *ckey = nullptr;
+    double *nw = nullptr, *cw = nullptr;
     int* tkey = nullptr;
     fx_t* tval = nullptr;
     void* tmp = nullptr;
     auto cleanup = [&]() {
         dev_free(arcs); dev_free(toff); dev_free(cnt); dev_free(noff);
         dev_free(ncol); dev_free(nw); dev_free(tkey); dev_free(tval);
+        dev_free(cpos); dev_free(mstart); dev_free(mfill); dev_free(members); dev_free(ckey); dev_free(cw);
         raw_free(tmp);
         tmp = nullptr;
     };
     int rc = NIDO_OK;
     do {
         if ((rc = dev_alloc(&arcs, K + 1)) || (rc = dev_alloc(&toff, K + 1)) ||
-            (rc = dev_alloc(&cnt, K + 1)) || (rc = dev_alloc(&noff, K + 1)))
+            (rc = dev_alloc(&cnt, K + 1)) || (rc = dev_alloc(&noff, K + 1)) ||
+            (rc = dev_alloc(&cpos, K + 1)) || (rc = dev_alloc(&mstart, K + 1)) ||
+            (rc = dev_alloc(&mfill, std::max<int64_t>(K, 1))) || (rc = dev_alloc(&members, std::max<int64_t>(nv, 1))))
             break;
         if ((rc = raw_alloc(&tmp, scan_temp_bytes(K + 1)))) break;
+        // per coarse row: its arcs and members; rows of few arcs are
+        // aggregated in LDS (compact entries at cpos), the others in HBM
+        // tables (toff)
         launch_fill_i64(arcs, 0, K + 1, s);
-        launch_member_counts(g->d_C, newid, g->d_off, nv, nullptr, arcs, s, gid, keep);
-        launch_table_sizes(arcs, K, arcs, s);                      // in place: arcs -> table size
-        launch_exclusive_scan(arcs, toff, K, toff + K, tmp, s);
-        int64_t T = 0;
+        launch_fill_i64(cnt, 0, K + 1, s);
+        launch_fill_i64(reinterpret_cast<int64_t*>(mfill), 0, std::max<int64_t>(K, 1), s);
+        launch_member_counts(g->d_C, newid, g->d_off, nv, cnt, arcs, s, gid, keep);
+        launch_exclusive_scan(cnt, mstart, K, mstart + K, tmp, s);
+        launch_coarse_sizes(arcs, K, toff, cpos, s);
+        launch_exclusive_scan(toff, toff, K, toff + K, tmp, s);
+        launch_exclusive_scan(cpos, cpos, K, cpos + K, tmp, s);
+        int64_t T = 0, Tc = 0;
         HIP_TRY(hipMemcpyAsync(&T, toff + K, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&Tc, cpos + K, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        if ((rc = dev_alloc(&tkey, T)) || (rc = dev_alloc(&tval, T))) break;
-        launch_fill_i32(tkey, -1, T, s);
-        launch_fill_fx(tval, T, s);
-        launch_coarse_insert(g->d_off, g->d_col, g->d_w, g->d_C, newid, nv, toff, tkey, tval, s, gid, keep);
-        launch_coarse_count(toff, tkey, K, cnt, s);
+        if ((rc = dev_alloc(&tkey, std::max<int64_t>(T, 1))) || (rc = dev_alloc(&tval, std::max<int64_t>(T, 1))) ||
+            (rc = dev_alloc(&ckey, std::max<int64_t>(Tc, 1))) || (rc = dev_alloc(&cw, std::max<int64_t>(Tc, 1))))
+            break;
+        if (T > 0) {
+            launch_fill_i32(tkey, -1, T, s);
+            launch_fill_fx(tval, T, s);
+        }
+        launch_member_scatter(g->d_C, newid, nv, mstart, mfill, members, s, gid, keep);
+        launch_coarse_lds(arcs, mstart, members, g->d_off, g->d_col, g->d_w, g->d_C, newid, K, cpos, ckey, cw, cnt, s);
+        if (T > 0) {
+            launch_coarse_insert(g->d_off, g->d_col, g->d_w, g->d_C, newid, nv, toff, tkey, tval, s, gid, keep);
+            launch_coarse_count(toff, tkey, K, cnt, s);
+        }
         launch_exclusive_scan(cnt, noff, K, noff + K, tmp, s);
         HIP_TRY(hipGetLastError());
         std::vector<int64_t> hoff((size_t)K + 1);
@@ -1237,7 +1259,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         HIP_TRY(hipStreamSynchronize(s));
         const int64_t nne = hoff[(size_t)K];
         if ((rc = dev_alloc(&ncol, nne)) || (rc = dev_alloc(&nw, nne))) break;
-        launch_coarse_emit(toff, tkey, tval, K, noff, ncol, nw, s);
+        launch_coarse_emit(toff, tkey, tval, K, noff, ncol, nw, cpos, ckey, cw, s);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(s));
         free_graph_arrays(g);

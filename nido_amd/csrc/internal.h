@@ -317,7 +317,19 @@ void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const
 void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t* off, int64_t nv,
                           int64_t* cnt, int64_t* arcs, hipStream_t s, const int* gid = nullptr,
                           const unsigned char* keep = nullptr);
-void launch_table_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, hipStream_t s);
+// Coarse rows of at most kLdsRowArcs arcs are aggregated in LDS, one
+// workgroup each, longer ones in HBM hash tables: tsz[i] table slots (0 for
+// an LDS row), csz[i] compact entries (0 for a table row).
+constexpr int64_t kLdsRowArcs = 1024;
+void launch_coarse_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, int64_t* csz, hipStream_t s);
+// members[mstart[i] ...]: the vertices of coarse row i (mfill zeroed, K).
+void launch_member_scatter(const int64_t* C, const int64_t* newid, int64_t nv, const int64_t* mstart,
+                           unsigned long long* mfill, int* members, hipStream_t s, const int* gid = nullptr,
+                           const unsigned char* keep = nullptr);
+// The LDS rows: distinct targets at ckey/cw[cpos[i] ...), their count in cnt[i].
+void launch_coarse_lds(const int64_t* arcs, const int64_t* mstart, const int* members, const int64_t* off,
+                       const int64_t* col, const double* w, const int64_t* C, const int64_t* newid, int64_t K,
+                       const int64_t* cpos, int* ckey, double* cw, int64_t* cnt, hipStream_t s);
 void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
                           const int64_t* newid, int64_t nv, const int64_t* toff, int* tkey,
                           fx_t* tval, hipStream_t s, const int* gid = nullptr,
@@ -325,6 +337,7 @@ void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* 
 void launch_coarse_count(const int64_t* toff, const int* tkey, int64_t K, int64_t* cnt,
                          hipStream_t s);
 void launch_coarse_emit(const int64_t* toff, const int* tkey, const fx_t* tval, int64_t K,
-                        const int64_t* noff, int64_t* ncol, double* nw, hipStream_t s);
+                        const int64_t* noff, int64_t* ncol, double* nw, const int64_t* cpos, const int* ckey,
+                        const double* cw, hipStream_t s);
 
 }  // namespace nido

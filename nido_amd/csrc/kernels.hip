@@ -485,18 +485,144 @@ member_counts_kernel(const int64_t* __restrict__ C, const int64_t* __restrict__ 
     }
 }
 
+// Coarse row i with arcs[i] <= kLdsRowArcs is aggregated in LDS and needs
+// csz[i] = min(arcs, K) compact entries; a longer one gets an HBM hash table
+// of tsz[i] slots (a power of two >= twice its possible keys).
 __global__ void __launch_bounds__(kBlock)
-table_sizes_kernel(const int64_t* __restrict__ arcs, int64_t K, int64_t* __restrict__ tsz)
+coarse_sizes_kernel(const int64_t* __restrict__ arcs, int64_t K, int64_t* __restrict__ tsz,
+                    int64_t* __restrict__ csz)
 {
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < K;
          i += (int64_t)gridDim.x * kBlock) {
-        int64_t need = arcs[i] < K ? arcs[i] : K;
+        const int64_t need = arcs[i] < K ? arcs[i] : K;
         int64_t s = 0;
-        if (need > 0) {
+        if (arcs[i] > kLdsRowArcs) {
             s = 2;
             while (s < 2 * need) s <<= 1;
         }
         tsz[i] = s;
+        csz[i] = arcs[i] > kLdsRowArcs ? 0 : need;
+    }
+}
+
+// members[mstart[i] + r] = the vertices of coarse row i, in no set order.
+__global__ void __launch_bounds__(kBlock)
+member_scatter_kernel(const int64_t* __restrict__ C, const int64_t* __restrict__ newid, int64_t nv,
+                      const int64_t* __restrict__ mstart, unsigned long long* __restrict__ mfill,
+                      int* __restrict__ members, const int* __restrict__ gid, const unsigned char* __restrict__ keep)
+{
+    for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kBlock) {
+        if (gid && !keep[gid[v]]) continue;
+        const int64_t i = newid[C[v]];
+        members[mstart[i] + (int64_t)atomicAdd(mfill + i, 1ull)] = (int)v;
+    }
+}
+
+// Coarse rows of at most kLdsRowArcs arcs, one workgroup each: the arcs of
+// its members (their concatenation, 256 at a time, each thread locating its
+// member by binary search over the scanned lengths) go into an LDS table
+// keyed by the coarse target, with exact Q64.64 sums; the arcs into the row
+// itself are summed per wavefront first and inserted once.  The row's
+// distinct targets land compactly at ckey/cw[cpos[i] ...) and their count
+// in cnt[i].  Keys whose weights sum to zero are kept, as in the HBM tables.
+constexpr int kLdsSlots = 2 * kLdsRowArcs;
+__global__ void __launch_bounds__(kBlock)
+coarse_lds_kernel(const int64_t* __restrict__ arcs, const int64_t* __restrict__ mstart,
+                  const int* __restrict__ members, const int64_t* __restrict__ off,
+                  const int64_t* __restrict__ col, const double* __restrict__ w, const int64_t* __restrict__ C,
+                  const int64_t* __restrict__ newid, int64_t K, const int64_t* __restrict__ cpos,
+                  int* __restrict__ ckey, double* __restrict__ cw, int64_t* __restrict__ cnt)
+{
+    __shared__ int key[kLdsSlots];
+    __shared__ unsigned long long lo[kLdsSlots], hi[kLdsSlots];
+    __shared__ int klist[kLdsRowArcs];
+    __shared__ int s_pre[kBlock], wtot[kBlock / 64];
+    __shared__ int64_t s_e0[kBlock];
+    __shared__ int s_nk;
+    const int lane = threadIdx.x & 63;
+    for (int64_t i = blockIdx.x; i < K; i += gridDim.x) {
+        const int64_t a = arcs[i];
+        if (a > kLdsRowArcs) continue;   // uniform: an HBM-table row
+        const int64_t need = a < K ? a : K;
+        int lg = 6;
+        while ((1ll << lg) < 2 * need) ++lg;
+        const int ts = 1 << lg;          // <= kLdsSlots
+        for (int s = threadIdx.x; s < ts; s += kBlock) {
+            key[s] = -1;
+            lo[s] = 0;
+            hi[s] = 0;
+        }
+        if (threadIdx.x == 0) s_nk = 0;
+        __syncthreads();
+        // ends: at most need <= ts / 2 distinct keys
+        auto insert = [&](int j, fx_t x) {
+            unsigned h = ((unsigned)j * kHashMul) >> (32 - lg);
+            for (;;) {
+                const int kk = key[h];
+                if (kk == j) break;
+                if (kk < 0) {
+                    const int prev = atomicCAS(&key[h], -1, j);
+                    if (prev < 0) {
+                        klist[atomicAdd(&s_nk, 1)] = (int)h;
+                        break;
+                    }
+                    if (prev == j) break;
+                }
+                h = (h + 1) & (unsigned)(ts - 1);
+            }
+            fx_atomic_add(&lo[h], &hi[h], x);
+        };
+        fx_t self = 0;
+        bool any_self = false;           // uniform in the wavefront
+        const int64_t m1 = mstart[i + 1];
+        for (int64_t m0 = mstart[i]; m0 < m1; m0 += kBlock) {   // uniform
+            const int64_t mi = m0 + threadIdx.x;
+            int d = 0;
+            int64_t e0 = 0;
+            if (mi < m1) {
+                const int v = members[mi];
+                e0 = off[v];
+                d = (int)(off[v + 1] - e0);   // <= a <= kLdsRowArcs
+            }
+            int tot;
+            const int pre = block_excl_scan_i32(d, wtot, &tot);
+            s_pre[threadIdx.x] = pre;
+            s_e0[threadIdx.x] = e0;
+            __syncthreads();
+            const int nm = m1 - m0 < kBlock ? (int)(m1 - m0) : kBlock;
+            int top = 1;
+            while (2 * top < nm) top *= 2;
+            for (int p0 = 0; p0 < tot; p0 += kBlock) {   // uniform
+                const int p = p0 + threadIdx.x;
+                int j = -1;
+                fx_t x = 0;
+                if (p < tot) {
+                    int l = 0;                 // the last member whose arcs start at or before p
+                    for (int st = top; st > 0; st >>= 1)
+                        if (l + st < nm && s_pre[l + st] <= p) l += st;
+                    const int64_t e = s_e0[l] + (p - s_pre[l]);
+                    j = (int)newid[C[col[e]]];
+                    x = fx_from_double(w[e]);
+                }
+                const bool own = j == (int)i;
+                any_self |= __ballot(own) != 0;
+                if (own) self += x;
+                else if (j >= 0) insert(j, x);
+            }
+            __syncthreads();               // s_pre / s_e0 / wtot are rewritten by the next chunk
+        }
+        self = wave_sum_fx(self);
+        if (lane == 0 && any_self) insert((int)i, self);
+        __syncthreads();
+        const int nk = s_nk;
+        const int64_t base = cpos[i];
+        for (int r = threadIdx.x; r < nk; r += kBlock) {
+            const int sl = klist[r];
+            ckey[base + r] = key[sl];
+            cw[base + r] = fx_to_double(fx_make(lo[sl], (int64_t)hi[sl]));
+        }
+        if (threadIdx.x == 0) cnt[i] = nk;
+        __syncthreads();                   // the table is cleared for the next row
     }
 }
 
@@ -544,6 +670,7 @@ coarse_insert_kernel(const int64_t* __restrict__ off, const int64_t* __restrict_
     for (int64_t v = wid; v < nv; v += nw) {
         if (gid && !keep[gid[v]]) continue;   // uniform in the wavefront
         const int64_t i = newid[C[v]];
+        if (toff[i + 1] == toff[i]) continue;   // uniform: a row aggregated in LDS
         const int64_t e1 = off[v + 1];
         for (int64_t e0 = off[v]; e0 < e1; e0 += 64) {   // uniform trip count in the wavefront
             const int64_t e = e0 + lane;
@@ -567,6 +694,7 @@ coarse_count_kernel(const int64_t* __restrict__ toff, const int* __restrict__ tk
     const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
     for (int64_t i = wid; i < K; i += nw) {
+        if (toff[i + 1] == toff[i]) continue;   // uniform: counted by the LDS aggregation
         int64_t c = 0;
         for (int64_t s = toff[i] + lane; s < toff[i + 1]; s += 64) c += ld_i32_agent(&tkey[s]) >= 0;
 #pragma unroll
@@ -578,13 +706,22 @@ coarse_count_kernel(const int64_t* __restrict__ toff, const int* __restrict__ tk
 __global__ void __launch_bounds__(kBlock)
 coarse_emit_kernel(const int64_t* __restrict__ toff, const int* __restrict__ tkey,
                    const fx_t* __restrict__ tval, int64_t K, const int64_t* __restrict__ noff,
-                   int64_t* __restrict__ ncol, double* __restrict__ nw_out)
+                   int64_t* __restrict__ ncol, double* __restrict__ nw_out, const int64_t* __restrict__ cpos,
+                   const int* __restrict__ ckey, const double* __restrict__ cw)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
     for (int64_t i = wid; i < K; i += nw) {
         int64_t pos = noff[i];
+        if (toff[i + 1] == toff[i]) {   // uniform: compact entries of an LDS-aggregated row
+            const int64_t n = noff[i + 1] - pos, c0 = cpos[i];
+            for (int64_t r = lane; r < n; r += 64) {
+                ncol[pos + r] = ckey[c0 + r];
+                nw_out[pos + r] = cw[c0 + r];
+            }
+            continue;
+        }
         for (int64_t s0 = toff[i]; s0 < toff[i + 1]; s0 += 64) {
             const int64_t s = s0 + lane;
             const int c = s < toff[i + 1] ? ld_i32_agent(&tkey[s]) : -1;
@@ -868,9 +1005,27 @@ void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t*
                                                                       keep);
 }
 
-void launch_table_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, hipStream_t s)
+void launch_coarse_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, int64_t* csz, hipStream_t s)
 {
-    if (K > 0) table_sizes_kernel<<<grid_for(K, kBlock), kBlock, 0, s>>>(arcs, K, tsz);
+    if (K > 0) coarse_sizes_kernel<<<grid_for(K, kBlock), kBlock, 0, s>>>(arcs, K, tsz, csz);
+}
+
+void launch_member_scatter(const int64_t* C, const int64_t* newid, int64_t nv, const int64_t* mstart,
+                           unsigned long long* mfill, int* members, hipStream_t s, const int* gid,
+                           const unsigned char* keep)
+{
+    if (nv > 0)
+        member_scatter_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, newid, nv, mstart, mfill, members, gid,
+                                                                       keep);
+}
+
+void launch_coarse_lds(const int64_t* arcs, const int64_t* mstart, const int* members, const int64_t* off,
+                       const int64_t* col, const double* w, const int64_t* C, const int64_t* newid, int64_t K,
+                       const int64_t* cpos, int* ckey, double* cw, int64_t* cnt, hipStream_t s)
+{
+    if (K > 0)
+        coarse_lds_kernel<<<grid_for(K, 1, 8192), kBlock, 0, s>>>(arcs, mstart, members, off, col, w, C, newid, K,
+                                                                  cpos, ckey, cw, cnt);
 }
 
 void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
@@ -918,11 +1073,12 @@ void launch_coarse_count(const int64_t* toff, const int* tkey, int64_t K, int64_
 }
 
 void launch_coarse_emit(const int64_t* toff, const int* tkey, const fx_t* tval, int64_t K,
-                        const int64_t* noff, int64_t* ncol, double* nw, hipStream_t s)
+                        const int64_t* noff, int64_t* ncol, double* nw, const int64_t* cpos, const int* ckey,
+                        const double* cw, hipStream_t s)
 {
     if (K > 0)
         coarse_emit_kernel<<<grid_for(K, kBlock / 64), kBlock, 0, s>>>(toff, tkey, tval, K, noff,
-                                                                       ncol, nw);
+                                                                       ncol, nw, cpos, ckey, cw);
 }
 
 }  // namespace nido
