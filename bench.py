@@ -48,6 +48,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -340,9 +341,10 @@ def main():
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="time the steps without per-kernel HIP events (roofline fields are then null)")
     ap.add_argument("--tuning", default="", help="comma list of nido_tuning knobs, e.g. big_slots=2048")
-    ap.add_argument("--event-stride", type=int, default=3,
+    ap.add_argument("--event-stride", type=int, default=0,
                     help="HIP events around every N-th launch of the dominant kernel class in the timed "
-                         "steps (rotating over the batches; 1 = every launch)")
+                         "steps (rotating over the batches; 1 = every launch; 0 = the first of 7, 5, 3 "
+                         "coprime with the batch count)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device copies of BYTES each (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a rocprofv3 --pmc pass) and exit")
@@ -422,6 +424,8 @@ def main():
         step()
     probe = g.kernel_stats()
     dom = max(probe, key=lambda k: probe[k]["ms"])
+    if args.event_stride <= 0:
+        args.event_stride = next((k for k in (7, 5, 3) if math.gcd(k, args.batches) == 1), 1)
     g.set_profiling(not args.no_kernel_events, classes=[dom], stride=args.event_stride)
     g.sync()
     t0 = time.perf_counter()

@@ -116,7 +116,7 @@ int nido_device_count(int* count);
 int nido_set_upload_staging(int64_t min_bytes, int64_t chunk_bytes);
 /* Phases of at least min_vertices vertices whose arcs are resident build
  * their degree bins and sweep row lists on the device, smaller ones on the
- * host (the same lists); 0 restores the default, 2^22.  Process-wide.  No
+ * host (the same lists); 0 restores the default, 2^17.  Process-wide.  No
  * counterpart in the reference (its bins are built by Thrust calls each
  * phase, graph_gpu.cpp:94-139); lowering it lets small inputs take the
  * device builder. */

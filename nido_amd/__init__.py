@@ -200,7 +200,7 @@ def set_upload_staging(min_bytes: int = 0, chunk_bytes: int = 0) -> None:
 
 def set_device_lists(min_vertices: int = 0) -> None:
     """Phases of at least min_vertices vertices build their degree bins and
-    row lists on the device (0: the default, 2^22); process-wide."""
+    row lists on the device (0: the default, 2^17); process-wide."""
     _check(lib.nido_set_device_lists(int(min_vertices)), "nido_set_device_lists")
 
 

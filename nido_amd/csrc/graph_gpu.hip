@@ -423,8 +423,10 @@ void drop_iteration_graph(nido_graph* g);
 // Phases of at least this many vertices (and arcs resident on the device)
 // build their degree bins and row lists on the device: for 4,096 graphs of
 // 48.5 M vertices in one handle the host built them in ~100 ms and uploaded
-// 1 GB of them.
-std::atomic<int64_t> g_device_lists_min{kParallelRows};
+// 1 GB of them, and one host thread took ~10 ms for the 0.8 M vertices of
+// their first coarse phase; the device builder costs a few round trips.
+constexpr int64_t kDeviceListsMin = int64_t(1) << 17;
+std::atomic<int64_t> g_device_lists_min{kDeviceListsMin};
 
 // The degree bins of the phase built on the device into d_binv (bins.hip);
 // the host gets the group bounds and each batch's long rows, and builds the
@@ -1916,7 +1918,7 @@ int nido_abi_version(void) { return NIDO_ABI_VERSION; }
 int nido_set_device_lists(int64_t min_vertices)
 {
     if (min_vertices < 0) return set_error(NIDO_ERR_INVALID, "min_vertices must be >= 0");
-    g_device_lists_min.store(min_vertices ? min_vertices : kParallelRows);
+    g_device_lists_min.store(min_vertices ? min_vertices : kDeviceListsMin);
     return NIDO_OK;
 }
 
