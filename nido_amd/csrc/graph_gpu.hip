@@ -1206,7 +1206,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
     int64_t *arcs = nullptr, *toff = nullptr, *cnt = nullptr, *noff = nullptr, *ncol = nullptr;
     int64_t *cpos = nullptr, *mstart = nullptr;
     unsigned long long* mfill = nullptr;
-    int *members = nullptr, *ckey = nullptr;
+    int *members = nullptr, *ckey = nullptr, *hlist = nullptr;
     double *nw = nullptr, *cw = nullptr;
     int* tkey = nullptr;
     fx_t* tval = nullptr;
@@ -1215,6 +1215,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         dev_free(arcs); dev_free(toff); dev_free(cnt); dev_free(noff);
         dev_free(ncol); dev_free(nw); dev_free(tkey); dev_free(tval);
         dev_free(cpos); dev_free(mstart); dev_free(mfill); dev_free(members); dev_free(ckey); dev_free(cw);
+        dev_free(hlist);
         raw_free(tmp);
         tmp = nullptr;
     };
@@ -1223,7 +1224,8 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         if ((rc = dev_alloc(&arcs, K + 1)) || (rc = dev_alloc(&toff, K + 1)) ||
             (rc = dev_alloc(&cnt, K + 1)) || (rc = dev_alloc(&noff, K + 1)) ||
             (rc = dev_alloc(&cpos, K + 1)) || (rc = dev_alloc(&mstart, K + 1)) ||
-            (rc = dev_alloc(&mfill, std::max<int64_t>(K, 1))) || (rc = dev_alloc(&members, std::max<int64_t>(nv, 1))))
+            (rc = dev_alloc(&mfill, K + 1)) || (rc = dev_alloc(&members, std::max<int64_t>(nv, 1))) ||
+            (rc = dev_alloc(&hlist, std::max<int64_t>(nv, 1))))
             break;
         if ((rc = raw_alloc(&tmp, scan_temp_bytes(K + 1)))) break;
         // per coarse row: its arcs and members; rows of few arcs are
@@ -1231,7 +1233,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         // tables (toff)
         launch_fill_i64(arcs, 0, K + 1, s);
         launch_fill_i64(cnt, 0, K + 1, s);
-        launch_fill_i64(reinterpret_cast<int64_t*>(mfill), 0, std::max<int64_t>(K, 1), s);
+        launch_fill_i64(reinterpret_cast<int64_t*>(mfill), 0, K + 1, s);   // [K]: HBM-row vertices
         launch_member_counts(g->d_C, newid, g->d_off, nv, cnt, arcs, s, gid, keep);
         launch_exclusive_scan(cnt, mstart, K, mstart + K, tmp, s);
         launch_coarse_sizes(arcs, K, toff, cpos, s);
@@ -1248,10 +1250,10 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
             launch_fill_i32(tkey, -1, T, s);
             launch_fill_fx(tval, T, s);
         }
-        launch_member_scatter(g->d_C, newid, nv, mstart, mfill, members, s, gid, keep);
+        launch_member_scatter(g->d_C, newid, nv, mstart, mfill, members, toff, hlist, mfill + K, s, gid, keep);
         launch_coarse_lds(arcs, mstart, members, g->d_off, g->d_col, g->d_w, g->d_C, newid, K, cpos, ckey, cw, cnt, s);
         if (T > 0) {
-            launch_coarse_insert(g->d_off, g->d_col, g->d_w, g->d_C, newid, nv, toff, tkey, tval, s, gid, keep);
+            launch_coarse_insert(g->d_off, g->d_col, g->d_w, g->d_C, newid, hlist, mfill + K, nv, toff, tkey, tval, s);
             launch_coarse_count(toff, tkey, K, cnt, s);
         }
         launch_exclusive_scan(cnt, noff, K, noff + K, tmp, s);

@@ -322,18 +322,21 @@ void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t*
 // an LDS row), csz[i] compact entries (0 for a table row).
 constexpr int64_t kLdsRowArcs = 1024;
 void launch_coarse_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, int64_t* csz, hipStream_t s);
-// members[mstart[i] ...]: the vertices of coarse row i (mfill zeroed, K).
+// members[mstart[i] ...]: the vertices of coarse row i (mfill zeroed, K);
+// hlist[0, *nh): the vertices of rows with an HBM table (*nh zeroed).
 void launch_member_scatter(const int64_t* C, const int64_t* newid, int64_t nv, const int64_t* mstart,
-                           unsigned long long* mfill, int* members, hipStream_t s, const int* gid = nullptr,
+                           unsigned long long* mfill, int* members, const int64_t* toff, int* hlist,
+                           unsigned long long* nh, hipStream_t s, const int* gid = nullptr,
                            const unsigned char* keep = nullptr);
 // The LDS rows: distinct targets at ckey/cw[cpos[i] ...), their count in cnt[i].
 void launch_coarse_lds(const int64_t* arcs, const int64_t* mstart, const int* members, const int64_t* off,
                        const int64_t* col, const double* w, const int64_t* C, const int64_t* newid, int64_t K,
                        const int64_t* cpos, int* ckey, double* cw, int64_t* cnt, hipStream_t s);
+// The HBM-table rows: every listed vertex's arcs (hlist[0, *nh), nmax >= *nh
+// sizes the grid).
 void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
-                          const int64_t* newid, int64_t nv, const int64_t* toff, int* tkey,
-                          fx_t* tval, hipStream_t s, const int* gid = nullptr,
-                          const unsigned char* keep = nullptr);
+                          const int64_t* newid, const int* hlist, const unsigned long long* nh, int64_t nmax,
+                          const int64_t* toff, int* tkey, fx_t* tval, hipStream_t s);
 void launch_coarse_count(const int64_t* toff, const int* tkey, int64_t K, int64_t* cnt,
                          hipStream_t s);
 void launch_coarse_emit(const int64_t* toff, const int* tkey, const fx_t* tval, int64_t K,

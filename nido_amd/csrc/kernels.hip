@@ -469,19 +469,55 @@ clustering_kernel(int64_t* __restrict__ clus, int64_t n0, const int64_t* __restr
         if (!gid0 || keep[gid0[i]]) clus[i] = newid[C[clus[i]]];
 }
 
+// The coarse rows of a workgroup's kBlock consecutive vertices, grouped in
+// an LDS table of 2 * kBlock slots (at most kBlock keys: half full at most,
+// so probing ends): vertices of one community are often neighbours in id,
+// and each distinct row then takes one global atomic instead of one per
+// vertex.  Returns the slot of row i (i >= 0).
+constexpr int kGroupSlots = 2 * kBlock;
+__device__ __forceinline__ int group_slot(int* key, int i)
+{
+    unsigned h = ((unsigned)i * kHashMul) >> (32 - ilog2c(kGroupSlots));
+    for (;;) {
+        const int kk = key[h];
+        if (kk == i) return (int)h;
+        if (kk < 0) {
+            const int prev = atomicCAS(&key[h], -1, i);
+            if (prev < 0 || prev == i) return (int)h;
+        }
+        h = (h + 1) & (kGroupSlots - 1);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
 member_counts_kernel(const int64_t* __restrict__ C, const int64_t* __restrict__ newid,
                      const int64_t* __restrict__ off, int64_t nv, int64_t* __restrict__ cnt,
                      int64_t* __restrict__ arcs, const int* __restrict__ gid,
                      const unsigned char* __restrict__ keep)
 {
-    for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < nv;
-         v += (int64_t)gridDim.x * kBlock) {
-        if (gid && !keep[gid[v]]) continue;
-        const int64_t i = newid[C[v]];
-        if (cnt) atomicAdd(reinterpret_cast<unsigned long long*>(cnt + i), 1ull);
-        const int64_t d = off[v + 1] - off[v];
-        if (d) atomicAdd(reinterpret_cast<unsigned long long*>(arcs + i), (unsigned long long)d);
+    __shared__ int key[kGroupSlots];
+    __shared__ unsigned long long nm[kGroupSlots], na[kGroupSlots];
+    for (int64_t base = blockIdx.x * (int64_t)kBlock; base < nv; base += (int64_t)gridDim.x * kBlock) {
+        for (int s = threadIdx.x; s < kGroupSlots; s += kBlock) {
+            key[s] = -1;
+            nm[s] = 0;
+            na[s] = 0;
+        }
+        __syncthreads();
+        const int64_t v = base + threadIdx.x;
+        if (v < nv && (!gid || keep[gid[v]])) {
+            const int h = group_slot(key, (int)newid[C[v]]);
+            atomicAdd(&nm[h], 1ull);
+            atomicAdd(&na[h], (unsigned long long)(off[v + 1] - off[v]));
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < kGroupSlots; s += kBlock) {
+            const int i = key[s];
+            if (i < 0) continue;
+            if (cnt) atomicAdd(reinterpret_cast<unsigned long long*>(cnt + i), nm[s]);
+            if (na[s]) atomicAdd(reinterpret_cast<unsigned long long*>(arcs + i), na[s]);
+        }
+        __syncthreads();   // the table is cleared for the next chunk
     }
 }
 
@@ -505,16 +541,49 @@ coarse_sizes_kernel(const int64_t* __restrict__ arcs, int64_t K, int64_t* __rest
     }
 }
 
-// members[mstart[i] + r] = the vertices of coarse row i, in no set order.
+// members[mstart[i] + r] = the vertices of coarse row i, in no set order,
+// and hlist[0, *nh) the vertices whose row has an HBM table (toff[i + 1] >
+// toff[i]).  Each workgroup reserves a block of its rows' member slots with
+// one atomic per distinct row (grouped in LDS as in member_counts_kernel).
 __global__ void __launch_bounds__(kBlock)
 member_scatter_kernel(const int64_t* __restrict__ C, const int64_t* __restrict__ newid, int64_t nv,
                       const int64_t* __restrict__ mstart, unsigned long long* __restrict__ mfill,
-                      int* __restrict__ members, const int* __restrict__ gid, const unsigned char* __restrict__ keep)
+                      int* __restrict__ members, const int64_t* __restrict__ toff, int* __restrict__ hlist,
+                      unsigned long long* __restrict__ nh, const int* __restrict__ gid,
+                      const unsigned char* __restrict__ keep)
 {
-    for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kBlock) {
-        if (gid && !keep[gid[v]]) continue;
-        const int64_t i = newid[C[v]];
-        members[mstart[i] + (int64_t)atomicAdd(mfill + i, 1ull)] = (int)v;
+    __shared__ int key[kGroupSlots];
+    __shared__ unsigned nm[kGroupSlots];
+    __shared__ unsigned long long at[kGroupSlots];
+    __shared__ unsigned s_nh;
+    __shared__ unsigned long long s_hbase;
+    for (int64_t base = blockIdx.x * (int64_t)kBlock; base < nv; base += (int64_t)gridDim.x * kBlock) {
+        for (int s = threadIdx.x; s < kGroupSlots; s += kBlock) {
+            key[s] = -1;
+            nm[s] = 0;
+        }
+        if (threadIdx.x == 0) s_nh = 0;
+        __syncthreads();
+        const int64_t v = base + threadIdx.x;
+        int64_t i = -1;
+        int h = 0;
+        unsigned rank = 0, hrank = 0;
+        bool hbm = false;
+        if (v < nv && (!gid || keep[gid[v]])) {
+            i = newid[C[v]];
+            h = group_slot(key, (int)i);
+            rank = atomicAdd(&nm[h], 1u);
+            hbm = toff[i + 1] > toff[i];
+            if (hbm) hrank = atomicAdd(&s_nh, 1u);
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < kGroupSlots; s += kBlock)
+            if (key[s] >= 0) at[s] = atomicAdd(mfill + key[s], (unsigned long long)nm[s]);
+        if (threadIdx.x == 0 && s_nh) s_hbase = atomicAdd(nh, (unsigned long long)s_nh);
+        __syncthreads();
+        if (i >= 0) members[mstart[i] + (int64_t)(at[h] + rank)] = (int)v;
+        if (hbm) hlist[s_hbase + hrank] = (int)v;
+        __syncthreads();   // the table is cleared for the next chunk
     }
 }
 
@@ -659,18 +728,17 @@ __device__ __forceinline__ void coarse_insert_one(int64_t i, int j, fx_t wf,
 __global__ void __launch_bounds__(kBlock)
 coarse_insert_kernel(const int64_t* __restrict__ off, const int64_t* __restrict__ col,
                      const double* __restrict__ w, const int64_t* __restrict__ C,
-                     const int64_t* __restrict__ newid, int64_t nv,
-                     const int64_t* __restrict__ toff, int* __restrict__ tkey,
-                     fx_t* __restrict__ tval, const int* __restrict__ gid,
-                     const unsigned char* __restrict__ keep)
+                     const int64_t* __restrict__ newid, const int* __restrict__ hlist,
+                     const unsigned long long* __restrict__ nh, const int64_t* __restrict__ toff,
+                     int* __restrict__ tkey, fx_t* __restrict__ tval)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
-    for (int64_t v = wid; v < nv; v += nw) {
-        if (gid && !keep[gid[v]]) continue;   // uniform in the wavefront
+    const int64_t n = (int64_t)*nh;
+    for (int64_t t = wid; t < n; t += nw) {
+        const int64_t v = hlist[t];              // a vertex whose coarse row has an HBM table
         const int64_t i = newid[C[v]];
-        if (toff[i + 1] == toff[i]) continue;   // uniform: a row aggregated in LDS
         const int64_t e1 = off[v + 1];
         for (int64_t e0 = off[v]; e0 < e1; e0 += 64) {   // uniform trip count in the wavefront
             const int64_t e = e0 + lane;
@@ -1011,12 +1079,12 @@ void launch_coarse_sizes(const int64_t* arcs, int64_t K, int64_t* tsz, int64_t* 
 }
 
 void launch_member_scatter(const int64_t* C, const int64_t* newid, int64_t nv, const int64_t* mstart,
-                           unsigned long long* mfill, int* members, hipStream_t s, const int* gid,
-                           const unsigned char* keep)
+                           unsigned long long* mfill, int* members, const int64_t* toff, int* hlist,
+                           unsigned long long* nh, hipStream_t s, const int* gid, const unsigned char* keep)
 {
     if (nv > 0)
-        member_scatter_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, newid, nv, mstart, mfill, members, gid,
-                                                                       keep);
+        member_scatter_kernel<<<grid_for(nv, kBlock), kBlock, 0, s>>>(C, newid, nv, mstart, mfill, members, toff,
+                                                                       hlist, nh, gid, keep);
 }
 
 void launch_coarse_lds(const int64_t* arcs, const int64_t* mstart, const int* members, const int64_t* off,
@@ -1029,12 +1097,12 @@ void launch_coarse_lds(const int64_t* arcs, const int64_t* mstart, const int* me
 }
 
 void launch_coarse_insert(const int64_t* off, const int64_t* col, const double* w, const int64_t* C,
-                          const int64_t* newid, int64_t nv, const int64_t* toff, int* tkey,
-                          fx_t* tval, hipStream_t s, const int* gid, const unsigned char* keep)
+                          const int64_t* newid, const int* hlist, const unsigned long long* nh, int64_t nmax,
+                          const int64_t* toff, int* tkey, fx_t* tval, hipStream_t s)
 {
-    if (nv > 0)
-        coarse_insert_kernel<<<grid_for(nv, kBlock / 64), kBlock, 0, s>>>(off, col, w, C, newid, nv,
-                                                                          toff, tkey, tval, gid, keep);
+    if (nmax > 0)
+        coarse_insert_kernel<<<grid_for(nmax, kBlock / 64), kBlock, 0, s>>>(off, col, w, C, newid, hlist, nh,
+                                                                            toff, tkey, tval);
 }
 
 void launch_modularity_seg(const SegItem* items, int64_t nitems, const int* src, const int* col,
