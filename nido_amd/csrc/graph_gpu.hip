@@ -1220,6 +1220,12 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         tmp = nullptr;
     };
     int rc = NIDO_OK;
+    // a failed HIP call leaves the loop with the error, and cleanup runs
+#define COARSEN_HIP(expr)                                                                                  \
+    if (hipError_t e_ = (expr); e_ != hipSuccess) {                                                        \
+        rc = set_error(NIDO_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+        break;                                                                                             \
+    }
     do {
         if ((rc = dev_alloc(&arcs, K + 1)) || (rc = dev_alloc(&toff, K + 1)) ||
             (rc = dev_alloc(&cnt, K + 1)) || (rc = dev_alloc(&noff, K + 1)) ||
@@ -1240,9 +1246,9 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         launch_exclusive_scan(toff, toff, K, toff + K, tmp, s);
         launch_exclusive_scan(cpos, cpos, K, cpos + K, tmp, s);
         int64_t T = 0, Tc = 0;
-        HIP_TRY(hipMemcpyAsync(&T, toff + K, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&Tc, cpos + K, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        COARSEN_HIP(hipMemcpyAsync(&T, toff + K, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        COARSEN_HIP(hipMemcpyAsync(&Tc, cpos + K, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        COARSEN_HIP(hipStreamSynchronize(s));
         if ((rc = dev_alloc(&tkey, std::max<int64_t>(T, 1))) || (rc = dev_alloc(&tval, std::max<int64_t>(T, 1))) ||
             (rc = dev_alloc(&ckey, std::max<int64_t>(Tc, 1))) || (rc = dev_alloc(&cw, std::max<int64_t>(Tc, 1))))
             break;
@@ -1257,15 +1263,15 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
             launch_coarse_count(toff, tkey, K, cnt, s);
         }
         launch_exclusive_scan(cnt, noff, K, noff + K, tmp, s);
-        HIP_TRY(hipGetLastError());
+        COARSEN_HIP(hipGetLastError());
         std::vector<int64_t> hoff((size_t)K + 1);
-        HIP_TRY(hipMemcpyAsync(hoff.data(), noff, sizeof(int64_t) * (size_t)(K + 1), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        COARSEN_HIP(hipMemcpyAsync(hoff.data(), noff, sizeof(int64_t) * (size_t)(K + 1), hipMemcpyDeviceToHost, s));
+        COARSEN_HIP(hipStreamSynchronize(s));
         const int64_t nne = hoff[(size_t)K];
         if ((rc = dev_alloc(&ncol, nne)) || (rc = dev_alloc(&nw, nne))) break;
         launch_coarse_emit(toff, tkey, tval, K, noff, ncol, nw, cpos, ckey, cw, s);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(s));
+        COARSEN_HIP(hipGetLastError());
+        COARSEN_HIP(hipStreamSynchronize(s));
         free_graph_arrays(g);
         g->d_off = noff; noff = nullptr;
         g->d_col = ncol; ncol = nullptr;
@@ -1274,6 +1280,7 @@ int coarsen(nido_graph* g, const int64_t* newid, int64_t K, const int* gid, cons
         g->ne = nne;
         g->h_off.swap(hoff);
     } while (0);
+#undef COARSEN_HIP
     cleanup();
     return rc;
 }
