@@ -121,6 +121,13 @@ int nido_set_upload_staging(int64_t min_bytes, int64_t chunk_bytes);
  * phase, graph_gpu.cpp:94-139); lowering it lets small inputs take the
  * device builder. */
 int nido_set_device_lists(int64_t min_vertices);
+/* A batched handle (several graphs) rebuilds its local-moving work lists
+ * for the graphs still iterating in the phase once their vertices are at
+ * most `percent` of the listed ones (0 never, 100 as soon as one graph
+ * stops; results do not change); a negative value restores the default,
+ * 50.  Process-wide.  No counterpart in the reference (it runs one graph
+ * per handle, louvain_gpu.cpp:40-120). */
+int nido_set_list_rebuild(int percent);
 
 /* ---- host graph: Graph (graph.hpp:14-102) ------------------------------- */
 
@@ -343,6 +350,9 @@ int nido_batch_create(int ngraphs, const int64_t* vptr, const int64_t* offsets, 
  * (phases = longest run, total_loops = shared iterations). */
 int nido_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase, nido_run_stats* stats);
 int nido_batch_size(const nido_graph* g, int* ngraphs);
+/* Work-list rebuilds for the graphs still iterating (nido_set_list_rebuild)
+ * in the last nido_batch_run.  No counterpart in the reference. */
+int nido_batch_list_rebuilds(const nido_graph* g, int64_t* n);
 /* One graph's run: q, phases, total_loops as LouvainGPU::run reports them. */
 int nido_batch_graph_stats(const nido_graph* g, int graph, nido_run_stats* out);
 int nido_batch_graph_phase(const nido_graph* g, int graph, int phase, nido_phase_stats* out);

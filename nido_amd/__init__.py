@@ -84,6 +84,7 @@ if lib.nido_abi_version() != ABI_VERSION:
 _sig("nido_device_count", C.c_int, C.POINTER(C.c_int))
 _sig("nido_set_upload_staging", C.c_int, _i64, _i64)
 _sig("nido_set_device_lists", C.c_int, _i64)
+_sig("nido_set_list_rebuild", C.c_int, C.c_int)
 _sig("nido_hgraph_read", C.c_int, C.c_char_p, C.POINTER(C.c_void_p))
 _sig("nido_hgraph_write", C.c_int, C.c_void_p, C.c_char_p)
 _sig("nido_hgraph_from_csr", C.c_int, _i64, _i64, _pi64, _pi64, _pdbl, C.POINTER(C.c_void_p))
@@ -143,6 +144,7 @@ _sig("nido_batch_create", C.c_int, C.c_int, _pi64, _pi64, _pi64, _pdbl, C.c_int,
      C.POINTER(C.c_void_p))
 _sig("nido_batch_run", C.c_int, C.c_void_p, _i64, C.c_double, C.c_double, C.POINTER(RunStats))
 _sig("nido_batch_size", C.c_int, C.c_void_p, C.POINTER(C.c_int))
+_sig("nido_batch_list_rebuilds", C.c_int, C.c_void_p, _pi64)
 _sig("nido_batch_graph_stats", C.c_int, C.c_void_p, C.c_int, C.POINTER(RunStats))
 _sig("nido_batch_graph_phase", C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(PhaseStats))
 _sig("nido_batch_clustering", C.c_int, C.c_void_p, C.c_int, _pi64)
@@ -202,6 +204,14 @@ def set_device_lists(min_vertices: int = 0) -> None:
     """Phases of at least min_vertices vertices build their degree bins and
     row lists on the device (0: the default, 2^17); process-wide."""
     _check(lib.nido_set_device_lists(int(min_vertices)), "nido_set_device_lists")
+
+
+def set_list_rebuild(percent: int = -1) -> None:
+    """A batched handle rebuilds its work lists for the graphs still
+    iterating once they hold at most `percent` of the listed vertices (0
+    never, 100 as soon as one stops; negative: the default, 50);
+    process-wide."""
+    _check(lib.nido_set_list_rebuild(int(percent)), "nido_set_list_rebuild")
 
 
 def modularity_noalloc(nv: int, ne: int, offsets: int, cols: int, weights: int, comm: int, q: int,
@@ -612,6 +622,12 @@ class BatchGPU:
         _check(lib.nido_batch_run(self._h, max_loops, tol, tol if tol_phase is None else tol_phase,
                                   C.byref(st)), "nido_batch_run")
         return st
+
+    def list_rebuilds(self) -> int:
+        """Work-list rebuilds for the graphs still iterating in the last run."""
+        n = C.c_int64()
+        _check(lib.nido_batch_list_rebuilds(self._h, C.byref(n)), "nido_batch_list_rebuilds")
+        return n.value
 
     def graph_stats(self, i: int) -> RunStats:
         st = RunStats()

@@ -302,6 +302,9 @@ struct nido_graph {
     fx_t twom = 0;
     double twom_d = 0, m = 0;
     bool nonneg = true;            // every weight >= 0 (coarse weights are sums of these)
+    int narrow = 0;                // long-row tables may keep 32-bit high words
+    int64_t listed = 0;            // vertices of the graphs the work lists hold
+    int64_t rebuilds = 0;          // work-list rebuilds for the iterating graphs (batched runs)
     std::vector<int> committed;    // per batch: weights applied since its last update
 
     std::vector<double> short_bytes, long_bytes;   // per batch, algorithmic
@@ -427,18 +430,24 @@ void drop_iteration_graph(nido_graph* g);
 // their first coarse phase; the device builder costs a few round trips.
 constexpr int64_t kDeviceListsMin = int64_t(1) << 17;
 std::atomic<int64_t> g_device_lists_min{kDeviceListsMin};
+// A handle of several graphs rebuilds its work lists for the graphs still
+// iterating once their vertices are at most this share (percent) of the
+// listed ones: the short-row launches otherwise visit every stopped graph's
+// rows, a floor that stays while the last graphs of a phase converge.
+constexpr int kListRebuildPct = 50;
+std::atomic<int> g_list_rebuild_pct{kListRebuildPct};
 
 // The degree bins of the phase built on the device into d_binv (bins.hip);
 // the host gets the group bounds and each batch's long rows, and builds the
 // long-row work from those.
-int device_bins(nido_graph* g, PhaseBins& pb)
+int device_bins(nido_graph* g, PhaseBins& pb, const unsigned char* keep)
 {
     const int nb = g->nb, G = g->G;
     const auto& off = g->h_off;
     hipStream_t s = g->stream;
     std::vector<BinTile> tiles;
     std::vector<int64_t> tb0;
-    bin_tiles(nb, G, g->parts, g->gparts, tiles, tb0);
+    bin_tiles(nb, G, g->parts, g->gparts, keep, tiles, tb0);
     const int64_t T = (int64_t)tiles.size(), nc = NUM_BINS * T;
     BinTile* d_tiles = nullptr;
     int64_t *d_tb0 = nullptr, *d_cnt = nullptr, *d_starts = nullptr, *d_total = nullptr;
@@ -515,7 +524,8 @@ int device_bins(nido_graph* g, PhaseBins& pb)
         if (G == 1) range(g->parts[(size_t)b], g->parts[(size_t)b + 1]);
         else
             for (int i = 0; i < G; ++i)
-                range(g->gparts[(size_t)i * (nb + 1) + b], g->gparts[(size_t)i * (nb + 1) + b + 1]);
+                if (!keep || keep[i])
+                    range(g->gparts[(size_t)i * (nb + 1) + b], g->gparts[(size_t)i * (nb + 1) + b + 1]);
     }
     HIP_TRY(hipStreamSynchronize(s));
     build_phase_lists(off.data(), nb, cnt, big, bverts, barcs, g->tuning, pb);
@@ -565,38 +575,24 @@ int device_row_lists(nido_graph* g)
     return rc;
 }
 
-// Per-phase set-up: maximum order, batches, degree bins, the row slices of
-// the degree/modularity sweeps, k_v and 2m (GraphGPU ctor steps
-// graph_gpu.cpp:94-139 and aggregation steps graph_gpu.cpp:1487-1516).
-int setup_phase(nido_graph* g)
+// The work lists of the phase's launches (degree bins, long-row items,
+// key-fill chunks, the per-launch arguments), for every graph of the
+// handle or, with keep set, only the graphs g with keep[g] set (the others
+// sit out the rest of the phase).
+int build_lists(nido_graph* g, const unsigned char* keep)
 {
     const int64_t nv = g->nv;
     const auto& off = g->h_off;
-    drop_iteration_graph(g);
-    g->inc_ok = false;
-    g->ident = false;
-    g->maxdeg = max_degree(off.data(), nv);
-    g->committed.assign((size_t)g->nb, 0);
-    // Batch b of the handle = batch b of every graph it holds (each graph
-    // cut by its own rule, graph_gpu.cpp:303-339).
     const int G = g->G;
-    if (G == 1) batch_bounds(off, nv, g->nb, g->parts);
-    else batch_bounds_all(off.data(), G, g->vb.data(), g->nb, g->gparts);
-    // A phase with more arcs than the device window streams them: every
-    // batch is cut into units of rows whose arcs fit the window, and each
-    // unit is one local-moving launch pair over the arcs copied in for it.
-    g->windowed = G == 1 && g->window > 0 && g->ne > g->window;
-    g->win_e0 = g->win_e1 = -1;
-    if (g->windowed) NIDO_TRY(window_units(off.data(), g->parts, g->nb, g->window, g->units, g->first_unit));
     // launch units: the batches, or their window units (a handle of several
     // graphs keeps its batch bounds per graph, in gparts)
     const std::vector<int64_t>& lparts = g->windowed ? g->units : g->parts;
     const int nl = g->windowed ? (int)g->units.size() - 1 : g->nb;
-
+    drop_iteration_graph(g);
     const bool dev_lists = !g->windowed && nv >= g_device_lists_min.load();
     PhaseBins pb;
-    if (dev_lists) NIDO_TRY(device_bins(g, pb));
-    else build_phase_bins(off.data(), nv, nl, G, lparts, g->gparts, g->tuning, pb);
+    if (dev_lists) NIDO_TRY(device_bins(g, pb, keep));
+    else build_phase_bins(off.data(), nv, nl, G, lparts, g->gparts, g->tuning, pb, keep);
     const auto& binv = pb.binv;
     const auto& bitems = pb.bitems;
     const auto& chunks = pb.chunks;
@@ -653,48 +649,6 @@ int setup_phase(nido_graph* g)
         launch_fill_i32(g->d_cnt, 0, slot_need, g->stream);   // the kernels keep them zero afterwards
         g->slot_cap = slot_need;
     }
-    if (!g->windowed) {
-        // incremental modularity: room for every row chunk of the largest batch
-        int64_t need = 0;
-        if (G == 1) {
-            for (int b = 0; b < g->nb; ++b) {
-                int64_t n = 0;
-                for (int64_t v = g->parts[(size_t)b]; v < g->parts[(size_t)b + 1]; ++v)
-                    n += (off[v + 1] - off[v] + kDeltaArcs - 1) / kDeltaArcs;
-                need = std::max(need, n);
-            }
-        } else {
-            // ceil(d / K) <= 1 + d / K: rows + arcs / K of the batch bound it
-            const size_t nb1 = (size_t)g->nb + 1;
-            for (int b = 0; b < g->nb; ++b) {
-                int64_t rows = 0, arcs = 0;
-                for (int i = 0; i < G; ++i) {
-                    const int64_t lo = g->gparts[(size_t)i * nb1 + (size_t)b];
-                    const int64_t hi = g->gparts[(size_t)i * nb1 + (size_t)b + 1];
-                    rows += hi - lo;
-                    arcs += off[hi] - off[lo];
-                }
-                need = std::max(need, rows + arcs / kDeltaArcs);
-            }
-            if ((int64_t)g->gparts.size() > g->gp_cap) {
-                dev_free(g->d_gp);
-                NIDO_TRY(dev_alloc(&g->d_gp, (int64_t)g->gparts.size()));
-                g->gp_cap = (int64_t)g->gparts.size();
-            }
-            HIP_TRY(hipMemcpyAsync(g->d_gp, g->gparts.data(), sizeof(int64_t) * g->gparts.size(),
-                                   hipMemcpyHostToDevice, g->stream));
-        }
-        if (need > g->dchunk_cap) {
-            dev_free(g->d_dchunks);
-            NIDO_TRY(dev_alloc(&g->d_dchunks, need));
-            g->dchunk_cap = need;
-        }
-        dev_free(g->d_dcounts);
-        NIDO_TRY(dev_alloc(&g->d_dcounts, 2 * (int64_t)g->nb));
-        launch_fill_i32(g->d_dcounts, 0, 2 * (int64_t)g->nb, g->stream);
-        launch_fill_i32(g->d_touched, 0, g->n0, g->stream);
-        HIP_TRY(hipStreamSynchronize(g->stream));
-    }
     const size_t nbig_items = bitems.size();
     g->small_bins.assign((size_t)nl, SmallBins{});
     g->big_args.assign((size_t)nl, BigArgs{});
@@ -738,6 +692,83 @@ int setup_phase(nido_graph* g)
         B.tval = g->d_tval;
     }
 
+    for (auto& B : g->big_args) B.narrow = g->narrow;
+    for (auto& L : g->small_bins) L.twom_d = g->twom_d;
+    g->listed = 0;
+    for (int i = 0; i < G; ++i)
+        if (!keep || keep[i]) g->listed += G == 1 ? nv : g->vb[(size_t)i + 1] - g->vb[(size_t)i];
+    return NIDO_OK;
+}
+
+// Per-phase set-up: maximum order, batches, degree bins, the row slices of
+// the degree/modularity sweeps, k_v and 2m (GraphGPU ctor steps
+// graph_gpu.cpp:94-139 and aggregation steps graph_gpu.cpp:1487-1516).
+int setup_phase(nido_graph* g)
+{
+    const int64_t nv = g->nv;
+    const auto& off = g->h_off;
+    drop_iteration_graph(g);
+    g->inc_ok = false;
+    g->ident = false;
+    g->maxdeg = max_degree(off.data(), nv);
+    g->committed.assign((size_t)g->nb, 0);
+    // Batch b of the handle = batch b of every graph it holds (each graph
+    // cut by its own rule, graph_gpu.cpp:303-339).
+    const int G = g->G;
+    if (G == 1) batch_bounds(off, nv, g->nb, g->parts);
+    else batch_bounds_all(off.data(), G, g->vb.data(), g->nb, g->gparts);
+    // A phase with more arcs than the device window streams them: every
+    // batch is cut into units of rows whose arcs fit the window, and each
+    // unit is one local-moving launch pair over the arcs copied in for it.
+    g->windowed = G == 1 && g->window > 0 && g->ne > g->window;
+    g->win_e0 = g->win_e1 = -1;
+    if (g->windowed) NIDO_TRY(window_units(off.data(), g->parts, g->nb, g->window, g->units, g->first_unit));
+    const bool dev_lists = !g->windowed && nv >= g_device_lists_min.load();
+    if (!g->windowed) {
+        // incremental modularity: room for every row chunk of the largest batch
+        int64_t need = 0;
+        if (G == 1) {
+            for (int b = 0; b < g->nb; ++b) {
+                int64_t n = 0;
+                for (int64_t v = g->parts[(size_t)b]; v < g->parts[(size_t)b + 1]; ++v)
+                    n += (off[v + 1] - off[v] + kDeltaArcs - 1) / kDeltaArcs;
+                need = std::max(need, n);
+            }
+        } else {
+            // ceil(d / K) <= 1 + d / K: rows + arcs / K of the batch bound it
+            const size_t nb1 = (size_t)g->nb + 1;
+            for (int b = 0; b < g->nb; ++b) {
+                int64_t rows = 0, arcs = 0;
+                for (int i = 0; i < G; ++i) {
+                    const int64_t lo = g->gparts[(size_t)i * nb1 + (size_t)b];
+                    const int64_t hi = g->gparts[(size_t)i * nb1 + (size_t)b + 1];
+                    rows += hi - lo;
+                    arcs += off[hi] - off[lo];
+                }
+                need = std::max(need, rows + arcs / kDeltaArcs);
+            }
+            if ((int64_t)g->gparts.size() > g->gp_cap) {
+                dev_free(g->d_gp);
+                NIDO_TRY(dev_alloc(&g->d_gp, (int64_t)g->gparts.size()));
+                g->gp_cap = (int64_t)g->gparts.size();
+            }
+            HIP_TRY(hipMemcpyAsync(g->d_gp, g->gparts.data(), sizeof(int64_t) * g->gparts.size(),
+                                   hipMemcpyHostToDevice, g->stream));
+        }
+        if (need > g->dchunk_cap) {
+            dev_free(g->d_dchunks);
+            NIDO_TRY(dev_alloc(&g->d_dchunks, need));
+            g->dchunk_cap = need;
+        }
+        dev_free(g->d_dcounts);
+        NIDO_TRY(dev_alloc(&g->d_dcounts, 2 * (int64_t)g->nb));
+        launch_fill_i32(g->d_dcounts, 0, 2 * (int64_t)g->nb, g->stream);
+        launch_fill_i32(g->d_touched, 0, g->n0, g->stream);
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    // after the delta buffers above: the short-row launches keep their
+    // addresses (g->delta())
+    NIDO_TRY(build_lists(g, nullptr));
     fx_t twom = 0;
     unsigned long long maxk_bits = 0;
     if (g->windowed) {
@@ -783,8 +814,8 @@ int setup_phase(nido_graph* g)
     // can reach 2^31: weights non-negative and every k_v < 2^31 (a rounded
     // k_v below 2^31 - 2^8 leaves room for the rounding).
     const double maxk = nido_bitsd(maxk_bits);
-    const int narrow = g->nonneg && maxk < 2147483648.0 - 256.0;
-    for (auto& B : g->big_args) B.narrow = narrow;
+    g->narrow = g->nonneg && maxk < 2147483648.0 - 256.0;
+    for (auto& B : g->big_args) B.narrow = g->narrow;
     for (auto& L : g->small_bins) L.twom_d = g->twom_d;
     // The short rows sum in 64-bit fixed point when every weight is a
     // multiple of 2^-F (F <= 62) and no short row's sum can reach 2^63 in
@@ -1148,6 +1179,10 @@ int run_iteration_launches(nido_graph* g)
     const int fm = g->tuning.full_modularity;
     bool inc = any_w && g->symmetric && g->inc_ok && fm != 1 && g->iters_since_singleton > 0 && !g->windowed;
     if (inc && fm == 0 && g->G > 1) inc = g->last_moved >= 0 && g->last_moved * kIncShare <= g->ne;
+    // the short-row launches must carry the delta buffers' current addresses
+    for (const SmallBins& L : g->small_bins)
+        if (L.dblocks && (L.D.counts != g->d_dcounts || L.D.chunks != g->d_dchunks || L.D.gp != g->d_gp))
+            return set_error(NIDO_ERR_STATE, "work lists hold stale delta buffers");
     ++g->iters_since_singleton;
     NIDO_TRY(launch_or_replay(g, inc));
     g->inc_ok = any_w && !g->windowed;   // both forms leave term/acc describing the new partition
@@ -1477,6 +1512,7 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
     const int G = g->G;
     const double t0 = now_s();
     g->ident = false;
+    g->rebuilds = 0;
     g->bres.assign((size_t)G, BatchResult{});
     std::vector<unsigned char> alive((size_t)G, 1), act((size_t)G), flag((size_t)G);
     std::vector<int64_t> base((size_t)G, 0);   // subtracted from clus to give graph-local ids
@@ -1540,6 +1576,19 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
                 // terms and sums again: the next iteration sweeps
                 for (int i = 0; i < G; ++i)
                     if (flag[(size_t)i] && act[(size_t)i]) g->inc_ok = false;
+            }
+            // Stopped graphs leave the work lists once the active ones hold
+            // at most the set share of the listed vertices (their rows
+            // would only be skipped; C' = C holds for them from here on).
+            const int pct = g_list_rebuild_pct.load();
+            if (G > 1 && pct > 0) {
+                int64_t live = 0;
+                for (int i = 0; i < G; ++i)
+                    if (act[(size_t)i]) live += g->vb[(size_t)i + 1] - g->vb[(size_t)i];
+                if (live > 0 && live < g->listed && live * 100 <= g->listed * pct) {
+                    NIDO_TRY(build_lists(g, act.data()));
+                    ++g->rebuilds;
+                }
             }
             ++iters;
         }
@@ -1931,6 +1980,13 @@ int nido_set_device_lists(int64_t min_vertices)
     return NIDO_OK;
 }
 
+int nido_set_list_rebuild(int percent)
+{
+    if (percent > 100) return set_error(NIDO_ERR_INVALID, "percent must be <= 100");
+    g_list_rebuild_pct.store(percent < 0 ? kListRebuildPct : percent);
+    return NIDO_OK;
+}
+
 int nido_device_count(int* count)
 {
     if (!count) return set_error(NIDO_ERR_INVALID, "null count");
@@ -2292,6 +2348,13 @@ int nido_batch_size(const nido_graph* g, int* ngraphs)
 {
     if (!g || !ngraphs) return set_error(NIDO_ERR_INVALID, "null argument");
     *ngraphs = g->G;
+    return NIDO_OK;
+}
+
+int nido_batch_list_rebuilds(const nido_graph* g, int64_t* n)
+{
+    if (!g || !n) return set_error(NIDO_ERR_INVALID, "null argument");
+    *n = g->rebuilds;
     return NIDO_OK;
 }
 

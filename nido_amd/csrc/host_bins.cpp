@@ -194,7 +194,8 @@ void merge_batches(const std::vector<BatchWork>& W, int nb, bool rows_on_host, P
 }  // namespace
 
 void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::vector<int64_t>& parts,
-                      const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb)
+                      const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb,
+                      const unsigned char* keep)
 {
     const size_t nb1 = (size_t)nb + 1;
     auto for_batch = [&](int b, auto&& fn) {
@@ -202,7 +203,8 @@ void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::
             fn(parts[(size_t)b], parts[(size_t)b + 1]);
             return;
         }
-        for (int gg = 0; gg < G; ++gg) fn(gparts[(size_t)gg * nb1 + b], gparts[(size_t)gg * nb1 + b + 1]);
+        for (int gg = 0; gg < G; ++gg)
+            if (!keep || keep[gg]) fn(gparts[(size_t)gg * nb1 + b], gparts[(size_t)gg * nb1 + b + 1]);
     };
     std::vector<BatchWork> W((size_t)nb);
     // Short rows: vertices grouped by (batch, bin), ascending id inside a
@@ -242,7 +244,7 @@ void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::
 }
 
 void bin_tiles(int nb, int G, const std::vector<int64_t>& parts, const std::vector<int64_t>& gparts,
-               std::vector<BinTile>& tiles, std::vector<int64_t>& tb0)
+               const unsigned char* keep, std::vector<BinTile>& tiles, std::vector<int64_t>& tb0)
 {
     const size_t nb1 = (size_t)nb + 1;
     tiles.clear();
@@ -255,7 +257,8 @@ void bin_tiles(int nb, int G, const std::vector<int64_t>& parts, const std::vect
         };
         if (G == 1) add(parts[(size_t)b], parts[(size_t)b + 1]);
         else
-            for (int gg = 0; gg < G; ++gg) add(gparts[(size_t)gg * nb1 + b], gparts[(size_t)gg * nb1 + b + 1]);
+            for (int gg = 0; gg < G; ++gg)
+                if (!keep || keep[gg]) add(gparts[(size_t)gg * nb1 + b], gparts[(size_t)gg * nb1 + b + 1]);
         const int64_t t0 = tb0[(size_t)b], nt = (int64_t)tiles.size() - t0;
         for (int64_t i = 0; i < nt; ++i) {
             tiles[(size_t)(t0 + i)].at = NUM_BINS * t0 + i;

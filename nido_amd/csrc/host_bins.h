@@ -30,7 +30,8 @@ int64_t max_degree(const int64_t* off, int64_t nv);
 
 // Degree bins and long-row work of every batch.  Batch b of the handle is
 // [parts[b], parts[b+1]) for one graph, or the union over graphs g of
-// [gparts[g (nb+1) + b], gparts[g (nb+1) + b + 1]).
+// [gparts[g (nb+1) + b], gparts[g (nb+1) + b + 1]) (over the graphs with
+// keep[g] set when keep is given).
 struct PhaseBins {
     std::vector<BinRow> binv;          // per batch: its rows by bin (ascending id inside a bin), long rows last
     std::vector<BigItem> bitems;       // per batch: partition items, longest row first
@@ -44,13 +45,14 @@ struct PhaseBins {
     int64_t tab_need = 0, part_need = 0, slot_need = 0, key_need = 0, cpo_need = 0;
 };
 void build_phase_bins(const int64_t* off, int64_t nv, int nb, int G, const std::vector<int64_t>& parts,
-                      const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb);
+                      const std::vector<int64_t>& gparts, const nido_tuning& t, PhaseBins& pb,
+                      const unsigned char* keep = nullptr);
 
 // Tiles of the device bin builder (BinTile): batch by batch, each batch's
-// vertex ranges (graph by graph) cut into tiles of kBinTileVerts; tb0 gets
-// nb + 1 tile bounds.
+// vertex ranges (graph by graph, the graphs with keep[g] set when keep is
+// given) cut into tiles of kBinTileVerts; tb0 gets nb + 1 tile bounds.
 void bin_tiles(int nb, int G, const std::vector<int64_t>& parts, const std::vector<int64_t>& gparts,
-               std::vector<BinTile>& tiles, std::vector<int64_t>& tb0);
+               const unsigned char* keep, std::vector<BinTile>& tiles, std::vector<int64_t>& tb0);
 // The lists of a phase whose binv the device built: cnt[b * NUM_BINS + k]
 // rows of batch b in bin k, big[b] the long rows of batch b (ascending id),
 // bverts / barcs the vertices and arcs of batch b.  Leaves pb.binv empty.

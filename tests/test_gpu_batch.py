@@ -48,7 +48,9 @@ def _check(na, graphs, nb, max_loops=500, tol=1e-6, tol_phase=None, **tuning):
             assert abs(ps.q - rp["q"]) <= Q_TOL and abs(ps.q0 - rp["q0"]) <= Q_TOL, f"graph {i} phase {p}"
         assert abs(gs.q - r["q"]) <= Q_TOL, f"graph {i}: Q {gs.q} vs {r['q']}"
         assert np.array_equal(bg.clustering(i), r["clustering"]), f"graph {i}: clustering differs"
+    rebuilds = bg.list_rebuilds()
     bg.close()
+    return rebuilds
 
 
 @pytest.mark.parametrize("nb", [1, 4, 32])

@@ -168,6 +168,19 @@ def test_device_entry_points_validate_before_touching_the_gpu():
         na.GraphGPU((off, np.array([1, 0]), np.array([1.0, 1.0])), 0)
 
 
+def test_process_wide_settings_validate_their_arguments():
+    import ctypes as C
+    with pytest.raises(na.NidoError):
+        na.set_list_rebuild(101)
+    with pytest.raises(na.NidoError):
+        na.set_device_lists(-1)
+    na.set_list_rebuild(0)
+    na.set_list_rebuild(-1)    # back to the default
+    n = C.c_int64()
+    assert na.lib.nido_batch_list_rebuilds(None, C.byref(n)) == -1
+    assert b"null argument" in na.lib.nido_last_error()
+
+
 def test_weight_totals_that_would_wrap_the_exact_sums_are_refused():
     """Each |w| below 2^62 is not enough: the Q64.64 sums (2m, k_v, a_c) need
     the total of |w| below 2^62 too.  The path 0-1-2 with two edges of
