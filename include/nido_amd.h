@@ -85,7 +85,7 @@ typedef struct {
     int32_t full_modularity;  /* how Q is evaluated after an iteration (the same Q either way): 0 (default)
                                  the exact incremental update over the moved rows after each phase's first
                                  iteration, for a handle of several graphs only when the iteration before
-                                 moved at most 1/32 of the arcs (else a sweep of every arc); 1 always the
+                                 moved at most 1/8 of the arcs (else a sweep of every arc); 1 always the
                                  sweep; 2 the update after each phase's first iteration (0/1/2) */
     int32_t fixed_short;      /* 1 (default): rows up to big_min_deg arcs sum in 64-bit fixed point when every
                                  weight is a multiple of one power of two 2^-F and those rows' sums stay below

@@ -1134,7 +1134,7 @@ int iteration_launches(nido_graph* g, bool inc)
 }
 
 constexpr int64_t kGraphMinArcs = int64_t(1) << 28;
-constexpr int64_t kIncShare = 32;
+constexpr int64_t kIncShare = 8;
 
 // Whether this phase's iterations are replayed from a hipGraph.
 bool replaying(const nido_graph* g)
