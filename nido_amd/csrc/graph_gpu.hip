@@ -11,6 +11,7 @@
 #include <mutex>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -133,7 +134,6 @@ struct PhaseHist {
 struct BatchResult {
     nido_run_stats st{};
     std::vector<nido_phase_stats> phases;
-    std::vector<int64_t> clustering;   // original vertex -> final community, graph-local ids
 };
 
 constexpr int64_t kSegArcs = 65536;    // arcs per per-graph modularity item
@@ -367,6 +367,11 @@ struct nido_graph {
                         d_gid,     d_gp,      d_twomg,  nb + 1,    moved()};
     }
     std::vector<BatchResult> bres;       // per graph, from the last batched run
+    // every graph's clustering from the last batched run (original vertex ->
+    // final community, graph-local ids; graph i at [vb0[i], vb0[i+1]), or
+    // all n0 vertices for one graph), left uninitialised until copied
+    std::unique_ptr<int64_t[]> bclus;
+    int64_t bclus_n = 0;
     // ---- arcs streamed through a device window (graph_gpu.cpp:341-417, 967-1027) ----
     int64_t window = 0;                  // arcs the device holds at once; 0: every phase resident
     bool windowed = false;               // this phase has more arcs than the window
@@ -382,6 +387,7 @@ namespace {
 
 template <typename T>
 int d2h(nido_graph* g, T* dst, const T* src, int64_t n);
+int host_clustering(nido_graph* g, const int64_t* src);
 
 // Device bytes of a resident graph: per vertex (state, bins, scratch) and per
 // arc at the peak of a phase (CSR, 32-bit mirrors, fixed-point weights, the
@@ -1666,17 +1672,21 @@ int do_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phase,
             if (cand[(size_t)i]) g->bres[(size_t)i].phases.back().t_agg = at;
         if (!any(cont)) break;
     }
-    // clusterings: graph-local ids
-    std::vector<int64_t> clus((size_t)std::max<int64_t>(g->n0, 1));
-    if (g->n0)
-        HIP_TRY(hipMemcpyAsync(clus.data(), g->d_clus, sizeof(int64_t) * (size_t)g->n0, hipMemcpyDeviceToHost,
-                               g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    for (int i = 0; i < G; ++i) {
-        BatchResult& R = g->bres[(size_t)i];
-        const int64_t o0 = g->vb0[(size_t)i], o1 = g->vb0[(size_t)i + 1];
-        R.clustering.resize((size_t)(o1 - o0));
-        for (int64_t v = o0; v < o1; ++v) R.clustering[(size_t)(v - o0)] = clus[(size_t)v] - base[(size_t)i];
+    // clusterings: graph-local ids on the device, then one copy to the host
+    if (g->n0) {
+        int64_t *d_loc = nullptr, *d_base = nullptr;
+        int rc = dev_alloc(&d_loc, g->n0);
+        if (rc == NIDO_OK) rc = dev_alloc(&d_base, G);
+        if (rc == NIDO_OK && hipMemcpyAsync(d_base, base.data(), sizeof(int64_t) * (size_t)G,
+                                            hipMemcpyHostToDevice, g->stream) != hipSuccess)
+            rc = set_error(NIDO_ERR_HIP, "clustering bases upload failed");
+        if (rc == NIDO_OK) {
+            launch_local_ids(g->d_clus, g->n0, g->d_gid0, d_base, d_loc, g->stream);
+            rc = host_clustering(g, d_loc);   // waits for the stream: base may go out of scope
+        }
+        dev_free(d_loc);
+        dev_free(d_base);
+        NIDO_TRY(rc);
     }
     S.t_total = now_s() - t0;
     S.phases = nphase;
@@ -1946,6 +1956,17 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     }
     if ((rc = do_singleton(g))) return fail(rc);
     *out = g;
+    return NIDO_OK;
+}
+
+// The n0 entries of src (device) into the handle's host clustering.
+int host_clustering(nido_graph* g, const int64_t* src)
+{
+    if (g->bclus_n != g->n0) {
+        g->bclus.reset(new int64_t[(size_t)std::max<int64_t>(g->n0, 1)]);   // no zero fill: copied over
+        g->bclus_n = g->n0;
+    }
+    HIP_TRY(staged_d2h(g->bclus.get(), src, sizeof(int64_t) * (size_t)g->n0, g->stream));
     return NIDO_OK;
 }
 
@@ -2328,8 +2349,7 @@ int nido_batch_run(nido_graph* g, int64_t max_loops, double tol, double tol_phas
                   g->bres.assign(1, BatchResult{});
                   g->bres[0].st = S;
                   for (const auto& h : g->hist) g->bres[0].phases.push_back(h.st);
-                  g->bres[0].clustering.resize((size_t)g->n0);
-                  NIDO_TRY(d2h(g, g->bres[0].clustering.data(), g->d_clus, g->n0));
+                  NIDO_TRY(host_clustering(g, g->d_clus));
                   if (stats) *stats = S;
                   return NIDO_OK;)
     }
@@ -2387,8 +2407,9 @@ int nido_batch_clustering(const nido_graph* g, int graph, int64_t* comm)
 {
     NIDO_TRY(check_result(g, graph));
     if (!comm) return set_error(NIDO_ERR_INVALID, "null argument");
-    const auto& c = g->bres[(size_t)graph].clustering;
-    std::copy(c.begin(), c.end(), comm);
+    const int64_t o0 = g->G == 1 ? 0 : g->vb0[(size_t)graph];
+    const int64_t o1 = g->G == 1 ? g->n0 : g->vb0[(size_t)graph + 1];
+    if (o1 > o0) std::memcpy(comm, g->bclus.get() + o0, sizeof(int64_t) * (size_t)(o1 - o0));
     return NIDO_OK;
 }
 

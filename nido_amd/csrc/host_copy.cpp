@@ -143,6 +143,52 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
     return e;
 }
 
+hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    constexpr int64_t kD2hMinBytes = int64_t(64) << 20;
+    const size_t kChunk = (size_t)g_chunk_bytes.load();
+    if (bytes < (size_t)std::min(g_min_bytes.load(), kD2hMinBytes)) {
+        hipError_t e = bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+        return e != hipSuccess ? e : hipStreamSynchronize(s);
+    }
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), kChunk);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+    }
+    const char* in = static_cast<const char*>(src);
+    char* out = static_cast<char*>(dst);
+    const size_t nchunks = (bytes + kChunk - 1) / kChunk;
+    auto issue = [&](size_t k) -> hipError_t {   // chunk k into buffer k & 1
+        const size_t at = k * kChunk, n = std::min(kChunk, bytes - at);
+        hipError_t r = hipMemcpyAsync(buf[k & 1], in + at, n, hipMemcpyDeviceToHost, s);
+        return r == hipSuccess ? hipEventRecord(done[k & 1], s) : r;
+    };
+    for (size_t k = 0; k < std::min<size_t>(2, nchunks) && e == hipSuccess; ++k) e = issue(k);
+    for (size_t k = 0; k < nchunks && e == hipSuccess; ++k) {
+        e = hipEventSynchronize(done[k & 1]);   // chunk k has landed in its buffer
+        if (e != hipSuccess) break;
+        const size_t at = k * kChunk, n = std::min(kChunk, bytes - at);
+        constexpr size_t kPiece = size_t(1) << 20;
+        const int64_t pieces = (int64_t)((n + kPiece - 1) / kPiece);
+#pragma omp parallel for schedule(static)
+        for (int64_t p = 0; p < pieces; ++p) {
+            const size_t o = (size_t)p * kPiece;
+            std::memcpy(out + at + o, buf[k & 1] + o, std::min(kPiece, n - o));
+        }
+        if (k + 2 < nchunks) e = issue(k + 2);   // the buffer is free again
+    }
+    const hipError_t w = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = w;
+    for (int i = 0; i < 2; ++i) {
+        if (done[i]) (void)hipEventDestroy(done[i]);
+        if (buf[i]) (void)hipHostFree(buf[i]);
+    }
+    return e;
+}
+
 hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s, ArcCheck* chk)
 {
     const int64_t kChunk = g_chunk_bytes.load() / (int64_t)sizeof(int);   // ids per pinned buffer

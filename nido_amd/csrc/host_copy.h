@@ -36,6 +36,13 @@ bool staged_path(size_t bytes);
 // pageable hipMemcpy stages through one thread.  Returns the first HIP error.
 hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 
+// Copies `bytes` from device memory to pageable host memory on stream s and
+// waits for the copy.  From 64 MiB (or the staging threshold, if lower) the
+// copy goes through two pinned buffers: while one chunk crosses the host
+// link, the chunk before it is copied out of the other by all host threads
+// (which also take the destination's first-touch page faults in parallel).
+hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // The same for n ids below 2^31 stored as int64 on the host, sent as int32
 // (narrowed while staging): half the bytes over the host link.  Pinned
 // staging from the same byte threshold as staged_h2d (2^28 ids by default).

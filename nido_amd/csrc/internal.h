@@ -314,6 +314,9 @@ void launch_mark_communities(const int64_t* C, int64_t nv, int64_t* flag, hipStr
                              const int* gid = nullptr, const unsigned char* keep = nullptr);
 void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const int64_t* newid,
                               hipStream_t s, const int* gid0 = nullptr, const unsigned char* keep = nullptr);
+// out[i] = clus[i] - base[gid0[i]] for the n0 original vertices.
+void launch_local_ids(const int64_t* clus, int64_t n0, const int* gid0, const int64_t* base, int64_t* out,
+                      hipStream_t s);
 void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t* off, int64_t nv,
                           int64_t* cnt, int64_t* arcs, hipStream_t s, const int* gid = nullptr,
                           const unsigned char* keep = nullptr);

@@ -469,6 +469,16 @@ clustering_kernel(int64_t* __restrict__ clus, int64_t n0, const int64_t* __restr
         if (!gid0 || keep[gid0[i]]) clus[i] = newid[C[clus[i]]];
 }
 
+// out[i] = clus[i] - base[graph of i]: every graph's clustering in its own
+// ids.
+__global__ void __launch_bounds__(kBlock)
+local_ids_kernel(const int64_t* __restrict__ clus, int64_t n0, const int* __restrict__ gid0,
+                 const int64_t* __restrict__ base, int64_t* __restrict__ out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n0; i += (int64_t)gridDim.x * kBlock)
+        out[i] = clus[i] - base[gid0[i]];
+}
+
 // The coarse rows of a workgroup's kBlock consecutive vertices, grouped in
 // an LDS table of 2 * kBlock slots (at most kBlock keys: half full at most,
 // so probing ends): vertices of one community are often neighbours in id,
@@ -1062,6 +1072,12 @@ void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const
                               hipStream_t s, const int* gid0, const unsigned char* keep)
 {
     if (n0 > 0) clustering_kernel<<<grid_for(n0, kBlock), kBlock, 0, s>>>(clus, n0, C, newid, gid0, keep);
+}
+
+void launch_local_ids(const int64_t* clus, int64_t n0, const int* gid0, const int64_t* base, int64_t* out,
+                      hipStream_t s)
+{
+    if (n0 > 0) local_ids_kernel<<<grid_for(n0, kBlock), kBlock, 0, s>>>(clus, n0, gid0, base, out);
 }
 
 void launch_member_counts(const int64_t* C, const int64_t* newid, const int64_t* off, int64_t nv,

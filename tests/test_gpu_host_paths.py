@@ -43,6 +43,27 @@ def test_pinned_staged_upload_with_many_chunks():
         na.set_upload_staging(4096, 1000)          # chunks are multiples of 4096
 
 
+def test_pinned_staged_download_of_clusterings():
+    """A batched run's clusterings come back through the two pinned buffers
+    (graph-local ids made on the device): forced with 16 KiB chunks, dozens
+    of hand-offs for 64 graphs, and the one-graph form of the batched API."""
+    from test_gpu_batch import _check
+    na = _na()
+    rng = np.random.default_rng(11)
+    graphs = []
+    for i in range(64):
+        n = int(rng.integers(200, 1500))
+        graphs.append(na.gen_sbm(n, 64, 12 / 63, 2 / max(n - 64, 1), True, 3000 + i).arrays())
+    na.set_upload_staging(4096, 16384)
+    try:
+        print("case: 64 sbms nb=8, clusterings through 16 KiB pinned chunks", flush=True)
+        _check(na, graphs, 8)
+        print("case: batch of one graph, clustering through 16 KiB pinned chunks", flush=True)
+        _check(na, [na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 9).arrays()], 8)
+    finally:
+        na.set_upload_staging(0, 0)
+
+
 def test_checks_made_while_staging_match_the_host_checks():
     """Inputs that take the pinned staging path are checked while they pass
     through it (arc columns against their graph, weights finite and below
