@@ -107,12 +107,14 @@ void nido_div_rcp(int64_t n, const double* a, const double* b, double* out);
 int nido_abi_version(void);
 /* Number of HIP devices visible (0 on a machine without a GPU). */
 int nido_device_count(int* count);
-/* Graph creation copies large inputs to the device through two pinned
- * staging buffers (ids narrowed to int32 on the way): from min_bytes crossing
- * the host link, chunk_bytes per buffer (a multiple of 4096); 0 restores the
- * defaults, 1 GiB and 64 MiB.  Process-wide.  No counterpart in the reference
- * (its uploads are plain pageable copies, graph_gpu.cpp:967-1027); lowering
- * it lets small inputs take the pinned path. */
+/* Graph creation copies large inputs to the device, and a batched run its
+ * clusterings back, through two pinned staging buffers (ids narrowed to
+ * int32 on the way up; the buffers are pooled and reused): from min_bytes
+ * crossing the host link, chunk_bytes per buffer (a multiple of 4096); 0
+ * restores the defaults, 256 MiB and 64 MiB.  Process-wide.  No counterpart
+ * in the reference (its uploads are plain pageable copies,
+ * graph_gpu.cpp:967-1027); lowering it lets small inputs take the pinned
+ * path. */
 int nido_set_upload_staging(int64_t min_bytes, int64_t chunk_bytes);
 /* Phases of at least min_vertices vertices whose arcs are resident build
  * their degree bins and sweep row lists on the device, smaller ones on the

@@ -196,7 +196,7 @@ def device_count() -> int:
 
 def set_upload_staging(min_bytes: int = 0, chunk_bytes: int = 0) -> None:
     """Pinned staging of graph uploads from min_bytes in chunk_bytes pieces
-    (0: the defaults, 1 GiB and 64 MiB); process-wide."""
+    (0: the defaults, 256 MiB and 64 MiB); process-wide."""
     _check(lib.nido_set_upload_staging(int(min_bytes), int(chunk_bytes)), "nido_set_upload_staging")
 
 

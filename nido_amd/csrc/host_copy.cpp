@@ -16,12 +16,93 @@ namespace nido {
 
 namespace {
 // Copies of at least kMinBytes go through two pinned buffers of kChunkBytes
-// each (below 1 GiB the runtime's own staging is as fast); both can be
+// each (a pageable copy of 388 MB crossed at 26 GB/s, the staged copies at
+// ~51 GB/s; below a few chunks there is little to overlap); both can be
 // lowered so that tests reach the pinned path with small inputs.
-constexpr int64_t kDefaultMinBytes = int64_t(1) << 30;
+constexpr int64_t kDefaultMinBytes = int64_t(256) << 20;
 constexpr int64_t kDefaultChunkBytes = int64_t(64) << 20;
 std::atomic<int64_t> g_min_bytes{kDefaultMinBytes};
 std::atomic<int64_t> g_chunk_bytes{kDefaultChunkBytes};
+
+// A pair of pinned staging buffers with their events, taken from a
+// process-wide pool and given back when the copy is done: pinning and
+// unpinning 128 MiB costs milliseconds per copy.  The pool keeps at most
+// kPoolPairs idle pairs (concurrent copies beyond that pin their own and
+// release them); the pairs live until the process ends.
+constexpr size_t kPoolPairs = 2;
+struct Pair {
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    int device = -1;
+};
+std::mutex g_pool_mu;
+std::vector<Pair>* g_pool = new std::vector<Pair>();   // never destroyed: no HIP calls at exit
+
+void free_pair(Pair& p)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (p.done[i]) (void)hipEventDestroy(p.done[i]);
+        if (p.buf[i]) (void)hipHostFree(p.buf[i]);
+        p.done[i] = nullptr;
+        p.buf[i] = nullptr;
+    }
+}
+
+class Staging {
+  public:
+    explicit Staging(size_t bytes)
+    {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        {
+            std::lock_guard<std::mutex> lk(g_pool_mu);
+            for (size_t i = 0; i < g_pool->size(); ++i)
+                if ((*g_pool)[i].bytes == bytes && (*g_pool)[i].device == dev) {
+                    p_ = (*g_pool)[i];
+                    g_pool->erase(g_pool->begin() + (ptrdiff_t)i);
+                    return;
+                }
+        }
+        p_.bytes = bytes;
+        p_.device = dev;
+        for (int i = 0; i < 2 && err_ == hipSuccess; ++i) {
+            err_ = hipHostMalloc(reinterpret_cast<void**>(&p_.buf[i]), bytes);
+            if (err_ == hipSuccess) err_ = hipEventCreateWithFlags(&p_.done[i], hipEventDisableTiming);
+        }
+    }
+    ~Staging()
+    {
+        if (err_ == hipSuccess && reuse_) {
+            std::lock_guard<std::mutex> lk(g_pool_mu);
+            if (g_pool->size() < kPoolPairs) {
+                g_pool->push_back(p_);
+                return;
+            }
+        }
+        free_pair(p_);
+    }
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    hipError_t status() const { return err_; }
+    // Marks a copy that ended without error: only then does the pair go
+    // back to the pool.
+    hipError_t finish(hipError_t e)
+    {
+        reuse_ = e == hipSuccess;
+        return e;
+    }
+    // The caller's stream has finished every copy out of or into the
+    // buffers before the pair goes back (each copy ends with a stream
+    // synchronisation), so the events may be recorded again at once.
+    char* buf(int i) const { return p_.buf[i]; }
+    hipEvent_t done(int i) const { return p_.done[i]; }
+
+  private:
+    Pair p_;
+    hipError_t err_ = hipSuccess;
+    bool reuse_ = false;
+};
 
 void atomic_min(std::atomic<int64_t>& a, int64_t x)
 {
@@ -111,13 +192,10 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
         hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
         return e != hipSuccess ? e : hipStreamSynchronize(s);
     }
-    char* buf[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr};
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), kChunk);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
-    }
+    Staging st(kChunk);
+    hipError_t e = st.status();
+    char* const buf[2] = {st.buf(0), st.buf(1)};
+    const hipEvent_t done[2] = {st.done(0), st.done(1)};
     const char* in = static_cast<const char*>(src);
     char* out = static_cast<char*>(dst);
     for (size_t at = 0, i = 0; e == hipSuccess && at < bytes; at += kChunk, i ^= 1) {
@@ -136,28 +214,20 @@ hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s)
     }
     const hipError_t w = hipStreamSynchronize(s);
     if (e == hipSuccess) e = w;
-    for (int i = 0; i < 2; ++i) {
-        if (done[i]) (void)hipEventDestroy(done[i]);
-        if (buf[i]) (void)hipHostFree(buf[i]);
-    }
-    return e;
+    return st.finish(e);
 }
 
 hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t s)
 {
-    constexpr int64_t kD2hMinBytes = int64_t(64) << 20;
     const size_t kChunk = (size_t)g_chunk_bytes.load();
-    if (bytes < (size_t)std::min(g_min_bytes.load(), kD2hMinBytes)) {
+    if (bytes < (size_t)g_min_bytes.load()) {
         hipError_t e = bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
         return e != hipSuccess ? e : hipStreamSynchronize(s);
     }
-    char* buf[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr};
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), kChunk);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
-    }
+    Staging st(kChunk);
+    hipError_t e = st.status();
+    char* const buf[2] = {st.buf(0), st.buf(1)};
+    const hipEvent_t done[2] = {st.done(0), st.done(1)};
     const char* in = static_cast<const char*>(src);
     char* out = static_cast<char*>(dst);
     const size_t nchunks = (bytes + kChunk - 1) / kChunk;
@@ -182,11 +252,7 @@ hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t s)
     }
     const hipError_t w = hipStreamSynchronize(s);
     if (e == hipSuccess) e = w;
-    for (int i = 0; i < 2; ++i) {
-        if (done[i]) (void)hipEventDestroy(done[i]);
-        if (buf[i]) (void)hipHostFree(buf[i]);
-    }
-    return e;
+    return st.finish(e);
 }
 
 hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s, ArcCheck* chk)
@@ -205,13 +271,10 @@ hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s
         hipError_t e = hipMemcpyAsync(dst, t.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s);
         return e != hipSuccess ? e : hipStreamSynchronize(s);
     }
-    int* buf[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr};
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), sizeof(int) * (size_t)kChunk);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
-    }
+    Staging st(sizeof(int) * (size_t)kChunk);
+    hipError_t e = st.status();
+    int* const buf[2] = {reinterpret_cast<int*>(st.buf(0)), reinterpret_cast<int*>(st.buf(1))};
+    const hipEvent_t done[2] = {st.done(0), st.done(1)};
     for (int64_t at = 0, i = 0; e == hipSuccess && at < n; at += kChunk, i ^= 1) {
         const int64_t m = std::min(kChunk, n - at);
         e = hipEventSynchronize(done[i]);      // the buffer's previous copy has landed
@@ -229,24 +292,17 @@ hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s
     }
     const hipError_t w = hipStreamSynchronize(s);
     if (e == hipSuccess) e = w;
-    for (int i = 0; i < 2; ++i) {
-        if (done[i]) (void)hipEventDestroy(done[i]);
-        if (buf[i]) (void)hipHostFree(buf[i]);
-    }
-    return e;
+    return st.finish(e);
 }
 
 hipError_t staged_h2d_weights(double* dst, const double* src, int64_t n, hipStream_t s, ArcCheck& chk)
 {
     const int64_t kChunk = g_chunk_bytes.load() / (int64_t)sizeof(double);
     chk.wabs.assign((size_t)chk.G, 0.0);
-    double* buf[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr};
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-        e = hipHostMalloc(reinterpret_cast<void**>(&buf[i]), sizeof(double) * (size_t)kChunk);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
-    }
+    Staging st(sizeof(double) * (size_t)kChunk);
+    hipError_t e = st.status();
+    double* const buf[2] = {reinterpret_cast<double*>(st.buf(0)), reinterpret_cast<double*>(st.buf(1))};
+    const hipEvent_t done[2] = {st.done(0), st.done(1)};
     for (int64_t at = 0, i = 0; e == hipSuccess && at < n; at += kChunk, i ^= 1) {
         const int64_t m = std::min(kChunk, n - at);
         e = hipEventSynchronize(done[i]);      // the buffer's previous copy has landed
@@ -257,11 +313,7 @@ hipError_t staged_h2d_weights(double* dst, const double* src, int64_t n, hipStre
     }
     const hipError_t w = hipStreamSynchronize(s);
     if (e == hipSuccess) e = w;
-    for (int i = 0; i < 2; ++i) {
-        if (done[i]) (void)hipEventDestroy(done[i]);
-        if (buf[i]) (void)hipHostFree(buf[i]);
-    }
-    return e;
+    return st.finish(e);
 }
 
 int set_staging(int64_t min_bytes, int64_t chunk_bytes)

@@ -37,15 +37,15 @@ bool staged_path(size_t bytes);
 hipError_t staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Copies `bytes` from device memory to pageable host memory on stream s and
-// waits for the copy.  From 64 MiB (or the staging threshold, if lower) the
-// copy goes through two pinned buffers: while one chunk crosses the host
-// link, the chunk before it is copied out of the other by all host threads
-// (which also take the destination's first-touch page faults in parallel).
+// waits for the copy.  From the staging threshold the copy goes through two
+// pinned buffers: while one chunk crosses the host link, the chunk before it
+// is copied out of the other by all host threads (which also take the
+// destination's first-touch page faults in parallel).
 hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // The same for n ids below 2^31 stored as int64 on the host, sent as int32
 // (narrowed while staging): half the bytes over the host link.  Pinned
-// staging from the same byte threshold as staged_h2d (2^28 ids by default).
+// staging from the same byte threshold as staged_h2d (2^26 ids by default).
 hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s, ArcCheck* chk = nullptr);
 
 // Weights through the staging buffers, checked by chk on the way (the
@@ -53,7 +53,7 @@ hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s
 hipError_t staged_h2d_weights(double* dst, const double* src, int64_t n, hipStream_t s, ArcCheck& chk);
 
 // Pinned staging from min_bytes (bytes crossing the host link) in chunks of
-// chunk_bytes; 0 restores a default (1 GiB, 64 MiB).
+// chunk_bytes; 0 restores a default (256 MiB, 64 MiB).
 int set_staging(int64_t min_bytes, int64_t chunk_bytes);
 
 // out[i] = off[i] - off[0] for i <= n, split across threads.
