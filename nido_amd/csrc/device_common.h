@@ -30,6 +30,14 @@ __device__ __forceinline__ bool better(double g, int c, double bg, int bc)
     return c != kNone && (bc == kNone || g > bg || (g == bg && c < bc));
 }
 
+// Makes one lane's LDS writes visible to the other lanes of its wavefront.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ fx_t wave_sum_fx(fx_t x)
 {
 #pragma unroll

@@ -803,11 +803,10 @@ int setup_phase(nido_graph* g)
             NIDO_TRY(upload(&g->d_items, items, g->stream));
         }
 
-        launch_row_ids(g->d_off, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_src, g->stream);
         launch_narrow_ids(g->d_col, g->d_col32, g->ne, g->stream);
         unsigned long long* d_maxk = reinterpret_cast<unsigned long long*>(g->d_tot + 1);
         launch_vertex_weights(g->d_off, g->d_w, g->d_small, g->nsmall, g->d_items, g->nitems, g->d_k,
-                              g->d_kd, nv, g->d_acc + 2, d_maxk, g->d_part, g->stream);
+                              g->d_kd, nv, g->d_acc + 2, d_maxk, g->d_part, g->d_src, g->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(&twom, g->d_acc + 2, sizeof(fx_t), hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipMemcpyAsync(&maxk_bits, d_maxk, sizeof(maxk_bits), hipMemcpyDeviceToHost, g->stream));

@@ -196,16 +196,16 @@ void launch_fill_fx(fx_t* p, int64_t n, hipStream_t s);
 // Blocks of the reduction kernels, each leaving one partial sum.
 constexpr unsigned kPartBlocks = 1024;
 
-// k_v and 2m.  `small` lists rows handled by one lane, `items` slices of
-// longer rows (one wavefront each).  *acc receives sum(k) exactly, *maxabs
-// the bits of max |FP64(k_v)|; part holds kPartBlocks partials.
+// k_v and 2m, and src[e] = the row of arc e, in one pass over the arcs.
+// `small` lists rows of at most kSmallRowArcs arcs (kRowLanes lanes each),
+// `items` slices of longer rows (one wavefront each).  *acc receives sum(k)
+// exactly, *maxabs the bits of max |FP64(k_v)|; part holds kPartBlocks
+// partials.
+constexpr int kRowLanes = 8;
 void launch_vertex_weights(const int64_t* off, const double* w, const int* small, int64_t nsmall,
                            const RowItem* items, int64_t nitems, fx_t* k, double* kd,
-                           int64_t nv, fx_t* acc, unsigned long long* maxabs, fx_t* part,
+                           int64_t nv, fx_t* acc, unsigned long long* maxabs, fx_t* part, int* src,
                            hipStream_t s);
-// src[e] = row of arc e.
-void launch_row_ids(const int64_t* off, const int* small, int64_t nsmall, const RowItem* items,
-                    int64_t nitems, int* src, hipStream_t s);
 void launch_singleton(int64_t* C, int64_t* Cn, int* C32, fx_t* a, const fx_t* k, int64_t nv,
                       hipStream_t s);
 // *fneed = the fractional bits the finest weight needs (max over nonzero

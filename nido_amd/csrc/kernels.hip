@@ -25,20 +25,40 @@ __global__ void fill_kernel(T* __restrict__ p, T val, int64_t n)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
 vw_small_kernel(const int64_t* __restrict__ off, const double* __restrict__ w,
-                const int* __restrict__ small, int64_t n, fx_t* __restrict__ k)
+                const int* __restrict__ small, int64_t n, fx_t* __restrict__ k, int* __restrict__ src)
 {
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock) {
-        const int v = small[i];
+    // kRowLanes lanes per row (at most kSmallRowArcs arcs), each taking
+    // every kRowLanes-th arc: neighbouring lanes read neighbouring arcs, and
+    // the group's first lane adds up their exact partial sums (any order
+    // gives the same sum).  The partials meet in LDS: summed with 128-bit
+    // lane shuffles of width kRowLanes instead, some k_v came out wrong on
+    // gfx950, a different few on every run.
+    __shared__ fx_t part[kBlock];
+    const int lane = threadIdx.x & (kRowLanes - 1);
+    const int64_t stride = (int64_t)gridDim.x * (kBlock / kRowLanes);
+    for (int64_t i = blockIdx.x * (int64_t)(kBlock / kRowLanes) + threadIdx.x / kRowLanes; i < n; i += stride) {
+        const int v = small[i];   // uniform within the group
+        const int64_t e1 = off[v + 1];
         fx_t s = 0;
-        for (int64_t e = off[v]; e < off[v + 1]; ++e) s += fx_from_double(w[e]);
-        k[v] = s;
+        for (int64_t e = off[v] + lane; e < e1; e += kRowLanes) {
+            s += fx_from_double(w[e]);
+            src[e] = v;
+        }
+        part[threadIdx.x] = s;
+        wave_lds_sync();   // a group's lanes share one wavefront
+        if (lane == 0) {
+            fx_t t = 0;
+#pragma unroll
+            for (int j = 0; j < kRowLanes; ++j) t += part[threadIdx.x + j];
+            k[v] = t;
+        }
+        wave_lds_sync();   // read before the group's next row overwrites them
     }
 }
 
 __global__ void __launch_bounds__(kBlock)
 vw_items_kernel(const double* __restrict__ w, const RowItem* __restrict__ items, int64_t n,
-                fx_t* __restrict__ k)
+                fx_t* __restrict__ k, int* __restrict__ src)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
@@ -46,7 +66,10 @@ vw_items_kernel(const double* __restrict__ w, const RowItem* __restrict__ items,
     for (int64_t i = wid; i < n; i += nw) {
         const RowItem it = items[i];
         fx_t s = 0;
-        for (int64_t e = it.e0 + lane; e < it.e1; e += 64) s += fx_from_double(w[e]);
+        for (int64_t e = it.e0 + lane; e < it.e1; e += 64) {
+            s += fx_from_double(w[e]);
+            src[e] = (int)it.v;
+        }
         s = wave_sum_fx(s);
         if (lane == 0) acc_add(k + it.v, s);
     }
@@ -161,27 +184,6 @@ to_fixed_kernel(const double* __restrict__ w, int64_t* __restrict__ wq, int64_t 
 {
     for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock)
         wq[e] = (int64_t)(w[e] * scale);
-}
-
-// src[e] = the row of arc e, from the per-phase row lists.
-__global__ void __launch_bounds__(kBlock)
-src_small_kernel(const int64_t* __restrict__ off, const int* __restrict__ small, int64_t n,
-                 int* __restrict__ src)
-{
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock) {
-        const int v = small[i];
-        for (int64_t e = off[v]; e < off[v + 1]; ++e) src[e] = v;
-    }
-}
-
-__global__ void __launch_bounds__(kBlock)
-src_items_kernel(const RowItem* __restrict__ items, int64_t n, int* __restrict__ src)
-{
-    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const RowItem it = items[i];
-        for (int64_t e = it.e0 + threadIdx.x; e < it.e1; e += kBlock) src[e] = (int)it.v;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -941,7 +943,7 @@ void launch_fill_fx(fx_t* p, int64_t n, hipStream_t s)
 
 void launch_vertex_weights(const int64_t* off, const double* w, const int* small, int64_t nsmall,
                            const RowItem* items, int64_t nitems, fx_t* k, double* kd,
-                           int64_t nv, fx_t* acc, unsigned long long* maxabs, fx_t* part,
+                           int64_t nv, fx_t* acc, unsigned long long* maxabs, fx_t* part, int* src,
                            hipStream_t s)
 {
     launch_fill_i64(reinterpret_cast<int64_t*>(maxabs), 0, 1, s);
@@ -949,20 +951,12 @@ void launch_vertex_weights(const int64_t* off, const double* w, const int* small
     if (nv <= 0) return;
     launch_fill_fx(k, nv, s);
     if (nsmall > 0)
-        vw_small_kernel<<<grid_for(nsmall, kBlock), kBlock, 0, s>>>(off, w, small, nsmall, k);
+        vw_small_kernel<<<grid_for(nsmall * kRowLanes, kBlock), kBlock, 0, s>>>(off, w, small, nsmall, k, src);
     if (nitems > 0)
-        vw_items_kernel<<<grid_for(nitems, kBlock / 64), kBlock, 0, s>>>(w, items, nitems, k);
+        vw_items_kernel<<<grid_for(nitems, kBlock / 64), kBlock, 0, s>>>(w, items, nitems, k, src);
     const unsigned g = grid_for(nv, kBlock, kPartBlocks);
     round_kernel<<<g, kBlock, 0, s>>>(k, kd, nv, maxabs, part);
     sum_partials_kernel<<<1, kBlock, 0, s>>>(part, (int)g, acc);
-}
-
-void launch_row_ids(const int64_t* off, const int* small, int64_t nsmall, const RowItem* items,
-                    int64_t nitems, int* src, hipStream_t s)
-{
-    if (nsmall > 0)
-        src_small_kernel<<<grid_for(nsmall, kBlock), kBlock, 0, s>>>(off, small, nsmall, src);
-    if (nitems > 0) src_items_kernel<<<grid_for(nitems, 1), kBlock, 0, s>>>(items, nitems, src);
 }
 
 void launch_singleton(int64_t* C, int64_t* Cn, int* C32, fx_t* a, const fx_t* k, int64_t nv,

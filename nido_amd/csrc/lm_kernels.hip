@@ -172,14 +172,6 @@ __device__ __forceinline__ void sweep_own(const LmArgs& A, Mass m, double ki, in
     }
 }
 
-// Makes one lane's LDS writes visible to the other lanes of its wavefront.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Set bits of `mask` below this lane.
 __device__ __forceinline__ int lane_rank(unsigned long long mask)
 {
