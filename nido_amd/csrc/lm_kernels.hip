@@ -20,7 +20,8 @@
 //               arcs by partition (a counting sort per 2048-arc chunk) and
 //               workgroup (v, p) of launch 2 reads only its own buckets;
 //               with P <= 2 each workgroup reads the row and filters it.
-// Launch 1 = short-row bins + long-row key fill (23 KB LDS per workgroup);
+// Launch 1 = short-row bins + long-row key fill (20 KB LDS per workgroup,
+// 12 KB with fixed-point sums);
 // launch 2 = long-row partitions (40 KB LDS with the default 2048-slot
 // table).  Row loops keep two to eight independent loads in flight per lane.
 #include <type_traits>
@@ -313,7 +314,8 @@ __device__ __forceinline__ void thread_body(const LmArgs& A, const BinRow* __res
 }
 
 // G lanes per vertex (G <= 64), S-slot LDS table per vertex; 256/G vertices
-// per workgroup; LDS (256/G)*S*20 bytes carved from `lds`.
+// per workgroup; LDS (256/G)*S*20 bytes carved from `lds` (12 per slot with
+// fixed-point sums, which leave the high words out).
 template <int G, int S, bool Multi, bool Fixed>
 __device__ __forceinline__ void group_body(const LmArgs& A, const BinRow* __restrict__ verts, int64_t n, int bi, int nbk,
                            char* lds, double* sself)
@@ -482,13 +484,19 @@ __device__ __forceinline__ void keyfill_body(const LmArgs& A, const SmallBins& L
 }
 
 constexpr int kSmallLds = 20480;   // (256/G)*S*20 for every short-row bin
+constexpr int kSmallLdsFixed = 12288;   // (256/G)*S*12: fixed-point sums need no high words
+static_assert((kMaxParts + 1 + kBlock / 64) * 4 <= kSmallLdsFixed, "key-fill counters exceed the LDS buffer");
 static_assert((kMaxParts + 1 + kBlock / 64) * 4 <= kSmallLds, "key-fill counters exceed the LDS buffer");
 
+// Seven waves per SIMD; eight for a handle of many graphs with fixed-point
+// sums (12 KB of tables instead of 20 KB, registers held to 64 with a few
+// spilled): on 4,096 SBMs, whose rows are short, their iterations took
+// 0.339 s against 0.350 s, while the same form cost R-MAT scale 22 2%.
 template <bool Multi, bool Fixed>
-__global__ void __launch_bounds__(kBlock, 7)
+__global__ void __launch_bounds__(kBlock, Multi && Fixed ? 8 : 7)
 lm_small_kernel(LmArgs A, SmallBins L)
 {
-    __shared__ __align__(16) char lds[kSmallLds];
+    __shared__ __align__(16) char lds[Fixed ? kSmallLdsFixed : kSmallLds];
     __shared__ BlockRed R;
     __shared__ double sself[kBlock / 8];       // e_{v,own} per row, rounded
     // Every index into L is a constant (a dynamically indexed kernel
