@@ -149,6 +149,32 @@ void narrow_checked(int* b, const int64_t* src, int64_t at, int64_t m, ArcCheck&
     }
 }
 
+// Arcs [at, at + m) of src as 16-bit ids local to their graph (column minus
+// the graph's first vertex; every graph has at most 2^16 vertices), their
+// columns checked.
+void narrow_local_checked(uint16_t* b, const int64_t* src, int64_t at, int64_t m, ArcCheck& chk)
+{
+#pragma omp parallel
+    {
+        const int nt = omp_get_num_threads(), tid = omp_get_thread_num();
+        by_graph(chk, at + m * tid / nt, at + m * (tid + 1) / nt, [&](int g, int64_t j0, int64_t j1) {
+            const int64_t lo = chk.vptr[g], hi = chk.vptr[g + 1];
+            bool out = false;
+            for (int64_t j = j0; j < j1; ++j) {
+                const int64_t c = src[j];
+                b[j - at] = (uint16_t)(c - lo);
+                out |= c < lo || c >= hi;
+            }
+            if (out)
+                for (int64_t j = j0; j < j1; ++j)
+                    if (src[j] < lo || src[j] >= hi) {
+                        atomic_min(chk.bad, j);
+                        break;
+                    }
+        });
+    }
+}
+
 // Weights [at, at + m) of src copied into b, checked, |w| summed per graph.
 void weights_checked(double* b, const double* src, int64_t at, int64_t m, ArcCheck& chk)
 {
@@ -288,6 +314,26 @@ hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s
             for (int64_t j = 0; j < m; ++j) b[j] = (int)in[j];
         }
         e = hipMemcpyAsync(dst + at, b, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(done[i], s);
+    }
+    const hipError_t w = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = w;
+    return st.finish(e);
+}
+
+hipError_t staged_h2d_local_ids(uint16_t* dst, const int64_t* src, int64_t n, hipStream_t s, ArcCheck& chk)
+{
+    const int64_t kChunk = g_chunk_bytes.load() / (int64_t)sizeof(uint16_t);
+    Staging st(sizeof(uint16_t) * (size_t)kChunk);
+    hipError_t e = st.status();
+    uint16_t* const buf[2] = {reinterpret_cast<uint16_t*>(st.buf(0)), reinterpret_cast<uint16_t*>(st.buf(1))};
+    const hipEvent_t done[2] = {st.done(0), st.done(1)};
+    for (int64_t at = 0, i = 0; e == hipSuccess && at < n; at += kChunk, i ^= 1) {
+        const int64_t m = std::min(kChunk, n - at);
+        e = hipEventSynchronize(done[i]);      // the buffer's previous copy has landed
+        if (e != hipSuccess) break;
+        narrow_local_checked(buf[i], src, at, m, chk);
+        e = hipMemcpyAsync(dst + at, buf[i], sizeof(uint16_t) * (size_t)m, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(done[i], s);
     }
     const hipError_t w = hipStreamSynchronize(s);

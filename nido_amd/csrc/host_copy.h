@@ -48,6 +48,12 @@ hipError_t staged_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 // staging from the same byte threshold as staged_h2d (2^26 ids by default).
 hipError_t staged_h2d_ids(int* dst, const int64_t* src, int64_t n, hipStream_t s, ArcCheck* chk = nullptr);
 
+// The ids of a handle whose graphs all have at most 2^16 vertices, sent as
+// 16-bit ids local to their graph (a quarter of the int64 bytes; the device
+// adds each graph's first vertex back), through the staging buffers and
+// checked by chk on the way.
+hipError_t staged_h2d_local_ids(uint16_t* dst, const int64_t* src, int64_t n, hipStream_t s, ArcCheck& chk);
+
 // Weights through the staging buffers, checked by chk on the way (the
 // caller has made sure staged_path holds for n weights).
 hipError_t staged_h2d_weights(double* dst, const double* src, int64_t n, hipStream_t s, ArcCheck& chk);

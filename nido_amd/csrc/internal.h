@@ -314,6 +314,14 @@ void launch_mark_communities(const int64_t* C, int64_t nv, int64_t* flag, hipStr
                              const int* gid = nullptr, const unsigned char* keep = nullptr);
 void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const int64_t* newid,
                               hipStream_t s, const int* gid0 = nullptr, const unsigned char* keep = nullptr);
+// Arcs [e0, e1) belong to a graph whose first vertex is base.
+struct ArcPiece {
+    int64_t e0, e1;
+    int64_t base;
+};
+// col32[e] = base + local[e] over the pieces: ids that crossed the host link
+// as 16-bit graph-local ids.
+void launch_widen_local(const ArcPiece* pieces, int64_t n, const uint16_t* local, int* col32, hipStream_t s);
 // out[i] = clus[i] - base[gid0[i]] for the n0 original vertices.
 void launch_local_ids(const int64_t* clus, int64_t n0, const int* gid0, const int64_t* base, int64_t* out,
                       hipStream_t s);

@@ -471,6 +471,16 @@ clustering_kernel(int64_t* __restrict__ clus, int64_t n0, const int64_t* __restr
         if (!gid0 || keep[gid0[i]]) clus[i] = newid[C[clus[i]]];
 }
 
+__global__ void __launch_bounds__(kBlock)
+widen_local_kernel(const ArcPiece* __restrict__ pieces, int64_t n, const uint16_t* __restrict__ local,
+                   int* __restrict__ col32)
+{
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const ArcPiece p = pieces[i];
+        for (int64_t e = p.e0 + threadIdx.x; e < p.e1; e += kBlock) col32[e] = (int)(p.base + local[e]);
+    }
+}
+
 // out[i] = clus[i] - base[graph of i]: every graph's clustering in its own
 // ids.
 __global__ void __launch_bounds__(kBlock)
@@ -1066,6 +1076,11 @@ void launch_update_clustering(int64_t* clus, int64_t n0, const int64_t* C, const
                               hipStream_t s, const int* gid0, const unsigned char* keep)
 {
     if (n0 > 0) clustering_kernel<<<grid_for(n0, kBlock), kBlock, 0, s>>>(clus, n0, C, newid, gid0, keep);
+}
+
+void launch_widen_local(const ArcPiece* pieces, int64_t n, const uint16_t* local, int* col32, hipStream_t s)
+{
+    if (n > 0) widen_local_kernel<<<grid_for(n, 1, 16384), kBlock, 0, s>>>(pieces, n, local, col32);
 }
 
 void launch_local_ids(const int64_t* clus, int64_t n0, const int* gid0, const int64_t* base, int64_t* out,
