@@ -43,6 +43,39 @@ def test_pinned_staged_upload_with_many_chunks():
         na.set_upload_staging(4096, 1000)          # chunks are multiples of 4096
 
 
+def test_batched_ids_sent_as_16_bit_graph_local_ids():
+    """A batched handle whose graphs all have at most 2^16 vertices sends
+    its ids as 16-bit graph-local ids while staging, and the device puts
+    the graph bases back: the device CSR must equal the input exactly (one
+    graph of exactly 2^16 vertices, the largest local id 65535), and a
+    handle with a larger graph takes the int32 form."""
+    import ctypes as C
+    na = _na()
+
+    def device_csr(bg, vptr, off, col):
+        o = np.zeros(len(off), np.int64)
+        c = np.zeros(len(col), np.int64)
+        x = np.zeros(len(col), np.float64)
+        assert na.lib.nido_get_graph(bg._h, o.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     c.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     x.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        return o, c, x
+
+    small = [na.gen_sbm(n, 64, 12 / 63, 2 / (n - 64), True, 70 + n).arrays() for n in (300, 65536, 1000)]
+    large = small + [na.gen_sbm(70000, 64, 6 / 63, 1 / 70000, True, 77).arrays()]
+    na.set_upload_staging(4096, 16384)
+    try:
+        for name, parts in (("all graphs <= 2^16 vertices", small), ("one graph above 2^16", large)):
+            print(f"case: batched upload, {name}", flush=True)
+            vptr, off, col, w = na.concat_graphs(parts)
+            bg = na.BatchGPU(csr=(vptr, off, col, w), nbatches=4)
+            o2, c2, w2 = device_csr(bg, vptr, off, col)
+            assert np.array_equal(o2, off) and np.array_equal(c2, col) and np.array_equal(w2, w)
+            bg.close()
+    finally:
+        na.set_upload_staging(0, 0)
+
+
 def test_pinned_staged_download_of_clusterings():
     """A batched run's clusterings come back through the two pinned buffers
     (graph-local ids made on the device): forced with 16 KiB chunks, dozens
