@@ -43,6 +43,17 @@ completed graphs/s over the whole run; p50/p99 latency (submission to result)
 of the completed requests.  One graph per (scale, edge factor) is generated
 up front (seed derived from the pair), since generating a scale-20 graph
 takes longer than the mean gap between arrivals.
+
+--window ARCS (rmat, chunglu): the same iterations with the device holding at
+most ARCS arcs at a time (the CSR stays in pinned host memory and every batch
+streams its rows' arcs through the window); value includes those copies.
+
+--workload coloring: the reference's -c on the R-MAT graph (graph.cpp:547-694):
+random weights from its engines, the greedy colouring on the device, the
+(colour, id) renumbering and the CSR permutation; value = vertices coloured
+and renumbered per second, end to end on host arrays.  cpu_baseline: the
+oracle's colouring loop (one thread, as the reference's reproducible case)
+on the same graph with weights of the same range.
 """
 from __future__ import annotations
 
@@ -309,10 +320,47 @@ def bench_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_coloring(args):
+    """The -c colouring and reorder of the R-MAT graph (one JSON line)."""
+    import numpy as np
+    import nido_amd as na
+    na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 1).coloring(1)   # warm-up: first launches
+    t0 = time.perf_counter()
+    hg = na.gen_rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, True, args.seed)
+    off, col, _ = hg.arrays()             # copies: the colouring permutes hg in place
+    nv, ne = len(off) - 1, len(col)
+    log(f"coloring: rmat s{args.scale} ef{args.edge_factor}: {nv} vertices, {ne} arcs "
+        f"({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    no, colors, K = hg.coloring(1)
+    dt = time.perf_counter() - t0
+    hg.close()
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        r = np.random.default_rng(args.seed).integers(0, 2**31 - 1, nv)
+        tc = time.perf_counter()
+        _, kc = oracle.coloring(off, col, r)
+        tc = time.perf_counter() - tc
+        cpu = {"value": round(nv / tc / 1e6, 3), "unit": "M vertices/s", "cores": 1, "kind": "port",
+               "sample": f"the whole graph, colouring loop only (no reorder), {tc:.2f} s, {kc} colours"}
+    out = {
+        "metric": METRIC, "value": round(nv / dt / 1e6, 3), "unit": "M vertices/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": f"-c colouring + reorder of rmat-s{args.scale}-ef{args.edge_factor} "
+                               f"(0.57,0.19,0.19,0.05), seed {args.seed}, one engine",
+                   "vertices": nv, "arcs": ne, "colours": int(K)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["rmat", "chunglu", "batched", "serve"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "chunglu", "batched", "serve", "coloring"], default="rmat")
+    ap.add_argument("--window", type=int, default=0, help="rmat/chunglu: arcs the device holds at a time (0: all)")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=8, help="serve: workers (streams)")
@@ -372,6 +420,9 @@ def main():
     if args.workload == "serve":
         bench_serve(args)
         return
+    if args.workload == "coloring":
+        bench_coloring(args)
+        return
 
     import nido_amd as na
 
@@ -389,7 +440,9 @@ def main():
     nv, ne = len(off) - 1, len(col)
     log(f"graph: {name}: {nv} vertices, {ne} arcs ({time.perf_counter()-t0:.1f}s)")
 
-    g = na.GraphGPU(hg, args.batches)
+    if args.window:
+        name += f", arcs streamed through a device window of {args.window} arcs"
+    g = na.GraphGPU(hg, args.batches, window=args.window)
     if args.tuning:
         g.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
     if args.autotune:

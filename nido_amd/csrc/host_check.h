@@ -16,7 +16,8 @@ int check_csr(int64_t nv, const int64_t* off, const int64_t* col, const double* 
 int check_offsets(int64_t nv, const int64_t* off, const int64_t* col, const double* w);
 // The greedy colouring of Graph::coloring run by one thread (coloring.hip):
 // colors[i] = smallest colour not held by a neighbour j < i with
-// r[j] >= r[i]; off[nv + 1], col[off[nv]] (ids in [0, nv)), host arrays.
-int device_coloring(int64_t nv, const int64_t* off, const int* col, const int64_t* r, int* colors,
+// r[j] >= r[i]; off[nv + 1], col[off[nv]] (ids in [0, nv), narrowed to
+// int32 on the way to the device), host arrays.
+int device_coloring(int64_t nv, const int64_t* off, const int64_t* col, const int64_t* r, int* colors,
                     int64_t* rounds);
 }  // namespace nido

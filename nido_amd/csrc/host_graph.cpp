@@ -15,11 +15,38 @@
 #include "errors.h"
 #include "host_check.h"
 
+// Vectors whose resize leaves the new elements uninitialised: every element
+// of a CSR array is written right after the array is sized, and a zero fill
+// of gigabytes on one thread (with its page faults) costs more than the
+// writes that follow, which the threads that write take in parallel.
+template <typename T>
+struct NoInit : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <typename U>
+    NoInit(const NoInit<U>&) noexcept {}
+    template <typename U>
+    void construct(U* p) noexcept
+    {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <typename U, typename... A>
+    void construct(U* p, A&&... a)
+    {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <typename T>
+using hvec = std::vector<T, NoInit<T>>;
+
 struct nido_hgraph {
     int64_t nv = 0, ne = 0;
-    std::vector<int64_t> off;
-    std::vector<int64_t> col;
-    std::vector<double> w;
+    hvec<int64_t> off;
+    hvec<int64_t> col;
+    hvec<double> w;
 };
 
 namespace nido {
@@ -525,9 +552,14 @@ int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* 
         return NIDO_OK;
     }
     try {
-        for (int64_t e = 0; e < ne; ++e)
-            if (g->col[(size_t)e] < 0 || g->col[(size_t)e] >= nv)
-                return set_error(NIDO_ERR_INVALID, "arc %lld: column outside [0, %lld)", (long long)e, (long long)nv);
+        bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+        for (int64_t e = 0; e < ne; ++e) bad = bad || g->col[(size_t)e] < 0 || g->col[(size_t)e] >= nv;
+        if (bad)
+            for (int64_t e = 0; e < ne; ++e)
+                if (g->col[(size_t)e] < 0 || g->col[(size_t)e] >= nv)
+                    return set_error(NIDO_ERR_INVALID, "arc %lld: column outside [0, %lld)", (long long)e,
+                                     (long long)nv);
         std::vector<int64_t> r((size_t)nv);
         {
             std::vector<std::mt19937_64> eng((size_t)engines);
@@ -539,14 +571,10 @@ int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* 
                 r[(size_t)i] = (int64_t)x;
             }
         }
-        std::vector<int> col32((size_t)ne);
-        #pragma omp parallel for schedule(static)
-        for (int64_t e = 0; e < ne; ++e) col32[(size_t)e] = (int)g->col[(size_t)e];
         std::vector<int> color((size_t)nv);
         int64_t rounds = 0;
-        int rc = device_coloring(nv, g->off.data(), col32.data(), r.data(), color.data(), &rounds);
+        int rc = device_coloring(nv, g->off.data(), g->col.data(), r.data(), color.data(), &rounds);
         if (rc != NIDO_OK) return rc;
-        std::vector<int>().swap(col32);
         // (colour, id) order, graph.cpp:609-629: a counting sort by colour
         // keeps ids ascending within a colour
         int maxc = 0;
@@ -565,13 +593,13 @@ int nido_hgraph_coloring(nido_hgraph* g, int engines, int64_t* colors, int64_t* 
             for (int64_t i = 0; i < nv; ++i) colors[i] = color[(size_t)i];
         // renumbered CSR, graph.cpp:631-691: row pos is old row new_orders[pos]
         // with its arcs in their order, every column mapped through orders
-        std::vector<int64_t> noff((size_t)nv + 1, 0);
+        hvec<int64_t> noff((size_t)nv + 1, 0);
         for (int64_t p = 0; p < nv; ++p) {
             const int64_t v = new_orders[p];
             noff[(size_t)p + 1] = noff[(size_t)p] + (g->off[(size_t)v + 1] - g->off[(size_t)v]);
         }
-        std::vector<int64_t> ncol((size_t)ne);
-        std::vector<double> nw((size_t)ne);
+        hvec<int64_t> ncol((size_t)ne);
+        hvec<double> nw((size_t)ne);
         #pragma omp parallel for schedule(dynamic, 1024)
         for (int64_t p = 0; p < nv; ++p) {
             const int64_t v = new_orders[p];
