@@ -49,15 +49,15 @@ __global__ void __launch_bounds__(kBlock)
 bin_scatter_kernel(const BinTile* __restrict__ tiles, int64_t ntiles, const int64_t* __restrict__ off, int thread_max,
                    int64_t big_min, const int64_t* __restrict__ base, BinRow* __restrict__ binv)
 {
-    __shared__ int wc[kBlock / 64][NUM_BINS];   // this round's rows per wavefront and bin
-    __shared__ int run[NUM_BINS];               // rows of the tile's earlier rounds per bin
+    __shared__ int wc[kBlock / 64][NUM_BINS];   // this pass's rows per wavefront and bin
+    __shared__ int run[NUM_BINS];               // rows of the tile's earlier passes per bin
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const BinTile T = tiles[t];
         if (threadIdx.x < NUM_BINS) run[threadIdx.x] = 0;
         __syncthreads();
-        for (int64_t r0 = T.v0; r0 < T.v1; r0 += kBlock) {   // uniform
+        for (int64_t r0 = T.v0; r0 < T.v1; r0 += kBlock) {   // uniform; one pass per kBlock rows
             const int64_t v = r0 + threadIdx.x;
             int k = -1;
             int64_t d = 0, e0 = 0;
@@ -156,7 +156,7 @@ rowlist_scatter_kernel(const int64_t* __restrict__ off, int64_t nv, int64_t ntil
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t v0 = t * kBinTileVerts, v1 = v0 + kBinTileVerts < nv ? v0 + kBinTileVerts : nv;
-        int64_t ps = cs[t], pi = ci[t];   // positions of the round's first row and slice
+        int64_t ps = cs[t], pi = ci[t];   // positions of the pass's first row and slice
         for (int64_t r0 = v0; r0 < v1; r0 += kBlock) {   // uniform
             const int64_t v = r0 + threadIdx.x;
             int64_t e0 = 0, e1 = 0, fi = 0;
@@ -200,7 +200,7 @@ rowlist_scatter_kernel(const int64_t* __restrict__ off, int64_t nv, int64_t ntil
             }
             ps += ts;
             pi += ti;
-            __syncthreads();   // wts / wti read before the next round writes them
+            __syncthreads();   // wts / wti read before the next pass writes them
         }
     }
 }
