@@ -375,6 +375,7 @@ struct nido_graph {
     // ---- arcs streamed through a device window (graph_gpu.cpp:341-417, 967-1027) ----
     int64_t window = 0;                  // arcs the device holds at once; 0: every phase resident
     bool windowed = false;               // this phase has more arcs than the window
+    bool win_single = false;             // windowed, and every batch's arcs fit the window at once
     int* h_col32 = nullptr;              // the phase's CSR on the host (pinned) while windowed
     double* h_w = nullptr;
     int64_t h_cap = 0;
@@ -729,8 +730,14 @@ int setup_phase(nido_graph* g)
     g->windowed = G == 1 && g->window > 0 && g->ne > g->window;
     g->win_e0 = g->win_e1 = -1;
     if (g->windowed) NIDO_TRY(window_units(off.data(), g->parts, g->nb, g->window, g->units, g->first_unit));
+    // A windowed phase whose batches each fit the window at once can update
+    // Q incrementally: a batch's arcs are still in the window when its commit
+    // has listed the rows that moved.
+    g->win_single = g->windowed;
+    for (int b = 0; g->win_single && b < g->nb; ++b)
+        g->win_single = g->first_unit[(size_t)b + 1] - g->first_unit[(size_t)b] <= 1;
     const bool dev_lists = !g->windowed && nv >= g_device_lists_min.load();
-    if (!g->windowed) {
+    if (!g->windowed || g->win_single) {
         // incremental modularity: room for every row chunk of the largest batch
         int64_t need = 0;
         if (G == 1) {
@@ -921,8 +928,10 @@ int windowed_modularity(nido_graph* g)
         launch_modularity_rows(g->d_off, g->d_col32 - e0, g->d_w - e0, g->d_C32, g->units[u], g->units[u + 1],
                                g->d_acc, g->stream);
     }
-    launch_modularity_terms(g->d_a, g->nv, g->twom_d, g->d_acc, g->d_q, g->stream);
+    launch_modularity_terms(g->d_a, g->nv, g->twom_d, g->d_acc, g->d_q, g->stream, g->d_term);
     HIP_TRY(hipGetLastError());
+    // the pair the incremental update continues from
+    HIP_TRY(hipMemcpyAsync(g->d_iacc, g->d_acc, 2 * sizeof(fx_t), hipMemcpyDeviceToDevice, g->stream));
     return NIDO_OK;
 }
 
@@ -941,7 +950,7 @@ int do_modularity(nido_graph* g, double* q)
                               g->d_part, g->d_q, g->stream, g->G == 1 ? g->d_term : nullptr,
                               g->G == 1 ? g->d_iacc : nullptr, g->ident);
     }
-    g->inc_ok = g->G == 1 && !g->windowed;
+    g->inc_ok = g->G == 1 && (!g->windowed || g->win_single);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(g->h_q, g->d_q, sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
@@ -1114,11 +1123,17 @@ int iteration_launches(nido_graph* g, bool inc)
                 launch_commit_tracked_list(g->d_binv + g->blist0[(size_t)b], g->blistn[(size_t)b], b, g->d_C,
                                            g->d_Cn, g->d_C32, g->d_k, g->d_a, g->d_off, D, g->d_act, g->stream);
         }
+        // A windowed phase (every batch one window load) takes the batch's
+        // delta now, while its arcs are still in the window and before the
+        // next batch's commit changes any community.
+        if (g->windowed && g->parts[(size_t)b + 1] > g->parts[(size_t)b])
+            launch_mod_delta(b, g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_Cn, g->d_C32,
+                             g->d_col32 - g->win_e0, g->d_w - g->win_e0, g->d_a, g->twom_d, D, g->stream);
         g->committed[(size_t)b] = 1;
     }
     g->ride_delta = false;
     ProfScope ps(g->prof, g->stream, NIDO_K_MODULARITY, 24.0 * (double)g->ne + 24.0 * (double)g->nv);
-    if (inc && g->nb > 0)   // the earlier batches' deltas rode in the next batch's short-row launch
+    if (inc && g->nb > 0 && !g->windowed)   // the earlier batches' deltas rode in the next batch's short-row launch
         launch_mod_delta(g->nb - 1, g->G == 1 ? g->parts[(size_t)g->nb - 1] : 0,
                          g->G == 1 ? g->parts[(size_t)g->nb] : 0, g->d_Cn, g->d_C32, g->d_col32, g->d_w, g->d_a,
                          g->twom_d, D, g->stream);
@@ -1182,7 +1197,8 @@ int run_iteration_launches(nido_graph* g)
     // every later iteration; 1: never.
     const bool any_w = g->G > 1 || g->twom != 0;
     const int fm = g->tuning.full_modularity;
-    bool inc = any_w && g->symmetric && g->inc_ok && fm != 1 && g->iters_since_singleton > 0 && !g->windowed;
+    bool inc = any_w && g->symmetric && g->inc_ok && fm != 1 && g->iters_since_singleton > 0 &&
+               (!g->windowed || g->win_single);
     if (inc && fm == 0 && g->G > 1) inc = g->last_moved >= 0 && g->last_moved * kIncShare <= g->ne;
     // the short-row launches must carry the delta buffers' current addresses
     for (const SmallBins& L : g->small_bins)
@@ -1190,7 +1206,7 @@ int run_iteration_launches(nido_graph* g)
             return set_error(NIDO_ERR_STATE, "work lists hold stale delta buffers");
     ++g->iters_since_singleton;
     NIDO_TRY(launch_or_replay(g, inc));
-    g->inc_ok = any_w && !g->windowed;   // both forms leave term/acc describing the new partition
+    g->inc_ok = any_w && (!g->windowed || g->win_single);   // both forms leave term/acc describing the new partition
     return NIDO_OK;
 }
 
@@ -1981,6 +1997,8 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
         dev_free(d_h);
         if (!ok) return fail(set_error(NIDO_ERR_HIP, "arc symmetry check failed"));
         g->symmetric = h[0] == h[1] && h[2] == h[3];
+    } else if (g->windowed && ne > 0) {   // the arcs are on the host
+        g->symmetric = host_arcs_symmetric(nv, g->h_off.data(), g->h_col32, g->h_w);
     }
     if ((rc = do_singleton(g))) return fail(rc);
     *out = g;

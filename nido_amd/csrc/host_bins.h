@@ -66,6 +66,11 @@ void build_phase_lists(const int64_t* off, int nb, const std::vector<int64_t>& c
 // entries in first_unit).  A row longer than the window is refused.
 int window_units(const int64_t* off, const std::vector<int64_t>& parts, int nb, int64_t window,
                  std::vector<int64_t>& units, std::vector<int>& first_unit);
+// Whether every arc has a reverse arc of the same weight, on host arrays:
+// the multisets {(v, col, w)} and {(col, v, w)} compared through two keyed
+// hash sums of each (mod 2^64), as the device check for resident graphs
+// does; a wrong "true" has probability about 2^-128.
+bool host_arcs_symmetric(int64_t nv, const int64_t* off, const int* col, const double* w);
 // Exact vertex weights on the host: k (Q64.64), kd = RN(k), 2m and max |kd|.
 void host_vertex_weights(int64_t nv, const int64_t* off, const double* w, fx_t* k, double* kd, fx_t* twom,
                          double* maxk);

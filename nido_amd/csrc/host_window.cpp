@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "errors.h"
@@ -42,6 +43,38 @@ int window_units(const int64_t* off, const std::vector<int64_t>& parts, int nb, 
     }
     first_unit[(size_t)nb] = (int)units.size() - 1;
     return NIDO_OK;
+}
+
+namespace {
+// splitmix64's output function (Vigna's public-domain splitmix64.c).
+inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+inline uint64_t arc_hash(uint32_t a, uint32_t b, uint64_t wb, uint64_t key)
+{
+    return mix64(mix64((((uint64_t)a << 32) | b) ^ key) ^ wb);
+}
+}  // namespace
+
+bool host_arcs_symmetric(int64_t nv, const int64_t* off, const int* col, const double* w)
+{
+    constexpr uint64_t k1 = 0x6a09e667f3bcc909ull, k2 = 0xbb67ae8584caa73bull;   // sqrt(2), sqrt(3) fractions
+    uint64_t h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : h0, h1, h2, h3)
+    for (int64_t v = 0; v < nv; ++v)
+        for (int64_t e = off[v]; e < off[v + 1]; ++e) {
+            const uint32_t a = (uint32_t)v, b = (uint32_t)col[e];
+            uint64_t wb;
+            std::memcpy(&wb, &w[e], sizeof(wb));
+            h0 += arc_hash(a, b, wb, k1);
+            h1 += arc_hash(b, a, wb, k1);
+            h2 += arc_hash(a, b, wb, k2);
+            h3 += arc_hash(b, a, wb, k2);
+        }
+    return h0 == h1 && h2 == h3;
 }
 
 void host_vertex_weights(int64_t nv, const int64_t* off, const double* w, fx_t* k, double* kd, fx_t* twom,

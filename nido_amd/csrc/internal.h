@@ -253,7 +253,9 @@ void launch_modularity(const int* src, const int* col, const double* w, const in
 // acc[1] and *q.  acc[0..1] must start at zero.
 void launch_modularity_rows(const int64_t* off, const int* col, const double* w, const int* C, int64_t v0,
                             int64_t v1, fx_t* acc, hipStream_t s);
-void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc, double* q, hipStream_t s);
+// term (if not null) receives every community's square term.
+void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc, double* q, hipStream_t s,
+                             fx_t* term = nullptr);
 
 void launch_commit_tracked(int64_t v0, int64_t v1, int b, int64_t* C, int64_t* Cn, int* C32, const fx_t* k,
                            fx_t* a, const int64_t* off, const ModDelta& D, hipStream_t s);

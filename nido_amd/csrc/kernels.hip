@@ -285,13 +285,16 @@ mod_rows_kernel(const int64_t* __restrict__ off, const int* __restrict__ col, co
 }
 
 __global__ void __launch_bounds__(kBlock)
-mod_terms_kernel(const fx_t* __restrict__ a, int64_t nv, double twom_d, fx_t* __restrict__ acc)
+mod_terms_kernel(const fx_t* __restrict__ a, int64_t nv, double twom_d, fx_t* __restrict__ acc,
+                 fx_t* __restrict__ term)
 {
     __shared__ fx_t red[kBlock / 64];
     fx_t T = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nv; i += (int64_t)gridDim.x * kBlock) {
         const double x = fx_to_double(a[i]) / twom_d;
-        T += fx_from_double(x * x);
+        const fx_t t = fx_from_double(x * x);
+        if (term) term[i] = t;
+        T += t;
     }
     T = block_sum_fx(T, red);
     if (threadIdx.x == 0 && T != 0) acc_add(acc + 1, T);
@@ -1006,9 +1009,10 @@ void launch_modularity_rows(const int64_t* off, const int* col, const double* w,
     if (v1 > v0) mod_rows_kernel<<<grid_for(v1 - v0, kBlock / 64, 8192), kBlock, 0, s>>>(off, col, w, C, v0, v1, acc);
 }
 
-void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc, double* q, hipStream_t s)
+void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc, double* q, hipStream_t s,
+                             fx_t* term)
 {
-    if (nv > 0) mod_terms_kernel<<<grid_for(nv, kBlock, kPartBlocks), kBlock, 0, s>>>(a, nv, twom_d, acc);
+    if (nv > 0) mod_terms_kernel<<<grid_for(nv, kBlock, kPartBlocks), kBlock, 0, s>>>(a, nv, twom_d, acc, term);
     mod_acc_finish_kernel<<<1, 64, 0, s>>>(acc, twom_d, q);
 }
 

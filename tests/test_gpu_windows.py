@@ -18,8 +18,10 @@ def _na():
     return na
 
 
-def _run(na, off, col, w, nb, window):
+def _run(na, off, col, w, nb, window, **knobs):
     g = na.GraphGPU((off, col, w), nb, window=window)
+    if knobs:
+        g.set_tuning(**knobs)
     st = na.LouvainGPU(500, 1e-6, nb).run(g, record=True)
     phases = []
     for i in range(st.phases):
@@ -84,3 +86,21 @@ def test_row_longer_than_the_window_is_refused():
     off, col, w = na.gen_rmat(10, 16, 0.57, 0.19, 0.19, True, 2).arrays()
     with pytest.raises(na.NidoError, match="window"):
         na.GraphGPU((off, col, w), 2, window=int(np.diff(off).max()) - 1)
+
+
+@pytest.mark.parametrize("full_modularity", [0, 2])
+def test_windowed_batches_that_fit_update_q_incrementally(full_modularity):
+    """A window that holds every batch at once (but not the graph): each
+    batch's modularity delta is taken right after its commit, from the
+    arcs still in the window, instead of a second pass over all arcs."""
+    import oracle
+    na = _na()
+    off, col, w = na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 6).arrays()
+    parts = oracle.batch_bounds(off, 8)
+    window = int(max(off[parts[b + 1]] - off[parts[b]] for b in range(8)))
+    print(f"case: rmat s12, {len(col)} arcs, window {window} (the largest batch), -b 8, "
+          f"full_modularity {full_modularity}", flush=True)
+    assert window < len(col)
+    ref = run_oracle(off, col, w, 8)
+    got = _run(na, off, col, w, 8, window, full_modularity=full_modularity)
+    assert_runs_equal(ref, got)
