@@ -214,8 +214,11 @@ int nido_graph_create(int64_t nv, int64_t ne, const int64_t* offsets, const int6
  * units of consecutive rows whose arcs fit the window, copies each unit's
  * arcs in before its local-moving launches and sweeps the modularity window
  * by window, and its coarse graph is built on the host; a phase whose arcs
- * fit runs resident.  Results are those of nido_graph_create.  A row longer
- * than the window is refused (NIDO_ERR_CAPACITY) when its phase starts.
+ * fit runs resident.  When every batch fits half the window, a batch's
+ * copy runs (on an internal stream, joined back by events) beside the
+ * previous batch's launches.  Results are those of nido_graph_create.  A row
+ * longer than the window is refused (NIDO_ERR_CAPACITY) when its phase
+ * starts.
  * nido_graph_create picks such a window by itself, sized from the free
  * device memory, when a graph would not fit resident. */
 int nido_graph_create_windowed(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,

@@ -376,6 +376,10 @@ struct nido_graph {
     int64_t window = 0;                  // arcs the device holds at once; 0: every phase resident
     bool windowed = false;               // this phase has more arcs than the window
     bool win_single = false;             // windowed, and every batch's arcs fit the window at once
+    bool win_double = false;             // ... and two batches at once: copies overlap the launches
+    hipEvent_t ev_copied[2] = {nullptr, nullptr};   // half h of the window holds its next batch
+    hipEvent_t ev_free[2] = {nullptr, nullptr};     // half h's last batch is done with it
+    bool win_live = false;               // an iteration is streaming its batches through the halves
     int* h_col32 = nullptr;              // the phase's CSR on the host (pinned) while windowed
     double* h_w = nullptr;
     int64_t h_cap = 0;
@@ -736,6 +740,11 @@ int setup_phase(nido_graph* g)
     g->win_single = g->windowed;
     for (int b = 0; g->win_single && b < g->nb; ++b)
         g->win_single = g->first_unit[(size_t)b + 1] - g->first_unit[(size_t)b] <= 1;
+    // Two batches at once: batch b + 1 is copied into one half of the window
+    // (on the side stream) while batch b's launches read the other.
+    g->win_double = g->win_single;
+    for (int b = 0; g->win_double && b < g->nb; ++b)
+        g->win_double = 2 * (off[(size_t)g->parts[(size_t)b + 1]] - off[(size_t)g->parts[(size_t)b]]) <= g->window;
     const bool dev_lists = !g->windowed && nv >= g_device_lists_min.load();
     if (!g->windowed || g->win_single) {
         // incremental modularity: room for every row chunk of the largest batch
@@ -982,6 +991,31 @@ int window_load(nido_graph* g, int64_t v0, int64_t v1, int64_t* e0)
     return NIDO_OK;
 }
 
+// Offset of the window half that batch b's arcs use while an iteration
+// streams its batches through the two halves.
+int64_t win_half(const nido_graph* g, int b)
+{
+    return (int64_t)(b & 1) * (g->window / 2);
+}
+
+// Copies batch b's arcs into its half of the window on the side stream, once
+// the half's previous batch is done with it; ev_copied marks the copy done.
+int window_prefetch(nido_graph* g, int b)
+{
+    const int h = b & 1;
+    const int64_t a = g->h_off[(size_t)g->parts[(size_t)b]], z = g->h_off[(size_t)g->parts[(size_t)b + 1]];
+    if (2 * (z - a) > g->window) return set_error(NIDO_ERR_CAPACITY, "batch %d does not fit half the window", b);
+    HIP_TRY(hipStreamWaitEvent(g->side, g->ev_free[h], 0));
+    if (z > a) {
+        HIP_TRY(hipMemcpyAsync(g->d_col32 + win_half(g, b), g->h_col32 + a, sizeof(int) * (size_t)(z - a),
+                               hipMemcpyHostToDevice, g->side));
+        HIP_TRY(hipMemcpyAsync(g->d_w + win_half(g, b), g->h_w + a, sizeof(double) * (size_t)(z - a),
+                               hipMemcpyHostToDevice, g->side));
+    }
+    HIP_TRY(hipEventRecord(g->ev_copied[h], g->side));
+    return NIDO_OK;
+}
+
 int do_lm(nido_graph* g, int b)
 {
     g->committed[(size_t)b] = 0;
@@ -1011,11 +1045,16 @@ int do_lm(nido_graph* g, int b)
         // launches, with the arc arrays based so that CSR indices address
         // the window (the unit's rows use arcs [E0, E1) only)
         for (int u = g->first_unit[(size_t)b]; u < g->first_unit[(size_t)b + 1]; ++u) {
-            int64_t e0 = 0;
-            NIDO_TRY(window_load(g, g->units[(size_t)u], g->units[(size_t)u + 1], &e0));
+            int64_t e0 = 0, at = 0;
+            if (g->win_live) {   // the batch (its one unit) was copied into its half
+                e0 = g->h_off[(size_t)g->units[(size_t)u]];
+                at = win_half(g, b);
+            } else {
+                NIDO_TRY(window_load(g, g->units[(size_t)u], g->units[(size_t)u + 1], &e0));
+            }
             LmArgs Aw = A;
-            Aw.col = g->d_col32 - e0;
-            Aw.w = g->d_w - e0;
+            Aw.col = g->d_col32 + at - e0;
+            Aw.w = g->d_w + at - e0;
             NIDO_TRY(lm_launch(g, Aw, u));
         }
         return NIDO_OK;
@@ -1104,14 +1143,31 @@ int iteration_launches(nido_graph* g, bool inc)
     const ModDelta D = g->delta();
     if (g->G > 1) launch_fill_i64(reinterpret_cast<int64_t*>(g->d_moved), 0, kMovedSlots, g->stream);
     g->ride_delta = inc;
+    // A windowed phase whose batches fit half the window each streams them
+    // through the two halves: batch b + 1's copy (side stream) runs beside
+    // batch b's launches (main stream).
+    struct LiveGuard {
+        nido_graph* g;
+        ~LiveGuard() { g->win_live = g->ride_delta = false; }
+    } live_guard{g};
+    g->win_live = g->win_double && g->nb > 0;
+    if (g->win_live) {
+        // the halves are free once the main stream's earlier work (which may
+        // read the window) is done; window_load must copy afresh afterwards
+        HIP_TRY(hipEventRecord(g->ev_free[0], g->stream));
+        HIP_TRY(hipEventRecord(g->ev_free[1], g->stream));
+        g->win_e0 = g->win_e1 = -1;
+        NIDO_TRY(window_prefetch(g, 0));
+    }
     for (int b = 0; b < g->nb; ++b) {
-        const int rc = do_lm(g, b);
-        if (rc != NIDO_OK) {
-            g->ride_delta = false;
-            return rc;
+        if (g->win_live) {
+            if (b + 1 < g->nb) NIDO_TRY(window_prefetch(g, b + 1));
+            HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_copied[b & 1], 0));
         }
+        NIDO_TRY(do_lm(g, b));
         if (!inc) {
             NIDO_TRY(do_commit(g, b));
+            if (g->win_live) HIP_TRY(hipEventRecord(g->ev_free[b & 1], g->stream));
             continue;
         }
         {
@@ -1126,9 +1182,13 @@ int iteration_launches(nido_graph* g, bool inc)
         // A windowed phase (every batch one window load) takes the batch's
         // delta now, while its arcs are still in the window and before the
         // next batch's commit changes any community.
-        if (g->windowed && g->parts[(size_t)b + 1] > g->parts[(size_t)b])
+        if (g->windowed && g->parts[(size_t)b + 1] > g->parts[(size_t)b]) {
+            const int64_t at = g->win_live ? win_half(g, b) : 0;
+            const int64_t e0 = g->win_live ? g->h_off[(size_t)g->parts[(size_t)b]] : g->win_e0;
             launch_mod_delta(b, g->parts[(size_t)b], g->parts[(size_t)b + 1], g->d_Cn, g->d_C32,
-                             g->d_col32 - g->win_e0, g->d_w - g->win_e0, g->d_a, g->twom_d, D, g->stream);
+                             g->d_col32 + at - e0, g->d_w + at - e0, g->d_a, g->twom_d, D, g->stream);
+        }
+        if (g->win_live) HIP_TRY(hipEventRecord(g->ev_free[b & 1], g->stream));
         g->committed[(size_t)b] = 1;
     }
     g->ride_delta = false;
@@ -1848,7 +1908,11 @@ int create_impl(int64_t nv, const int64_t* off, const int64_t* col, const double
     }
     if (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_copied[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_copied[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_free[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_free[1], hipEventDisableTiming) != hipSuccess)
         return fail(set_error(NIDO_ERR_HIP, "creating the side stream or its events failed"));
     AllocScope scope(g->stream);
     g->n0 = nv;
@@ -2122,6 +2186,10 @@ int nido_graph_destroy(nido_graph* g)
     if (g->h_w) (void)hipHostFree(g->h_w);
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+    for (int h = 0; h < 2; ++h) {
+        if (g->ev_copied[h]) (void)hipEventDestroy(g->ev_copied[h]);
+        if (g->ev_free[h]) (void)hipEventDestroy(g->ev_free[h]);
+    }
     if (g->side) (void)hipStreamDestroy(g->side);
     if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
     delete g;

@@ -88,17 +88,21 @@ def test_row_longer_than_the_window_is_refused():
         na.GraphGPU((off, col, w), 2, window=int(np.diff(off).max()) - 1)
 
 
+@pytest.mark.parametrize("fit", [1, 2, 3])
 @pytest.mark.parametrize("full_modularity", [0, 2])
-def test_windowed_batches_that_fit_update_q_incrementally(full_modularity):
+def test_windowed_batches_that_fit_update_q_incrementally(full_modularity, fit):
     """A window that holds every batch at once (but not the graph): each
     batch's modularity delta is taken right after its commit, from the
-    arcs still in the window, instead of a second pass over all arcs."""
+    arcs still in the window, instead of a second pass over all arcs.
+    With room for two batches (fit 2, and fit 3 for an odd window) the
+    next batch is copied into the other half while the current one runs."""
     import oracle
     na = _na()
     off, col, w = na.gen_rmat(12, 16, 0.57, 0.19, 0.19, True, 6).arrays()
     parts = oracle.batch_bounds(off, 8)
-    window = int(max(off[parts[b + 1]] - off[parts[b]] for b in range(8)))
-    print(f"case: rmat s12, {len(col)} arcs, window {window} (the largest batch), -b 8, "
+    largest = int(max(off[parts[b + 1]] - off[parts[b]] for b in range(8)))
+    window = {1: largest, 2: 2 * largest, 3: 2 * largest + 1}[fit]
+    print(f"case: rmat s12, {len(col)} arcs, window {window} ({fit} x the largest batch), -b 8, "
           f"full_modularity {full_modularity}", flush=True)
     assert window < len(col)
     ref = run_oracle(off, col, w, 8)
