@@ -140,7 +140,9 @@ int nido_set_list_rebuild(int percent);
 int nido_hgraph_read(const char* path, nido_hgraph** out);
 /* Writes the same format (the converter in Vite that the README points at). */
 int nido_hgraph_write(const nido_hgraph* g, const char* path);
-/* Copies a caller CSR into a host graph. */
+/* Copies a caller CSR into a host graph: arcs [offsets[0], offsets[nv]) of
+ * cols/weights, which hold ne entries from offsets[0] on (offsets spanning
+ * more than ne arcs, or decreasing, are refused with NIDO_ERR_INVALID). */
 int nido_hgraph_from_csr(int64_t nv, int64_t ne, const int64_t* offsets, const int64_t* cols,
                          const double* weights, nido_hgraph** out);
 /* get_num_vertices / get_num_edges (graph.hpp:72-73). */

@@ -157,8 +157,14 @@ int nido_hgraph_from_csr(int64_t nv, int64_t ne, const int64_t* off, const int64
         return set_error(NIDO_ERR_INVALID, "invalid CSR arguments");
     const int64_t base = off[0];
     const int64_t arcs = off[nv] - base;
-    if (arcs < 0) return set_error(NIDO_ERR_INVALID, "offsets decrease");
-    (void)ne;
+    if (base < 0 || arcs < 0) return set_error(NIDO_ERR_INVALID, "offsets negative or decreasing");
+    // cols/weights hold ne entries from offsets[0] on: offsets that claim
+    // more arcs would read past them
+    if (arcs > ne)
+        return set_error(NIDO_ERR_INVALID, "offsets span %lld arcs, more than the %lld given", (long long)arcs,
+                         (long long)ne);
+    rc = nido::check_offsets(nv, off, col, w);
+    if (rc) return rc;
     try {
         nido_hgraph* g = new nido_hgraph;
         g->nv = nv;
