@@ -935,7 +935,7 @@ int windowed_modularity(nido_graph* g)
         int64_t e0 = 0;
         NIDO_TRY(window_load(g, g->units[u], g->units[u + 1], &e0));
         launch_modularity_rows(g->d_off, g->d_col32 - e0, g->d_w - e0, g->d_C32, g->units[u], g->units[u + 1],
-                               g->d_acc, g->stream);
+                               g->h_off[(size_t)g->units[u + 1]] - e0, g->d_acc, g->stream);
     }
     launch_modularity_terms(g->d_a, g->nv, g->twom_d, g->d_acc, g->d_q, g->stream, g->d_term);
     HIP_TRY(hipGetLastError());

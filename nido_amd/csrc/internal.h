@@ -250,9 +250,10 @@ void launch_modularity(const int* src, const int* col, const double* w, const in
 // Modularity of a phase whose arcs stream through a window: per window,
 // the internal weight of rows [v0, v1) added into acc[0] (col / w based so
 // that CSR arc indices address the window); then the square terms into
-// acc[1] and *q.  acc[0..1] must start at zero.
+// acc[1] and *q.  acc[0..1] must start at zero.  arcs = off[v1] - off[v0]
+// (sizes the grid).
 void launch_modularity_rows(const int64_t* off, const int* col, const double* w, const int* C, int64_t v0,
-                            int64_t v1, fx_t* acc, hipStream_t s);
+                            int64_t v1, int64_t arcs, fx_t* acc, hipStream_t s);
 // term (if not null) receives every community's square term.
 void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc, double* q, hipStream_t s,
                              fx_t* term = nullptr);

@@ -264,21 +264,41 @@ mod_finalize_kernel(const fx_t* __restrict__ part, int nblocks, double twom_d, d
 
 // Modularity of a phase whose arcs stream through a window: the internal
 // weight of rows [v0, v1) (col / w may be based so that the window's arcs
-// keep their CSR indices), one wavefront per row, added exactly into
-// acc[0]; then every community's square term into acc[1]; then Q.
+// keep their CSR indices), added exactly into acc[0]; then every
+// community's square term into acc[1]; then Q.  Each thread takes runs of
+// kModRun consecutive arcs of [off[v0], off[v1]): a binary search over
+// off[v0 .. v1 - 1] finds the row of a run's first arc, then the run walks
+// the rows forward, so a long row is spread over many threads and short
+// rows leave no lane idle.
+constexpr int kModRun = 16;
+
 __global__ void __launch_bounds__(kBlock)
 mod_rows_kernel(const int64_t* __restrict__ off, const int* __restrict__ col, const double* __restrict__ w,
                 const int* __restrict__ C, int64_t v0, int64_t v1, fx_t* __restrict__ acc)
 {
     __shared__ fx_t red[kBlock / 64];
-    const int lane = threadIdx.x & 63;
-    const int64_t wid = (blockIdx.x * (int64_t)kBlock + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+    const int64_t e0 = off[v0], e1 = off[v1];
     fx_t I = 0;
-    for (int64_t v = v0 + wid; v < v1; v += nw) {
-        const int cv = C[v];
-        for (int64_t e = off[v] + lane; e < off[v + 1]; e += 64)
+    for (int64_t a = e0 + (blockIdx.x * (int64_t)kBlock + threadIdx.x) * kModRun; a < e1;
+         a += (int64_t)gridDim.x * kBlock * kModRun) {
+        // the last row r of [v0, v1) with off[r] <= a: off[v0] <= a < off[v1]
+        int64_t lo = v0, hi = v1 - 1;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo + 1) / 2;
+            if (off[mid] <= a) lo = mid;
+            else hi = mid - 1;
+        }
+        int64_t r = lo, rend = off[r + 1];
+        int cv = C[r];
+        const int64_t z = a + kModRun < e1 ? a + kModRun : e1;
+        for (int64_t e = a; e < z; ++e) {
+            while (rend <= e) {   // e < off[v1]: r stays below v1
+                ++r;
+                rend = off[r + 1];
+                cv = C[r];
+            }
             if (C[col[e]] == cv) I += fx_from_double(w[e]);   // self arcs are internal
+        }
     }
     I = block_sum_fx(I, red);
     if (threadIdx.x == 0 && I != 0) acc_add(acc, I);
@@ -1004,9 +1024,11 @@ void launch_modularity(const int* src, const int* col, const double* w, const in
 }
 
 void launch_modularity_rows(const int64_t* off, const int* col, const double* w, const int* C, int64_t v0,
-                            int64_t v1, fx_t* acc, hipStream_t s)
+                            int64_t v1, int64_t arcs, fx_t* acc, hipStream_t s)
 {
-    if (v1 > v0) mod_rows_kernel<<<grid_for(v1 - v0, kBlock / 64, 8192), kBlock, 0, s>>>(off, col, w, C, v0, v1, acc);
+    if (v1 > v0 && arcs > 0)
+        mod_rows_kernel<<<grid_for(arcs, (int64_t)kBlock * kModRun, 8192), kBlock, 0, s>>>(off, col, w, C, v0, v1,
+                                                                                            acc);
 }
 
 void launch_modularity_terms(const fx_t* a, int64_t nv, double twom_d, fx_t* acc, double* q, hipStream_t s,
