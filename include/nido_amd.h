@@ -86,7 +86,9 @@ typedef struct {
                                  the exact incremental update over the moved rows after each phase's first
                                  iteration, for a handle of several graphs only when the iteration before
                                  moved at most 1/8 of the arcs (else a sweep of every arc); 1 always the
-                                 sweep; 2 the update after each phase's first iteration (0/1/2) */
+                                 sweep; 2 the update after each phase's first iteration (0/1/2).  With 0
+                                 or 2, a windowed phase whose batches each fit the window takes the
+                                 update from its first iteration on */
     int32_t fixed_short;      /* 1 (default): rows up to big_min_deg arcs sum in 64-bit fixed point when every
                                  weight is a multiple of one power of two 2^-F and those rows' sums stay below
                                  2^63 in units of 2^-F (the same exact sums); 0: always Q64.64 (0/1) */

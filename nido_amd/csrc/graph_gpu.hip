@@ -1254,10 +1254,13 @@ int run_iteration_launches(nido_graph* g)
     // moved at most 1/kIncShare of the arcs: its batches are all short rows,
     // so the deltas' dependent gathers set the length of those launches and
     // cost several times the sweep's streaming per arc.  2: the update in
-    // every later iteration; 1: never.
+    // every later iteration; 1: never.  A windowed phase whose batches each
+    // fit the window takes the update from its first iteration on: there a
+    // sweep would stream every arc over the host link a second time, while
+    // each batch's delta reads arcs already in the window.
     const bool any_w = g->G > 1 || g->twom != 0;
     const int fm = g->tuning.full_modularity;
-    bool inc = any_w && g->symmetric && g->inc_ok && fm != 1 && g->iters_since_singleton > 0 &&
+    bool inc = any_w && g->symmetric && g->inc_ok && fm != 1 && (g->iters_since_singleton > 0 || g->windowed) &&
                (!g->windowed || g->win_single);
     if (inc && fm == 0 && g->G > 1) inc = g->last_moved >= 0 && g->last_moved * kIncShare <= g->ne;
     // the short-row launches must carry the delta buffers' current addresses
